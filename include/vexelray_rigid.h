@@ -1,0 +1,116 @@
+/*
+ * vexelray_rigid.h — the C ABI of the MI355X rigid-body step.
+ *
+ * One opaque world per simulation, owned by the caller from vr_world_create to vr_world_destroy. Every call returns
+ * an int status: VR_OK (0), or a negative code whose message vr_last_error() copies out. Host arrays are copied in
+ * and out; no host pointer is kept past the call. A world is not thread-safe: one thread owns it after it is made,
+ * and all of its device work runs on one HIP stream of its own.
+ *
+ * Each entry point replaces part of the reference's Java world/body/step API (paths relative to the
+ * vexelray-sim-rigid repository):
+ *
+ *   vr_world_create   SphereStep(spheres, substeps, iterations, grid, solve, rounds, contactsPerSphere)
+ *                     (core/.../sphere/SphereStep.java:115-209) plus the runner it is registered on
+ *                     (gui/.../SphereSimulation.java:70-97 -> PassRunner.gpu(...)); SphereGrid(nx, ny, nz, cell)
+ *                     (core/.../sphere/SphereGrid.java:11-19).
+ *   vr_world_clear    PassRunner.clear(), then SphereStep.constants() written (SphereSimulation.java:154-166,
+ *                     SphereStep.java:157-160, 224-226, 419).
+ *   vr_world_write    PassRunner.write(name, int[]|float[]) (ctest/.../sphere/Scene.java:127-140).
+ *   vr_world_read     PassRunner.floats(name) / read(name) (Scene.java:176-204; SphereSimulation.state() :250-254).
+ *   vr_world_params   SphereSimulation.params(int[]) (SphereSimulation.java:182-185): 13 words of raw float bits
+ *                     laid out as Spheres.params (core/.../sphere/Spheres.java:173-226).
+ *   vr_world_friction SphereStepper.friction(boolean) (core/.../sphere/SphereStepper.java:110-112).
+ *   vr_world_step     SphereStepper.step(Runner) -> Report (SphereStepper.java:66-103) as driven by
+ *                     SphereSimulation.step()/stepAndWait() (SphereSimulation.java:190-229).
+ *   vr_world_finish   SphereSimulation.finish() / PassRunner.finish() (SphereSimulation.java:243-245).
+ *   vr_world_destroy  SphereSimulation.close() (SphereSimulation.java:287-295).
+ *   vr_last_error     the exception message a Java caller would have seen (IllegalArgumentException for
+ *                     validation, IllegalStateException for a pass that never ends or a wrong mode).
+ */
+#ifndef VEXELRAY_RIGID_H
+#define VEXELRAY_RIGID_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vr_world vr_world;
+
+/* Status codes. */
+#define VR_OK 0
+#define VR_ERR_ARGUMENT (-1)  /* IllegalArgumentException in the reference */
+#define VR_ERR_STATE (-2)     /* IllegalStateException in the reference */
+#define VR_ERR_DEVICE (-3)    /* a HIP call failed, or no device */
+#define VR_ERR_NAME (-4)      /* no buffer of that name, or the wrong length */
+
+/* World kinds. */
+#define VR_SPHERES 1
+
+/* SphereStep.Solve (SphereStep.java:37-43). */
+#define VR_JACOBI 0
+#define VR_GAUSS_SEIDEL_BY_CONTACT 1
+
+/* Element types of the named buffers, as vr_world_buffer_info reports them. */
+#define VR_F32 1
+#define VR_I32 2
+
+/*
+ * What a step is made of: SphereStep's constructor arguments (SphereStep.java:115-120). has_grid = 0 is the
+ * reference's null grid (every pair). fixed_rounds = 1 runs Gauss-Seidel as the one list SphereStep.step() builds,
+ * `rounds` rounds a pass, instead of until done (ctest/.../sphere/Scene.java:69-75, 158-161).
+ */
+typedef struct vr_step_desc {
+    int32_t bodies;
+    int32_t substeps;
+    int32_t iterations;
+    int32_t has_grid;
+    int32_t nx, ny, nz;
+    float cell;
+    int32_t solve;
+    int32_t rounds;
+    int32_t contacts_per_sphere;
+    int32_t fixed_rounds;
+    /*
+     * 0: the reference's contact priority, (a*n + o)*0x9E3779B1 + g*0x632BE5AB kept to 31 bits, and its limit of
+     * 46,340 spheres for Gauss-Seidel (Spheres.java:783-787, 1166-1168; SphereStep.java:128-131).
+     * 1: the same formula in 63-bit arithmetic with 64-bit odd multipliers, one to one over every pair of up to
+     * 2^31 spheres, so Gauss-Seidel takes any sphere count. Identical rounds wherever both apply is not claimed:
+     * the priorities differ, so the colouring does.
+     */
+    int32_t wide_pairs;
+} vr_step_desc;
+
+/*
+ * SphereStepper.Report (SphereStepper.java:42) and SphereSimulation.StepReport's timings (SphereSimulation.java:48).
+ * `waits` counts the host waits the reference's stepper makes for the same step (one per report it reads), so the
+ * two reports compare; this build decides "done" on the device and waits for none of them.
+ */
+typedef struct vr_step_report {
+    int32_t waits;
+    int32_t rounds;
+    int32_t most_rounds;
+    int32_t unlisted;
+    int64_t wall_ns;
+    int64_t gpu_ns;
+} vr_step_report;
+
+int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out);
+int vr_world_clear(vr_world *w);
+int vr_world_write(vr_world *w, const char *name, const void *src, int64_t count);
+int vr_world_read(vr_world *w, const char *name, void *dst, int64_t count);
+/* Length and element type (VR_F32 / VR_I32) of a named buffer. */
+int vr_world_buffer_info(vr_world *w, const char *name, int64_t *count, int32_t *dtype);
+int vr_world_params(vr_world *w, const int32_t *words, int32_t n);
+int vr_world_friction(vr_world *w, int32_t on);
+int vr_world_step(vr_world *w, int32_t wait, vr_step_report *report);
+int vr_world_finish(vr_world *w);
+int vr_world_destroy(vr_world *w);
+int vr_last_error(char *buf, int32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,171 @@
+"""ctypes binding of the C ABI declared in include/vexelray_rigid.h.
+
+``product()`` loads the HIP library built beside this package (``libvexelray_rigid.so``). There is no fallback: if
+the library is missing or cannot load, this raises.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PRODUCT_LIB = os.path.join(HERE, "libvexelray_rigid.so")
+
+VR_OK = 0
+VR_ERR_ARGUMENT = -1
+VR_ERR_STATE = -2
+VR_ERR_DEVICE = -3
+VR_ERR_NAME = -4
+VR_SPHERES = 1
+VR_F32 = 1
+VR_I32 = 2
+
+# Every entry point the header declares, for the export check.
+ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "world_buffer_info", "world_params",
+                "world_friction", "world_step", "world_finish", "world_destroy", "last_error")
+
+
+class StepDesc(ctypes.Structure):
+    _fields_ = [("bodies", ctypes.c_int32), ("substeps", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("has_grid", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+                ("nz", ctypes.c_int32), ("cell", ctypes.c_float), ("solve", ctypes.c_int32),
+                ("rounds", ctypes.c_int32), ("contacts_per_sphere", ctypes.c_int32),
+                ("fixed_rounds", ctypes.c_int32), ("wide_pairs", ctypes.c_int32)]
+
+
+class StepReport(ctypes.Structure):
+    _fields_ = [("waits", ctypes.c_int32), ("rounds", ctypes.c_int32), ("most_rounds", ctypes.c_int32),
+                ("unlisted", ctypes.c_int32), ("wall_ns", ctypes.c_int64), ("gpu_ns", ctypes.c_int64)]
+
+
+class Abi:
+    """One library exposing the ABI, its symbols named ``prefix + entry``."""
+
+    def __init__(self, path: str, prefix: str, where: str):
+        if not os.path.exists(path):
+            raise OSError(f"{path} is not built; run __graft_entry__.build() first")
+        self.path = path
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+        self._where = where
+        p = ctypes.c_void_p
+        sig = {
+            "world_create": ([ctypes.c_int32, ctypes.POINTER(StepDesc), ctypes.POINTER(p)]),
+            "world_clear": ([p]),
+            "world_write": ([p, ctypes.c_char_p, p, ctypes.c_int64]),
+            "world_read": ([p, ctypes.c_char_p, p, ctypes.c_int64]),
+            "world_buffer_info": ([p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
+                                   ctypes.POINTER(ctypes.c_int32)]),
+            "world_params": ([p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
+            "world_friction": ([p, ctypes.c_int32]),
+            "world_step": ([p, ctypes.c_int32, ctypes.POINTER(StepReport)]),
+            "world_finish": ([p]),
+            "world_destroy": ([p]),
+            "last_error": ([ctypes.c_char_p, ctypes.c_int32]),
+        }
+        for name, args in sig.items():
+            fn = getattr(self.lib, prefix + name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+
+    def fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def last_error(self) -> str:
+        buf = ctypes.create_string_buffer(1024)
+        self.fn("last_error")(buf, 1024)
+        return buf.value.decode("utf-8", "replace")
+
+    def where(self) -> str:
+        return self._where
+
+    def create(self, kind, desc, arg_error, state_error) -> "World":
+        handle = ctypes.c_void_p()
+        rc = self.fn("world_create")(kind, ctypes.byref(desc), ctypes.byref(handle))
+        World.check(self, rc, arg_error, state_error)
+        return World(self, handle, arg_error, state_error)
+
+
+class World:
+    def __init__(self, abi: Abi, handle, arg_error, state_error):
+        self.abi = abi
+        self.handle = handle
+        self.arg_error = arg_error
+        self.state_error = state_error
+
+    @staticmethod
+    def check(abi, rc, arg_error=ValueError, state_error=RuntimeError):
+        if rc == VR_OK:
+            return
+        msg = abi.last_error()
+        if rc == VR_ERR_ARGUMENT:
+            raise arg_error(msg)
+        if rc == VR_ERR_STATE:
+            raise state_error(msg)
+        if rc == VR_ERR_NAME:
+            raise KeyError(msg)
+        raise RuntimeError(f"device error: {msg}")
+
+    def _check(self, rc):
+        World.check(self.abi, rc, self.arg_error, self.state_error)
+
+    def info(self, name: str):
+        count = ctypes.c_int64()
+        dtype = ctypes.c_int32()
+        self._check(self.abi.fn("world_buffer_info")(self.handle, name.encode(), ctypes.byref(count),
+                                                     ctypes.byref(dtype)))
+        return count.value, dtype.value
+
+    def write(self, name: str, values) -> None:
+        count, dtype = self.info(name)
+        arr = np.ascontiguousarray(values, dtype=np.float32 if dtype == VR_F32 else np.int32)
+        if arr.dtype == np.float32 and dtype == VR_I32:
+            arr = arr.view(np.int32)
+        arr = np.ascontiguousarray(arr.reshape(-1))
+        if arr.size > count:
+            raise KeyError(f"buffer {name} holds {count} words, not {arr.size}")
+        self._check(self.abi.fn("world_write")(self.handle, name.encode(), arr.ctypes.data_as(ctypes.c_void_p),
+                                               arr.size))
+
+    def read(self, name: str, as_dtype) -> np.ndarray:
+        count, dtype = self.info(name)
+        arr = np.empty(count, dtype=np.float32 if dtype == VR_F32 else np.int32)
+        self._check(self.abi.fn("world_read")(self.handle, name.encode(), arr.ctypes.data_as(ctypes.c_void_p),
+                                              count))
+        return arr.view(as_dtype) if arr.dtype != as_dtype else arr
+
+    def clear(self) -> None:
+        self._check(self.abi.fn("world_clear")(self.handle))
+
+    def params(self, words: np.ndarray) -> None:
+        words = np.ascontiguousarray(words, dtype=np.int32)
+        self._check(self.abi.fn("world_params")(self.handle, words.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                words.size))
+
+    def friction(self, on: bool) -> None:
+        self._check(self.abi.fn("world_friction")(self.handle, 1 if on else 0))
+
+    def step(self, wait: bool) -> StepReport:
+        rep = StepReport()
+        self._check(self.abi.fn("world_step")(self.handle, 1 if wait else 0, ctypes.byref(rep)))
+        return rep
+
+    def finish(self) -> None:
+        self._check(self.abi.fn("world_finish")(self.handle))
+
+    def destroy(self) -> None:
+        if self.handle:
+            self.abi.fn("world_destroy")(self.handle)
+            self.handle = None
+
+
+_PRODUCT = None
+
+
+def product() -> Abi:
+    """The HIP library. Raises if it is not built."""
+    global _PRODUCT
+    if _PRODUCT is None:
+        _PRODUCT = Abi(PRODUCT_LIB, "vr_", "MI355X (HIP, gfx950)")
+    return _PRODUCT

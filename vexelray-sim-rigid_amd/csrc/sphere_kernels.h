@@ -1,0 +1,77 @@
+// sphere_kernels.h — device-side layout of a sphere world and the launchers of its kernels.
+//
+// State is structure-of-arrays in HBM: one f32 array per per-sphere field (x, y, z, u, v, w, r, im, cx..cz, dx..dz,
+// ax..az, tx..tz, qx..qw), so every per-sphere pass reads and writes whole cache lines with coalesced loads.
+// The contact list is compressed rows by lower sphere: sphere a's contacts occupy slots off[a] .. off[a]+cnt[a]-1,
+// in the order the 27-cell walk meets them. Two such lists are kept, the substep's and the one before it, and swap.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace vr {
+
+enum { H = 0, GX, GY, GZ, SX, SY, SZ, OMEGA, AVERAGED, RESTITUTION, MU_STATIC, MU_KINETIC, MU_ROLLING, PARAM_COUNT };
+enum { LISTED = 0, MISSED = 1, LONGEST = 2 };
+constexpr int IMPULSE_WORDS = 7;
+constexpr int WALL_IMPULSE_WORDS = 21;
+constexpr int SHOWN_STRIDE = 40;
+constexpr int MOST_KEPT = 32;
+
+// One contact list: compressed rows by lower sphere a.
+struct ContactList {
+    int32_t *a, *b, *done;  // cap
+    float *impulse;         // 7 * cap
+    int32_t *off, *cnt;     // n + 1, n: sphere a's first slot and how many it found (fitting or not)
+};
+
+struct SphereDev {
+    int n, cap, nx, ny, nz, cells;
+    float cell;
+    int wide;
+    const float *params;
+    float *x, *y, *z, *u, *v, *w, *r, *im, *cx, *cy, *cz, *dx, *dy, *dz, *ax, *ay, *az, *tx, *ty, *tz, *qx, *qy,
+        *qz, *qw;
+    float *shown;
+    int32_t *keys, *order, *starts;
+    long long *claims[3];
+    ContactList cur, kept;
+    int32_t *count;  // contactCount: LISTED, MISSED, LONGEST
+    int32_t *open;   // two words
+    float *wallImpulse;
+};
+
+// A round's constants (Spheres.java:1069-1075) and the claim array it claims into (checked = slot+2, cleared =
+// slot+1, all mod 3).
+struct RoundConst {
+    int checks, claims, pass, dispatch, mark, checksDispatch, rubs, slot;
+};
+
+struct SortScratch {
+    int32_t *keysA, *valsA, *keysB, *hist, *scanTmp;
+    int maxBlocks;
+};
+
+// Launchers (all asynchronous on `s`).
+void launch_predict(const SphereDev &D, hipStream_t s);
+void launch_bin(const SphereDev &D, hipStream_t s);
+void launch_sort(const SphereDev &D, const SortScratch &S, hipStream_t s);
+void launch_warm_walls(const SphereDev &D, hipStream_t s);
+void launch_clear_contacts(const SphereDev &D, hipStream_t s);
+void launch_list(const SphereDev &D, const SortScratch &S, hipStream_t s);
+void launch_round(const SphereDev &D, const RoundConst &rc, hipStream_t s);
+void launch_walls(const SphereDev &D, hipStream_t s);
+void launch_velocity(const SphereDev &D, hipStream_t s);
+void launch_bounce(const SphereDev &D, bool grid, hipStream_t s);
+void launch_bounced(const SphereDev &D, hipStream_t s);
+void launch_jacobi_solve(const SphereDev &D, bool grid, hipStream_t s);
+void launch_jacobi_apply(const SphereDev &D, hipStream_t s);
+void launch_show(const SphereDev &D, bool grid, hipStream_t s);
+void launch_fill(float *p, float value, int64_t count, hipStream_t s);
+void launch_fill_i32(int32_t *p, int32_t value, int64_t count, hipStream_t s);
+
+// Exclusive scan of `n` ints from `in` to `out` (may alias); the total goes to *total if not null.
+void scan_exclusive(const int32_t *in, int32_t *out, int n, int32_t *total, int32_t *tmp, hipStream_t s);
+int scan_tmp_words(int n);
+
+}  // namespace vr

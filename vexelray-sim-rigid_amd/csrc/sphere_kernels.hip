@@ -1,0 +1,1055 @@
+// sphere_kernels.hip — the sphere step's kernels for gfx950.
+//
+// One kernel per pass of the reference's sphere step (vexelray-sim-rigid-core/.../sphere/Spheres.java), over
+// structure-of-arrays state. f32 arithmetic follows the reference's expression trees operation by operation and is
+// compiled with -ffp-contract=off, so no multiply-add is fused; division and square root are IEEE-rounded (hipcc's
+// default). Nothing here depends on the order in which threads run: contacts solved in one round share no sphere,
+// claims are integer atomic maxima, and every sum over neighbours runs in one thread in a fixed order.
+#include "sphere_kernels.h"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace vr {
+
+namespace {
+
+constexpr int THREADS = 256;
+constexpr float COINCIDENT = 1e-12f;   // Spheres.java:156
+constexpr float LIST_MARGIN = 0.02f;   // Spheres.java:795
+constexpr float INERTIA = 0.4f;        // Spheres.java:246
+constexpr float MARGIN1 = 1.0f + LIST_MARGIN;  // f(1 + LIST_MARGIN)
+
+inline int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, (n + THREADS - 1) / THREADS); }
+
+inline void check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+__device__ __forceinline__ float fmax_(float a, float b) { return a < b ? b : a; }  // std::max
+__device__ __forceinline__ float fmin_(float a, float b) { return b < a ? b : a; }  // std::min
+
+__device__ __forceinline__ float dot3(const float *x, const float *y) {
+    return x[0] * y[0] + (x[1] * y[1] + x[2] * y[2]);
+}
+
+__device__ __forceinline__ void cross3(const float *x, const float *y, float *z) {
+    z[0] = x[1] * y[2] - x[2] * y[1];
+    z[1] = x[2] * y[0] - x[0] * y[2];
+    z[2] = x[0] * y[1] - x[1] * y[0];
+}
+
+// A sphere's position, moves and turns, loaded once and stored once (the reference's Held, Spheres.java:532-569).
+struct Held {
+    int s;
+    float at[3], displaced[3], turned[3];
+};
+
+__device__ __forceinline__ Held hold(const SphereDev &D, int s) {
+    Held h;
+    h.s = s;
+    h.at[0] = D.x[s]; h.at[1] = D.y[s]; h.at[2] = D.z[s];
+    h.displaced[0] = D.dx[s]; h.displaced[1] = D.dy[s]; h.displaced[2] = D.dz[s];
+    h.turned[0] = D.tx[s]; h.turned[1] = D.ty[s]; h.turned[2] = D.tz[s];
+    return h;
+}
+
+__device__ __forceinline__ void store(const SphereDev &D, const Held &h, bool turns) {
+    D.x[h.s] = h.at[0]; D.y[h.s] = h.at[1]; D.z[h.s] = h.at[2];
+    D.dx[h.s] = h.displaced[0]; D.dy[h.s] = h.displaced[1]; D.dz[h.s] = h.displaced[2];
+    if (turns) { D.tx[h.s] = h.turned[0]; D.ty[h.s] = h.turned[1]; D.tz[h.s] = h.turned[2]; }
+}
+
+struct Impulse {  // Spheres.java:490-522
+    float normal, p[3], q[3];
+};
+
+__device__ __forceinline__ Impulse load_impulse(const float *b) {
+    Impulse t;
+    t.normal = b[0];
+    for (int k = 0; k < 3; k++) { t.p[k] = b[1 + k]; t.q[k] = b[4 + k]; }
+    return t;
+}
+
+__device__ __forceinline__ void store_impulse(float *b, const Impulse &t) {
+    b[0] = t.normal;
+    for (int k = 0; k < 3; k++) { b[1 + k] = t.p[k]; b[4 + k] = t.q[k]; }
+}
+
+__device__ __forceinline__ void clear_impulse(Impulse &t) {
+    t.normal = 0;
+    for (int k = 0; k < 3; k++) { t.p[k] = 0; t.q[k] = 0; }
+}
+
+__device__ __forceinline__ float turns_per_impulse(float w, float lever, float radius) {  // Spheres.java:743-745
+    return (w * lever) / (INERTIA * (radius * radius));
+}
+
+__device__ __forceinline__ void spin_of(const SphereDev &D, int s, float out[3]) {
+    out[0] = D.ax[s]; out[1] = D.ay[s]; out[2] = D.az[s];
+}
+
+__device__ __forceinline__ void vel_of(const SphereDev &D, int s, float out[3]) {
+    out[0] = D.u[s]; out[1] = D.v[s]; out[2] = D.w[s];
+}
+
+// Spheres.rub, Spheres.java:662-677.
+__device__ __forceinline__ void rub(Held &a, float wa, float turnsA, Held *o, float wo, float turnsO,
+                                    const float n[3], const float p[3]) {
+    float np[3];
+    cross3(n, p, np);
+    for (int k = 0; k < 3; k++) {
+        float move = wa * p[k];
+        a.at[k] = a.at[k] + move;
+        a.displaced[k] = a.displaced[k] + move;
+        a.turned[k] = a.turned[k] - turnsA * np[k];
+        if (o) {
+            float back = wo * p[k];
+            o->at[k] = o->at[k] - back;
+            o->displaced[k] = o->displaced[k] - back;
+            o->turned[k] = o->turned[k] - turnsO * np[k];
+        }
+    }
+}
+
+// Spheres.contactTravel, Spheres.java:751-765.
+__device__ __forceinline__ void contact_travel(const SphereDev &D, const Held &s, float h, const float n[3],
+                                               float lever, int side, float travel[3]) {
+    float spin[3], vel[3], turn[3], turnN[3];
+    spin_of(D, s.s, spin);
+    vel_of(D, s.s, vel);
+    for (int k = 0; k < 3; k++) turn[k] = h * spin[k] + s.turned[k];
+    cross3(turn, n, turnN);
+    for (int k = 0; k < 3; k++) {
+        float centre = h * vel[k] + s.displaced[k];
+        float spun = lever * turnN[k];
+        travel[k] = side > 0 ? centre + spun : centre - spun;
+    }
+}
+
+// Spheres.friction, Spheres.java:595-654; o == nullptr for a wall.
+__device__ void friction_at(const SphereDev &D, const float *P, Held &a, float wa, float leverA, Held *o, float wo,
+                            float leverO, const float n[3], Impulse &taken) {
+    float muS = P[MU_STATIC], muK = P[MU_KINETIC];
+    if (!(muS + muK > 0)) return;
+    float h = P[H];
+    float la = leverA;
+    float slip[3];
+    contact_travel(D, a, h, n, la, -1, slip);
+    float ia = wa;
+    float leverOverIa = turns_per_impulse(ia, la, D.r[a.s]);
+    float weight = ia + leverOverIa * la;
+    float io = 0, lo = 0, leverOverIo = 0;
+    if (o) {
+        io = wo;
+        lo = leverO;
+        float other[3];
+        contact_travel(D, *o, h, n, lo, 1, other);
+        for (int k = 0; k < 3; k++) slip[k] = slip[k] - other[k];
+        leverOverIo = turns_per_impulse(io, lo, D.r[o->s]);
+        weight = weight + (io + leverOverIo * lo);
+    }
+    float along = dot3(slip, n);
+    float tangent[3];
+    for (int k = 0; k < 3; k++) tangent[k] = slip[k] - along * n[k];
+    if (!(weight > 0)) return;
+    float takenAlong = dot3(taken.p, n);
+    float before[3], whole[3];
+    for (int k = 0; k < 3; k++) {
+        before[k] = taken.p[k] - takenAlong * n[k];
+        whole[k] = before[k] - tangent[k] / weight;
+    }
+    float size2 = dot3(whole, whole);
+    float lambdaN = taken.normal;
+    float holds = muS * lambdaN;
+    if (size2 > holds * holds) {
+        float cut = fmin_(1.0f, (muK * lambdaN) / sqrtf(size2));
+        for (int k = 0; k < 3; k++) whole[k] = whole[k] * cut;
+    }
+    float p[3];
+    for (int k = 0; k < 3; k++) {
+        p[k] = whole[k] - before[k];
+        taken.p[k] = whole[k];
+    }
+    rub(a, ia, leverOverIa, o, io, leverOverIo, n, p);
+}
+
+// Spheres.roll, Spheres.java:699-741; o == nullptr for a wall.
+__device__ void roll(const SphereDev &D, const float *P, Held &a, float wa, Held *o, float wo, float reach,
+                     Impulse &taken) {
+    float muR = P[MU_ROLLING], h = P[H];
+    float ra = D.r[a.s];
+    float ia = wa / (INERTIA * (ra * ra));
+    float inverse = ia;
+    float io = 0;
+    float spinA[3], against[3];
+    spin_of(D, a.s, spinA);
+    for (int k = 0; k < 3; k++) against[k] = h * spinA[k] + a.turned[k];
+    if (o) {
+        float ro = D.r[o->s];
+        io = wo / (INERTIA * (ro * ro));
+        inverse = inverse + io;
+        float spinO[3];
+        spin_of(D, o->s, spinO);
+        for (int k = 0; k < 3; k++) against[k] = against[k] - (h * spinO[k] + o->turned[k]);
+    }
+    if (!(inverse > 0)) return;
+    float whole[3];
+    for (int k = 0; k < 3; k++) whole[k] = taken.q[k] - against[k] / inverse;
+    float size2 = dot3(whole, whole);
+    float holds = (muR * reach) * taken.normal;
+    if (size2 > holds * holds) {
+        float cut = fmin_(1.0f, holds / sqrtf(size2));
+        for (int k = 0; k < 3; k++) whole[k] = whole[k] * cut;
+    }
+    for (int k = 0; k < 3; k++) {
+        float change = whole[k] - taken.q[k];
+        taken.q[k] = whole[k];
+        a.turned[k] = a.turned[k] + ia * change;
+        if (o) o->turned[k] = o->turned[k] - io * change;
+    }
+}
+
+__device__ __forceinline__ float rubbing(const float *P) {  // Spheres.java:684-687
+    return P[MU_STATIC] + (P[MU_KINETIC] + P[MU_ROLLING]);
+}
+
+// Spheres.warm, Spheres.java:448-481.
+__device__ void warm(const SphereDev &D, Held &a, float wa, Held &o, float wo, const float d[3], float d2,
+                     float reach, Impulse &taken) {
+    float dist = sqrtf(d2);
+    float n[3];
+    for (int k = 0; k < 3; k++) n[k] = d[k] / dist;
+    for (int k = 0; k < 3; k++) {
+        float push = taken.normal * n[k];
+        float ma = push * wa, mo = -(push * wo);
+        a.at[k] = a.at[k] + ma;
+        o.at[k] = o.at[k] + mo;
+        a.displaced[k] = a.displaced[k] + ma;
+        o.displaced[k] = o.displaced[k] + mo;
+    }
+    float along = dot3(taken.p, n);
+    for (int k = 0; k < 3; k++) taken.p[k] = taken.p[k] - along * n[k];
+    float share = dist / reach;
+    float ra = D.r[a.s], ro = D.r[o.s];
+    float turnsA = turns_per_impulse(wa, share * ra, ra);
+    float turnsO = turns_per_impulse(wo, share * ro, ro);
+    rub(a, wa, turnsA, &o, wo, turnsO, n, taken.p);
+    float ia = wa / (INERTIA * (ra * ra));
+    float io = wo / (INERTIA * (ro * ro));
+    for (int k = 0; k < 3; k++) {
+        a.turned[k] = a.turned[k] + ia * taken.q[k];
+        o.turned[k] = o.turned[k] - io * taken.q[k];
+    }
+}
+
+// Spheres.contact, Spheres.java:369-439: contact c of spheres a < o, parting (rubs == 0) or rubbing.
+__device__ void contact(const SphereDev &D, const float *P, int c, int a, int o, int pass, int rubs) {
+    Held ha = hold(D, a), ho = hold(D, o);
+    float d[3];
+    for (int k = 0; k < 3; k++) d[k] = ha.at[k] - ho.at[k];
+    float d2 = dot3(d, d);
+    float ra = D.r[a], ro = D.r[o];
+    float reach = ra + ro;
+    float wa = D.im[a], wo = D.im[o];
+    float wsum = wa + wo;
+    float listed = reach * MARGIN1;
+    if (!(d2 < listed * listed) || !(d2 > COINCIDENT) || !(wsum > 0)) return;
+    float *imp = D.cur.impulse + (size_t)c * IMPULSE_WORDS;
+    Impulse taken = load_impulse(imp);
+    if (rubs == 0) {
+        if (pass == 1) {
+            if (rubbing(P) > 0) warm(D, ha, wa, ho, wo, d, d2, reach, taken);
+            else clear_impulse(taken);
+        }
+        for (int k = 0; k < 3; k++) d[k] = ha.at[k] - ho.at[k];
+        float dist = sqrtf(fmax_(dot3(d, d), COINCIDENT));
+        float depth = fmax_(-(taken.normal * wsum), P[OMEGA] * (reach - dist));
+        taken.normal = taken.normal + depth / wsum;
+        float kk = depth / (dist * wsum);
+        for (int k = 0; k < 3; k++) {
+            float step = kk * d[k];
+            float ma = step * wa, mo = -(step * wo);
+            ha.at[k] = ha.at[k] + ma;
+            ho.at[k] = ho.at[k] + mo;
+            ha.displaced[k] = ha.displaced[k] + ma;
+            ho.displaced[k] = ho.displaced[k] + mo;
+        }
+    } else {
+        float dist = sqrtf(d2);
+        float nn[3];
+        for (int k = 0; k < 3; k++) nn[k] = d[k] / dist;
+        float share = dist / reach;
+        float pairRadius = (ra * ro) / reach;
+        roll(D, P, ha, wa, &ho, wo, pairRadius, taken);
+        friction_at(D, P, ha, wa, share * ra, &ho, wo, share * ro, nn, taken);
+    }
+    store_impulse(imp, taken);
+    store(D, ha, true);
+    store(D, ho, true);
+}
+
+// Spheres.priority, Spheres.java:1166-1168 (31 bits); the same formula in 63 bits when wide.
+__device__ __forceinline__ long long priority(const SphereDev &D, int a, int o, int g) {
+    if (!D.wide) {
+        uint32_t pair = (uint32_t)a * (uint32_t)D.n + (uint32_t)o;
+        uint32_t p = pair * 0x9E3779B1u + (uint32_t)g * 0x632BE5ABu;
+        return (long long)(p & 0x7FFFFFFFu);
+    }
+    uint64_t pair = (uint64_t)a * (uint64_t)D.n + (uint64_t)o;
+    uint64_t p = pair * 0x9E3779B97F4A7C15ull + (uint64_t)(int64_t)g * 0x632BE59BD9B4E019ull;
+    return (long long)(p & 0x7FFFFFFFFFFFFFFFull);
+}
+
+// Spheres.turn, Spheres.java:1468-1493.
+__device__ __forceinline__ void turn(const SphereDev &D, int s, const float angle[3]) {
+    float a[3], was[4], now[4];
+    for (int k = 0; k < 3; k++) a[k] = 0.5f * angle[k];
+    was[0] = D.qx[s]; was[1] = D.qy[s]; was[2] = D.qz[s]; was[3] = D.qw[s];
+    now[0] = was[0] + (a[0] * was[3] + (a[1] * was[2] - a[2] * was[1]));
+    now[1] = was[1] + (a[1] * was[3] + (a[2] * was[0] - a[0] * was[2]));
+    now[2] = was[2] + (a[2] * was[3] + (a[0] * was[1] - a[1] * was[0]));
+    now[3] = was[3] - (a[0] * was[0] + (a[1] * was[1] + a[2] * was[2]));
+    float len2 = (now[0] * now[0] + now[1] * now[1]) + (now[2] * now[2] + now[3] * now[3]);
+    float inverse = 1.0f / sqrtf(fmax_(len2, 1e-30f));
+    D.qx[s] = now[0] * inverse; D.qy[s] = now[1] * inverse; D.qz[s] = now[2] * inverse; D.qw[s] = now[3] * inverse;
+}
+
+// Spheres.around, Spheres.java:1207-1229: the 27 cells about `key`, each cell's spheres in sorted order.
+template <class Visit>
+__device__ __forceinline__ void around(const SphereDev &D, int key, Visit visit) {
+    int ix = key % D.nx, iy = (key / D.nx) % D.ny, iz = key / (D.nx * D.ny);
+    for (int m = 0; m < 27; m++) {
+        int jx = ix + (m % 3 - 1), jy = iy + ((m / 3) % 3 - 1), jz = iz + (m / 9 - 1);
+        if (jx < 0 || jx >= D.nx || jy < 0 || jy >= D.ny || jz < 0 || jz >= D.nz) continue;
+        int cell = jx + D.nx * (jy + D.ny * jz);
+        int end = D.starts[cell + 1];
+        for (int k = D.starts[cell]; k < end; k++) visit(D.order[k]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Per-sphere passes.
+
+// Spheres.predict, Spheres.java:269-295.
+__global__ void k_predict(SphereDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    if (!(D.im[s] > 0)) return;
+    const float *P = D.params;
+    float h = P[H];
+    float v0 = D.u[s] + h * P[GX];
+    float v1 = D.v[s] + h * P[GY];
+    float v2 = D.w[s] + h * P[GZ];
+    D.u[s] = v0; D.v[s] = v1; D.w[s] = v2;
+    D.x[s] = D.x[s] + h * v0;
+    D.y[s] = D.y[s] + h * v1;
+    D.z[s] = D.z[s] + h * v2;
+    float angle[3] = {h * D.ax[s], h * D.ay[s], h * D.az[s]};
+    turn(D, s, angle);
+}
+
+// Spheres.bin, Spheres.java:1308-1323: the cell key of each centre, clamped into the grid.
+__global__ void k_bin(SphereDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    float at[3] = {D.x[s], D.y[s], D.z[s]};
+    int cells[3] = {D.nx, D.ny, D.nz};
+    int c[3];
+    for (int a = 0; a < 3; a++) {
+        float fl = floorf(at[a] / D.cell);
+        c[a] = (int)fmax_(0.f, fmin_((float)(cells[a] - 1), fl));
+    }
+    D.keys[s] = c[0] + D.nx * (c[1] + D.ny * c[2]);
+}
+
+// Spheres.warmWalls, Spheres.java:890-939.
+__global__ void k_warm_walls(SphereDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    if (!(D.im[s] > 0)) return;
+    const float *P = D.params;
+    float ext[3] = {P[SX], P[SY], P[SZ]};
+    float rb = rubbing(P);
+    float rs = D.r[s], w = D.im[s];
+    float turns = turns_per_impulse(w, rs, rs);
+    Held held = hold(D, s);
+    for (int axis = 0; axis < 3; axis++) {
+        float *base = D.wallImpulse + (size_t)s * WALL_IMPULSE_WORDS + axis * IMPULSE_WORDS;
+        Impulse taken = load_impulse(base);
+        float reach = rs * MARGIN1;
+        float position = held.at[axis];
+        float out = 0;
+        if (position < reach) out = 1;
+        if (position > ext[axis] - reach) out = -1;
+        bool near = rb > 0 && out != 0;
+        if (!near) clear_impulse(taken);
+        if (near) {
+            float push = (taken.normal * w) * out;
+            held.at[axis] = held.at[axis] + push;
+            held.displaced[axis] = held.displaced[axis] + push;
+            taken.p[axis] = 0;
+            float nrm[3] = {0, 0, 0};
+            nrm[axis] = out;
+            rub(held, w, turns, nullptr, 0, 0, nrm, taken.p);
+            for (int k = 0; k < 3; k++) held.turned[k] = held.turned[k] + (turns * taken.q[k]) / rs;
+        }
+        store_impulse(base, taken);
+    }
+    store(D, held, true);
+}
+
+// Spheres.clearContacts, Spheres.java:829-844: claims cleared; the list's length kept as the longest, then zeroed.
+__global__ void k_clear_contacts(SphereDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    D.claims[0][s] = 0;
+    D.claims[1][s] = 0;
+    D.claims[2][s] = 0;
+    if (s == 0) {
+        D.count[LONGEST] = max(D.count[LONGEST], D.count[LISTED]);
+        D.count[LISTED] = 0;
+    }
+}
+
+// Spheres.listContacts, Spheres.java:961-1022, in two sweeps: first each sphere counts the higher-index spheres
+// within the list margin (k_list_count); the counts are scanned into rows; then each sphere writes its row in the
+// order the walk meets them (k_list_write), with each pair's warm start found in the last substep's row of its
+// lower sphere, walking back from the row's end as the reference walks its chain from the head.
+__device__ __forceinline__ bool near_pair(const SphereDev &D, const float mine[3], float ra, int o) {
+    float dd[3] = {mine[0] - D.x[o], mine[1] - D.y[o], mine[2] - D.z[o]};
+    float d2 = dd[0] * dd[0] + (dd[1] * dd[1] + dd[2] * dd[2]);
+    float near = (ra + D.r[o]) * MARGIN1;
+    return d2 < near * near;
+}
+
+__global__ void k_list_count(SphereDev D) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= D.n) return;
+    float mine[3] = {D.x[a], D.y[a], D.z[a]};
+    float ra = D.r[a];
+    int found = 0;
+    around(D, D.keys[a], [&](int o) {
+        if (o > a && near_pair(D, mine, ra, o)) found++;
+    });
+    D.cur.cnt[a] = found;
+}
+
+__global__ void k_list_write(SphereDev D) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= D.n) return;
+    float mine[3] = {D.x[a], D.y[a], D.z[a]};
+    float ra = D.r[a];
+    int slot = D.cur.off[a];
+    int cap = D.cap;
+    int kStart = D.kept.off[a];
+    int kEnd = min(kStart + D.kept.cnt[a], cap);
+    int kLow = max(kStart, kEnd - MOST_KEPT);
+    around(D, D.keys[a], [&](int o) {
+        if (!(o > a) || !near_pair(D, mine, ra, o)) return;
+        int here = slot++;
+        if (here >= cap) return;
+        D.cur.a[here] = a;
+        D.cur.b[here] = o;
+        D.cur.done[here] = 0;
+        float warmth[IMPULSE_WORDS] = {0, 0, 0, 0, 0, 0, 0};
+        for (int j = kEnd - 1; j >= kLow; j--) {
+            if (D.kept.b[j] == o) {
+                const float *src = D.kept.impulse + (size_t)j * IMPULSE_WORDS;
+                for (int k = 0; k < IMPULSE_WORDS; k++) warmth[k] = src[k];
+                break;
+            }
+        }
+        float *dst = D.cur.impulse + (size_t)here * IMPULSE_WORDS;
+        for (int k = 0; k < IMPULSE_WORDS; k++) dst[k] = warmth[k];
+    });
+}
+
+// Spheres.contactRound, Spheres.java:1076-1133: one claim round over the list, one thread a contact.
+__global__ void k_round(SphereDev D, RoundConst rc) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) D.open[1 - rc.mark] = 0;
+    int listed = min(D.count[LISTED], D.cap);
+    if (c >= listed) return;
+    if (D.cur.done[c] == rc.pass) return;
+    int a = D.cur.a[c], o = D.cur.b[c];
+    const long long *checked = D.claims[(rc.slot + 2) % 3];
+    long long *claimed = D.claims[rc.slot % 3];
+    long long *cleared = D.claims[(rc.slot + 1) % 3];
+    bool won = false;
+    if (rc.checks > 0) {
+        long long last = priority(D, a, o, rc.checksDispatch);
+        won = checked[a] == last && checked[o] == last;
+    }
+    if (won) {
+        contact(D, D.params, c, a, o, rc.pass, rc.rubs);
+        D.cur.done[c] = rc.pass;
+    } else {
+        cleared[a] = 0;
+        cleared[o] = 0;
+        if (rc.claims > 0) {
+            D.open[rc.mark] = 1;
+            long long mine = priority(D, a, o, rc.dispatch);
+            atomicMax(&claimed[a], mine);
+            atomicMax(&claimed[o], mine);
+        } else {
+            atomicAdd(&D.count[MISSED], 1);
+        }
+    }
+}
+
+// Spheres.walls -> shift with friction, Spheres.java:1185-1198, 1351-1418.
+__global__ void k_walls(SphereDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    if (!(D.im[s] > 0)) return;
+    const float *P = D.params;
+    float ext[3] = {P[SX], P[SY], P[SZ]};
+    float rs = D.r[s];
+    Held held = hold(D, s);
+    for (int a = 0; a < 3; a++) {
+        float c = 0.f;
+        float moved = held.at[a] + c;
+        float high = ext[a] - rs;
+        float side = 1;
+        float into = rs - moved;
+        if (moved * 2.0f > ext[a]) {
+            side = -1;
+            into = moved - high;
+        }
+        float w = D.im[s];
+        float *base = D.wallImpulse + (size_t)s * WALL_IMPULSE_WORDS + a * IMPULSE_WORDS;
+        Impulse taken = load_impulse(base);
+        float inside = fmax_(rs, fmin_(high, moved));
+        float push = fmax_(into, 0.f);
+        if (!(into > 0) && taken.normal > 0) {
+            push = fmax_(into, -(taken.normal * w));
+            inside = moved + side * push;
+        }
+        taken.normal = taken.normal + push / w;
+        held.at[a] = inside;
+        held.displaced[a] = held.displaced[a] + (c + (inside - moved));
+        float reach = rs * LIST_MARGIN;
+        float out = 0;
+        if (!(inside > rs + reach)) out = 1;
+        if (!(inside < high - reach)) out = -1;
+        if (out != 0) {
+            float nrm[3] = {0, 0, 0};
+            nrm[a] = out;
+            roll(D, P, held, w, nullptr, 0, rs, taken);
+            friction_at(D, P, held, w, rs, nullptr, 0, 0, nrm, taken);
+        }
+        store_impulse(base, taken);
+    }
+    for (int a = 0; a < 3; a++) {
+        float high = ext[a] - rs;
+        float inside = fmax_(rs, fmin_(high, held.at[a]));
+        held.displaced[a] = held.displaced[a] + (inside - held.at[a]);
+        held.at[a] = inside;
+    }
+    store(D, held, true);
+}
+
+// Spheres.velocity, Spheres.java:1430-1460.
+__global__ void k_velocity(SphereDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    float perH = 1.0f / D.params[H];
+    float *vel[3] = {D.u, D.v, D.w}, *dsp[3] = {D.dx, D.dy, D.dz}, *bef[3] = {D.cx, D.cy, D.cz};
+    float *spin[3] = {D.ax, D.ay, D.az}, *tr[3] = {D.tx, D.ty, D.tz};
+    for (int a = 0; a < 3; a++) {
+        float was = vel[a][s];
+        bef[a][s] = was;
+        vel[a][s] = was + dsp[a][s] * perH;
+        dsp[a][s] = 0;
+    }
+    float angle[3];
+    for (int a = 0; a < 3; a++) {
+        float by = tr[a][s];
+        spin[a][s] = spin[a][s] + by * perH;
+        tr[a][s] = 0;
+        angle[a] = by;
+    }
+    turn(D, s, angle);
+}
+
+// Spheres.bounce / bounce(grid), Spheres.java:1523-1627.
+template <bool GRID>
+__global__ void k_bounce(SphereDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    const float *P = D.params;
+    float e = P[RESTITUTION];
+    float wa = D.im[s];
+    if (!(e > 0 && wa > 0)) return;
+    float h = P[H], gx = P[GX], gy = P[GY], gz = P[GZ];
+    float resting = (2.0f * h) * sqrtf(gx * gx + (gy * gy + gz * gz));
+    float ext[3] = {P[SX], P[SY], P[SZ]};
+    float ra = D.r[s];
+    float mine[3] = {D.x[s], D.y[s], D.z[s]};
+    float v[3] = {D.u[s], D.v[s], D.w[s]};
+    float was[3] = {D.cx[s], D.cy[s], D.cz[s]};
+    float sum[3] = {0, 0, 0};
+    auto against = [&](int o) {
+        if (o == s) return;
+        float dd[3] = {mine[0] - D.x[o], mine[1] - D.y[o], mine[2] - D.z[o]};
+        float d2 = dd[0] * dd[0] + (dd[1] * dd[1] + dd[2] * dd[2]);
+        float near = (ra + D.r[o]) * MARGIN1;
+        float wsum = wa + D.im[o];
+        if (!(d2 < near * near && d2 > COINCIDENT)) return;
+        float dist = sqrtf(d2);
+        float bo[3] = {D.cx[o], D.cy[o], D.cz[o]};
+        float vo[3] = {D.u[o], D.v[o], D.w[o]};
+        float closing = 0, parting = 0;
+        for (int a = 0; a < 3; a++) {
+            float nn = dd[a] / dist;
+            closing = closing + nn * (was[a] - bo[a]);
+            parting = parting + nn * (v[a] - vo[a]);
+        }
+        if (closing < -resting) {
+            float k = ((-(e * closing)) - parting) * wa / (wsum * dist);
+            for (int a = 0; a < 3; a++) sum[a] = sum[a] + k * dd[a];
+        }
+    };
+    if (GRID) {
+        around(D, D.keys[s], against);
+    } else {
+        for (int o = 0; o < D.n; o++) against(o);
+    }
+    for (int a = 0; a < 3; a++) {
+        float reach = ra * MARGIN1;
+        if (mine[a] < reach && was[a] < -resting) sum[a] = sum[a] + ((-(e * was[a])) - v[a]);
+        if (mine[a] > ext[a] - reach && was[a] > resting) sum[a] = sum[a] + ((-(e * was[a])) - v[a]);
+    }
+    D.dx[s] = sum[0];
+    D.dy[s] = sum[1];
+    D.dz[s] = sum[2];
+}
+
+// Spheres.bounced, Spheres.java:1633-1644.
+__global__ void k_bounced(SphereDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    D.u[s] = D.u[s] + D.dx[s]; D.dx[s] = 0;
+    D.v[s] = D.v[s] + D.dy[s]; D.dy[s] = 0;
+    D.w[s] = D.w[s] + D.dz[s]; D.dz[s] = 0;
+}
+
+// Spheres.solve / solve(grid) with Solving, Spheres.java:302-347, 1245-1298: the Jacobi move into cx, cy, cz.
+template <bool GRID>
+__global__ void k_jacobi_solve(SphereDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    float mine[3] = {D.x[s], D.y[s], D.z[s]};
+    float rs = D.r[s], ws = D.im[s];
+    float sum[3] = {0, 0, 0}, contacts = 0;
+    auto against = [&](int o) {
+        if (o == s) return;
+        float dd[3] = {mine[0] - D.x[o], mine[1] - D.y[o], mine[2] - D.z[o]};
+        float d2 = dd[0] * dd[0] + (dd[1] * dd[1] + dd[2] * dd[2]);
+        float reach = rs + D.r[o];
+        float wsum = ws + D.im[o];
+        if (!(d2 < reach * reach) || !(d2 > COINCIDENT) || !(wsum > 0)) return;
+        float dist = sqrtf(d2);
+        float share = ((reach - dist) * (ws / wsum)) / dist;
+        for (int a = 0; a < 3; a++) sum[a] = sum[a] + share * dd[a];
+        contacts = contacts + 1.0f;
+    };
+    if (GRID) {
+        around(D, D.keys[s], against);
+    } else {
+        for (int o = 0; o < D.n; o++) against(o);
+    }
+    const float *P = D.params;
+    float omega = P[OMEGA];
+    float scale = omega;
+    if (P[AVERAGED] > 0) scale = omega / fmax_(contacts, 1.0f);
+    D.cx[s] = scale * sum[0];
+    D.cy[s] = scale * sum[1];
+    D.cz[s] = scale * sum[2];
+}
+
+// Spheres.apply -> shift without friction, Spheres.java:1326-1342, 1351-1368.
+__global__ void k_jacobi_apply(SphereDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    if (!(D.im[s] > 0)) return;
+    const float *P = D.params;
+    float ext[3] = {P[SX], P[SY], P[SZ]};
+    float rs = D.r[s];
+    float cor[3] = {D.cx[s], D.cy[s], D.cz[s]};
+    Held held = hold(D, s);
+    for (int a = 0; a < 3; a++) {
+        float c = cor[a];
+        float moved = held.at[a] + c;
+        float high = ext[a] - rs;
+        float inside = fmax_(rs, fmin_(high, moved));
+        held.at[a] = inside;
+        float push = inside - moved;
+        held.displaced[a] = held.displaced[a] + (c + push);
+    }
+    store(D, held, false);
+}
+
+// Spheres.show / show(grid), Spheres.java:1697-1794.
+template <bool GRID>
+__global__ void k_show(SphereDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    const float *P = D.params;
+    float ext[3] = {P[SX], P[SY], P[SZ]};
+    float *b = D.shown + (size_t)s * SHOWN_STRIDE;
+    float mine[3] = {D.x[s], D.y[s], D.z[s]};
+    for (int a = 0; a < 3; a++) {
+        b[a] = b[4 + a];
+        b[4 + a] = mine[a];
+    }
+    float rs = D.r[s];
+    b[3] = rs;
+    for (int k = 0; k < 12; k++) b[8 + k] = b[20 + k];
+    float q[4] = {D.qx[s], D.qy[s], D.qz[s], D.qw[s]};
+    for (int k = 0; k < 4; k++) {
+        b[32 + k] = b[36 + k];
+        b[36 + k] = q[k];
+    }
+    float flat[6] = {0, 0, 0, 0, 0, 0}, lean[3] = {0, 0, 0};
+    for (int a = 0; a < 3; a++) {
+        float low = rs - mine[a];
+        float high = (mine[a] + rs) - ext[a];
+        if (low > 0) {
+            float share = low / rs;
+            flat[a] = flat[a] + share;
+            lean[a] = lean[a] + -1.0f * share;
+        }
+        if (high > 0) {
+            float share = high / rs;
+            flat[a] = flat[a] + share;
+            lean[a] = lean[a] + 1.0f * share;
+        }
+    }
+    auto against = [&](int o) {
+        if (o == s) return;
+        float dd[3] = {D.x[o] - mine[0], D.y[o] - mine[1], D.z[o] - mine[2]};
+        float d2 = dot3(dd, dd);
+        float reach = rs + D.r[o];
+        if (!(d2 < reach * reach && d2 > COINCIDENT)) return;
+        float dist = sqrtf(d2);
+        float share = (reach - dist) / reach;
+        float g = share / d2;
+        float along = share / dist;
+        flat[0] = flat[0] + g * (dd[0] * dd[0]);
+        flat[1] = flat[1] + g * (dd[1] * dd[1]);
+        flat[2] = flat[2] + g * (dd[2] * dd[2]);
+        flat[3] = flat[3] + g * (dd[0] * dd[1]);
+        flat[4] = flat[4] + g * (dd[0] * dd[2]);
+        flat[5] = flat[5] + g * (dd[1] * dd[2]);
+        for (int a = 0; a < 3; a++) lean[a] = lean[a] + along * dd[a];
+    };
+    if (GRID) {
+        around(D, D.keys[s], against);
+    } else {
+        for (int o = 0; o < D.n; o++) against(o);
+    }
+    float *after = b + 20;
+    for (int k = 0; k < 6; k++) after[k] = flat[k];
+    for (int a = 0; a < 3; a++) after[6 + a] = lean[a];
+    after[9] = (flat[0] + flat[1]) + flat[2];
+    after[10] = sqrtf(dot3(lean, lean));
+}
+
+__global__ void k_fill(float *p, float value, int64_t count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
+}
+
+__global__ void k_fill_i32(int32_t *p, int32_t value, int64_t count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Exclusive scan: blocks of SCAN_TILE elements scanned in LDS, their sums scanned recursively, then added back.
+
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = THREADS * SCAN_ITEMS;
+
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+    int lane = threadIdx.x & 63;
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the exclusive prefix, and the block total in *total.
+__device__ __forceinline__ int block_exclusive_scan(int v, int *total) {
+    __shared__ int waveSums[THREADS / 64];
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = wave_inclusive_scan(v);
+    if (lane == 63) waveSums[wave] = inc;
+    __syncthreads();
+    int base = 0, all = 0;
+    for (int w = 0; w < THREADS / 64; w++) {
+        int t = waveSums[w];
+        if (w < wave) base += t;
+        all += t;
+    }
+    __syncthreads();
+    *total = all;
+    return base + inc - v;
+}
+
+__global__ void k_scan_tile(const int32_t *in, int32_t *out, int n, int32_t *blockSums) {
+    int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int vals[SCAN_ITEMS];
+    int sum = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int i = base + k;
+        vals[k] = i < n ? in[i] : 0;
+        sum += vals[k];
+    }
+    int total;
+    int run = block_exclusive_scan(sum, &total);
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int i = base + k;
+        if (i < n) out[i] = run;
+        run += vals[k];
+    }
+    if (threadIdx.x == 0 && blockSums) blockSums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_add(int32_t *out, int n, const int32_t *blockOffsets) {
+    int i = blockIdx.x * SCAN_TILE + threadIdx.x;
+    int add = blockOffsets[blockIdx.x];
+    for (int k = 0; k < SCAN_ITEMS; k++, i += THREADS)
+        if (i < n) out[i] += add;
+}
+
+__global__ void k_scan_total(const int32_t *in, const int32_t *out, int n, int32_t *total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total = n > 0 ? out[n - 1] + in[n - 1] : 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Stable LSD radix sort of the cell keys, 8 bits a pass. Each block owns a tile of RS_TILE keys; the scatter ranks
+// each key among the tile's earlier keys of the same digit with wavefront ballots (eight ballots give the lanes that
+// share a lane's digit; a popcount of the lower ones is its rank in the wave), and the waves' counts are combined in
+// LDS, so equal keys keep their order and the result lists each cell's spheres by ascending index.
+
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = THREADS * RS_ITEMS;
+
+__global__ void k_rs_hist(const int32_t *keys, int n, int shift, int32_t *hist, int nblocks) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int base = blockIdx.x * RS_TILE;
+    for (int k = 0; k < RS_ITEMS; k++) {
+        int i = base + k * THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void k_rs_scatter(const int32_t *keysIn, const int32_t *valsIn, int32_t *keysOut, int32_t *valsOut, int n,
+                             int shift, const int32_t *offsets, int nblocks) {
+    __shared__ int base[256];
+    __shared__ int wcount[THREADS / 64][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    base[threadIdx.x] = offsets[threadIdx.x * nblocks + blockIdx.x];
+    const unsigned long long lower = (1ull << lane) - 1ull;
+    for (int k = 0; k < RS_ITEMS; k++) {
+        for (int w = 0; w < THREADS / 64; w++) wcount[w][threadIdx.x] = 0;
+        __syncthreads();
+        int i = blockIdx.x * RS_TILE + k * THREADS + threadIdx.x;
+        bool valid = i < n;
+        int key = valid ? keysIn[i] : 0;
+        int val = valid ? (valsIn ? valsIn[i] : i) : 0;
+        int d = (key >> shift) & 255;
+        unsigned long long peers = __ballot(valid);
+        for (int b = 0; b < 8; b++) {
+            int bit = (d >> b) & 1;
+            unsigned long long bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        int rank = __popcll(peers & lower);
+        if (valid && rank == 0) wcount[wave][d] = __popcll(peers);
+        __syncthreads();
+        {
+            int dg = threadIdx.x;
+            int run = base[dg];
+            for (int w = 0; w < THREADS / 64; w++) {
+                int t = wcount[w][dg];
+                wcount[w][dg] = run;
+                run += t;
+            }
+            base[dg] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            int pos = wcount[wave][d] + rank;
+            keysOut[pos] = key;
+            valsOut[pos] = val;
+        }
+        __syncthreads();
+    }
+}
+
+// starts[c] = the first sorted position whose key is at least c, for every c in 0..cells.
+__global__ void k_starts(const int32_t *sortedKeys, int n, int32_t *starts, int cells) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > cells) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (sortedKeys[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    starts[c] = lo;
+}
+
+}  // namespace
+
+int scan_tmp_words(int n) {
+    int words = 0;
+    while (n > SCAN_TILE) {
+        n = (n + SCAN_TILE - 1) / SCAN_TILE;
+        words += n + 1;
+    }
+    return words + 1;
+}
+
+void scan_exclusive(const int32_t *in, int32_t *out, int n, int32_t *total, int32_t *tmp, hipStream_t s) {
+    if (n <= 0) {
+        if (total) launch_fill_i32(total, 0, 1, s);
+        return;
+    }
+    int blocks = (n + SCAN_TILE - 1) / SCAN_TILE;
+    // The total needs in[n-1] before `out` overwrites it when the two alias.
+    int32_t *last = nullptr;
+    if (total && in == out) {
+        last = tmp;
+        tmp += 1;
+        (void)hipMemcpyAsync(last, in + n - 1, sizeof(int32_t), hipMemcpyDeviceToDevice, s);
+    }
+    if (blocks == 1) {
+        k_scan_tile<<<1, THREADS, 0, s>>>(in, out, n, nullptr);
+    } else {
+        int32_t *sums = tmp;
+        k_scan_tile<<<blocks, THREADS, 0, s>>>(in, out, n, sums);
+        scan_exclusive(sums, sums, blocks, nullptr, tmp + blocks, s);
+        k_scan_add<<<blocks, THREADS, 0, s>>>(out, n, sums);
+    }
+    check_launch("scan");
+    if (total) k_scan_total<<<1, 1, 0, s>>>(last ? last - (n - 1) : in, out, n, total);
+}
+
+void launch_fill(float *p, float value, int64_t count, hipStream_t s) {
+    if (count <= 0) return;
+    k_fill<<<blocks_for(count), THREADS, 0, s>>>(p, value, count);
+    check_launch("fill");
+}
+
+void launch_fill_i32(int32_t *p, int32_t value, int64_t count, hipStream_t s) {
+    if (count <= 0) return;
+    k_fill_i32<<<blocks_for(count), THREADS, 0, s>>>(p, value, count);
+    check_launch("fill");
+}
+
+void launch_predict(const SphereDev &D, hipStream_t s) {
+    k_predict<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("predict");
+}
+
+void launch_bin(const SphereDev &D, hipStream_t s) {
+    k_bin<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("bin");
+}
+
+void launch_sort(const SphereDev &D, const SortScratch &S, hipStream_t s) {
+    int bits = 1;
+    while ((1 << bits) < D.cells && bits < 31) bits++;
+    int passes = (bits + 7) / 8;
+    int nblocks = (D.n + RS_TILE - 1) / RS_TILE;
+    const int32_t *kin = D.keys;
+    const int32_t *vin = nullptr;
+    for (int p = 0; p < passes; p++) {
+        bool toOrder = ((passes - 1 - p) % 2) == 0;
+        int32_t *kout = toOrder ? S.keysB : S.keysA;
+        int32_t *vout = toOrder ? D.order : S.valsA;
+        k_rs_hist<<<nblocks, THREADS, 0, s>>>(kin, D.n, 8 * p, S.hist, nblocks);
+        scan_exclusive(S.hist, S.hist, 256 * nblocks, nullptr, S.scanTmp, s);
+        k_rs_scatter<<<nblocks, THREADS, 0, s>>>(kin, vin, kout, vout, D.n, 8 * p, S.hist, nblocks);
+        check_launch("radix sort");
+        kin = kout;
+        vin = vout;
+    }
+    k_starts<<<blocks_for((int64_t)D.cells + 1), THREADS, 0, s>>>(kin, D.n, D.starts, D.cells);
+    check_launch("starts");
+}
+
+void launch_warm_walls(const SphereDev &D, hipStream_t s) {
+    k_warm_walls<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("warm walls");
+}
+
+void launch_clear_contacts(const SphereDev &D, hipStream_t s) {
+    k_clear_contacts<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("clear contacts");
+}
+
+void launch_list(const SphereDev &D, const SortScratch &S, hipStream_t s) {
+    k_list_count<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("list count");
+    scan_exclusive(D.cur.cnt, D.cur.off, D.n, D.count + LISTED, S.scanTmp, s);
+    k_list_write<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("list write");
+}
+
+void launch_round(const SphereDev &D, const RoundConst &rc, hipStream_t s) {
+    k_round<<<blocks_for(D.cap), THREADS, 0, s>>>(D, rc);
+    check_launch("contact round");
+}
+
+void launch_walls(const SphereDev &D, hipStream_t s) {
+    k_walls<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("walls");
+}
+
+void launch_velocity(const SphereDev &D, hipStream_t s) {
+    k_velocity<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("velocity");
+}
+
+void launch_bounce(const SphereDev &D, bool grid, hipStream_t s) {
+    if (grid) k_bounce<true><<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    else k_bounce<false><<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("bounce");
+}
+
+void launch_bounced(const SphereDev &D, hipStream_t s) {
+    k_bounced<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("bounced");
+}
+
+void launch_jacobi_solve(const SphereDev &D, bool grid, hipStream_t s) {
+    if (grid) k_jacobi_solve<true><<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    else k_jacobi_solve<false><<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("jacobi solve");
+}
+
+void launch_jacobi_apply(const SphereDev &D, hipStream_t s) {
+    k_jacobi_apply<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("jacobi apply");
+}
+
+void launch_show(const SphereDev &D, bool grid, hipStream_t s) {
+    if (grid) k_show<true><<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    else k_show<false><<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("show");
+}
+
+}  // namespace vr

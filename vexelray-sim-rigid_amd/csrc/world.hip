@@ -1,0 +1,499 @@
+// world.hip — the C ABI (include/vexelray_rigid.h) over the sphere kernels: buffers by name, and the step schedule.
+//
+// The schedule is the reference's: SphereStep's pass lists (core/.../sphere/SphereStep.java:184-208, 216-262,
+// 348-392) run by SphereStepper.step (core/.../sphere/SphereStepper.java:66-103), every pass a kernel on the world's
+// stream. A Gauss-Seidel pass runs until no contact is left open: its opening batch of rounds, then batches of
+// SphereStep.BATCH, with the open word read back after each batch, and the opening batch adapted as the reference's
+// stepper adapts it (SphereStepper.java:118-126), so the rounds run, and the report, are the reference's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sphere_kernels.h"
+#include "vexelray_rigid.h"
+
+namespace {
+
+using namespace vr;
+
+thread_local std::string g_error;
+
+struct ArgError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct StateError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct NameError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct DeviceError : std::runtime_error { using std::runtime_error::runtime_error; };
+
+#define HIP_OK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+constexpr int BATCH = 8, CYCLE = 48, SHRINK_AFTER = 120;  // SphereStep.java:48-59, SphereStepper.java:49
+constexpr int MOST_LISTED_SPHERES = 46340;                 // Spheres.java:787
+
+struct Buf {
+    void *p;
+    int64_t count;
+    int dtype;
+};
+
+}  // namespace
+
+struct vr_world {
+    vr_step_desc d{};
+    int n = 0, cap = 0, cells = 0;
+    bool gs = false, grid = false, friction = true;
+    hipStream_t stream = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    std::vector<void *> allocs;
+    std::map<std::string, Buf> fixed;  // buffers whose address never changes
+    SphereDev D{};
+    SortScratch S{};
+    ContactList lists[2]{};
+    int par = 0;  // lists[par] is this substep's list, lists[1 - par] the one kept from the substep before
+    int32_t *control = nullptr;  // device: contactCount[3], open[2]
+    int32_t *pinned = nullptr;   // host copy of control
+    std::vector<int> opening, enough;
+
+    template <class T>
+    T *alloc(int64_t count) {
+        void *p = nullptr;
+        size_t bytes = (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+        HIP_OK(hipMalloc(&p, bytes));
+        allocs.push_back(p);
+        return (T *)p;
+    }
+
+    void reg(const char *name, void *p, int64_t count, int dtype) { fixed[name] = Buf{p, count, dtype}; }
+
+    void build(const vr_step_desc &desc) {
+        d = desc;
+        if (d.bodies < 1 || d.substeps < 1 || d.iterations < 1)
+            throw ArgError("a step needs spheres, substeps and iterations, got " + std::to_string(d.bodies) + ", " +
+                           std::to_string(d.substeps) + ", " + std::to_string(d.iterations));
+        if (d.solve != VR_JACOBI && d.solve != VR_GAUSS_SEIDEL_BY_CONTACT) throw ArgError("unknown solve");
+        grid = d.has_grid != 0;
+        gs = d.solve == VR_GAUSS_SEIDEL_BY_CONTACT;
+        if (gs && !grid) throw ArgError("Gauss–Seidel works over the grid's cells, and there is no grid");
+        if (d.rounds < 1 || d.contacts_per_sphere < 1)
+            throw ArgError("at least a round and a place a sphere, got " + std::to_string(d.rounds) + " and " +
+                           std::to_string(d.contacts_per_sphere));
+        if (gs && !d.wide_pairs && d.bodies > MOST_LISTED_SPHERES)
+            throw ArgError("a contact list orders at most 46340 spheres, got " + std::to_string(d.bodies));
+        if (grid && (d.nx < 1 || d.ny < 1 || d.nz < 1 || !(d.cell > 0)))
+            throw ArgError("a grid of at least one cell, of positive size");
+        int devices = 0;
+        if (hipGetDeviceCount(&devices) != hipSuccess || devices < 1) throw DeviceError("no HIP device");
+        n = d.bodies;
+        cap = n * d.contacts_per_sphere;
+        HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_OK(hipEventCreate(&t0));
+        HIP_OK(hipEventCreate(&t1));
+        HIP_OK(hipHostMalloc((void **)&pinned, 8 * sizeof(int32_t), hipHostMallocDefault));
+
+        D.n = n;
+        D.cap = cap;
+        D.wide = d.wide_pairs ? 1 : 0;
+        static const char *sphere[] = {"x", "y", "z", "u", "v", "w", "r", "im", "cx", "cy", "cz", "dx",
+                                       "dy", "dz", "ax", "ay", "az", "tx", "ty", "tz", "qx", "qy", "qz", "qw"};
+        float **fields[] = {&D.x, &D.y, &D.z, &D.u, &D.v, &D.w, &D.r, &D.im, &D.cx, &D.cy, &D.cz, &D.dx,
+                            &D.dy, &D.dz, &D.ax, &D.ay, &D.az, &D.tx, &D.ty, &D.tz, &D.qx, &D.qy, &D.qz, &D.qw};
+        // One arena for the 24 fields, each padded to 64 floats so every field starts on a 256-byte boundary.
+        int64_t stride = ((int64_t)n + 63) / 64 * 64;
+        float *arena = alloc<float>(24 * stride);
+        for (int k = 0; k < 24; k++) {
+            *fields[k] = arena + k * stride;
+            reg(sphere[k], *fields[k], n, VR_F32);
+        }
+        float *params = alloc<float>(PARAM_COUNT);
+        D.params = params;
+        reg("params", params, PARAM_COUNT, VR_F32);
+        D.shown = alloc<float>((int64_t)SHOWN_STRIDE * n);
+        reg("shown", D.shown, (int64_t)SHOWN_STRIDE * n, VR_F32);
+        if (grid) {
+            D.nx = d.nx;
+            D.ny = d.ny;
+            D.nz = d.nz;
+            D.cell = d.cell;
+            cells = d.nx * d.ny * d.nz;
+            D.cells = cells;
+            D.keys = alloc<int32_t>(n);
+            D.order = alloc<int32_t>(n);
+            D.starts = alloc<int32_t>((int64_t)cells + 1);
+            reg("keys", D.keys, n, VR_I32);
+            reg("order", D.order, n, VR_I32);
+            reg("starts", D.starts, (int64_t)cells + 1, VR_I32);
+            int nblocks = (n + 2047) / 2048;
+            S.maxBlocks = nblocks;
+            S.keysA = alloc<int32_t>(n);
+            S.valsA = alloc<int32_t>(n);
+            S.keysB = alloc<int32_t>(n);
+            S.hist = alloc<int32_t>(256 * (int64_t)nblocks);
+            int tmpWords = std::max(scan_tmp_words(256 * nblocks), scan_tmp_words(n)) + 64;
+            S.scanTmp = alloc<int32_t>(tmpWords);
+        }
+        control = alloc<int32_t>(8);
+        D.count = control;
+        D.open = control + 3;
+        if (gs) {
+            for (int k = 0; k < 3; k++) D.claims[k] = alloc<long long>(n);
+            for (auto &L : lists) {
+                L.a = alloc<int32_t>(cap);
+                L.b = alloc<int32_t>(cap);
+                L.done = alloc<int32_t>(cap);
+                L.impulse = alloc<float>((int64_t)IMPULSE_WORDS * cap);
+                L.off = alloc<int32_t>((int64_t)n + 1);
+                L.cnt = alloc<int32_t>(n);
+            }
+            D.wallImpulse = alloc<float>((int64_t)WALL_IMPULSE_WORDS * n);
+            reg("wallImpulse", D.wallImpulse, (int64_t)WALL_IMPULSE_WORDS * n, VR_F32);
+            reg("contactCount", D.count, 3, VR_I32);
+            reg("open", D.open, 2, VR_I32);
+        }
+        opening.assign(d.iterations, BATCH);
+        enough.assign(d.iterations, 0);
+        set_lists();
+        clear();
+    }
+
+    void set_lists() {
+        D.cur = lists[par];
+        D.kept = lists[1 - par];
+    }
+
+    Buf find(const std::string &name) {
+        auto it = fixed.find(name);
+        if (it != fixed.end()) return it->second;
+        if (gs) {
+            const ContactList &c = lists[par], &k = lists[1 - par];
+            if (name == "contactA") return Buf{c.a, cap, VR_I32};
+            if (name == "contactB") return Buf{c.b, cap, VR_I32};
+            if (name == "contactDone") return Buf{c.done, cap, VR_I32};
+            if (name == "contactImpulse") return Buf{c.impulse, (int64_t)IMPULSE_WORDS * cap, VR_F32};
+            if (name == "keptB") return Buf{k.b, cap, VR_I32};
+            if (name == "keptImpulse") return Buf{k.impulse, (int64_t)IMPULSE_WORDS * cap, VR_F32};
+        }
+        throw NameError("no buffer named " + name);
+    }
+
+    // PassRunner.clear() then SphereStep.constants(): every buffer zero, qw = 1 (SphereStep.java:157-160).
+    void clear() {
+        for (auto &kv : fixed) {
+            const Buf &b = kv.second;
+            HIP_OK(hipMemsetAsync(b.p, 0, (size_t)b.count * 4, stream));
+        }
+        HIP_OK(hipMemsetAsync(control, 0, 8 * sizeof(int32_t), stream));
+        if (gs) {
+            for (int k = 0; k < 3; k++) HIP_OK(hipMemsetAsync(D.claims[k], 0, (size_t)n * 8, stream));
+            for (auto &L : lists) {
+                HIP_OK(hipMemsetAsync(L.a, 0, (size_t)cap * 4, stream));
+                HIP_OK(hipMemsetAsync(L.b, 0, (size_t)cap * 4, stream));
+                HIP_OK(hipMemsetAsync(L.done, 0, (size_t)cap * 4, stream));
+                HIP_OK(hipMemsetAsync(L.impulse, 0, (size_t)cap * IMPULSE_WORDS * 4, stream));
+                HIP_OK(hipMemsetAsync(L.off, 0, (size_t)(n + 1) * 4, stream));
+                HIP_OK(hipMemsetAsync(L.cnt, 0, (size_t)n * 4, stream));
+            }
+        }
+        launch_fill(D.qw, 1.0f, n, stream);
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    void write(const std::string &name, const void *src, int64_t count) {
+        Buf b = find(name);
+        if (count < 0 || count > b.count) throw NameError("buffer " + name + " is shorter than the write");
+        HIP_OK(hipMemcpyAsync(b.p, src, (size_t)count * 4, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    void read(const std::string &name, void *dst, int64_t count) {
+        Buf b = find(name);
+        if (count < 0 || count > b.count) throw NameError("buffer " + name + " is shorter than the read");
+        HIP_OK(hipMemcpyAsync(dst, b.p, (size_t)count * 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    void params(const int32_t *words, int32_t count) {
+        if (count != PARAM_COUNT) throw ArgError("sphere parameters are 13 words");
+        HIP_OK(hipMemcpyAsync((void *)D.params, words, sizeof(int32_t) * PARAM_COUNT, hipMemcpyHostToDevice,
+                              stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        float f[PARAM_COUNT];
+        std::memcpy(f, words, sizeof(f));
+        friction = f[MU_STATIC] + f[MU_KINETIC] + f[MU_ROLLING] > 0;  // Spheres.rubs, Spheres.java:232-235
+    }
+
+    // ---- pass lists ----
+
+    static int position(int k) { return k == 0 ? 0 : 1 + (k - 1) % CYCLE; }  // SphereStep.java:337-339
+
+    RoundConst cycle_round(int it, int p, bool rubs) const {  // SphereStep.java:286-315, 320-327
+        RoundConst rc;
+        rc.checks = p == 0 ? 0 : 1;
+        rc.claims = 1;
+        rc.pass = rubs ? d.iterations + it + 1 : it + 1;
+        rc.dispatch = it * CYCLE + p % CYCLE;
+        rc.mark = p % 2;
+        rc.checksDispatch = it * CYCLE + std::max(p - 1, 0) % CYCLE;
+        rc.rubs = rubs ? 1 : 0;
+        rc.slot = p % 3;
+        return rc;
+    }
+
+    void predict() { launch_predict(D, stream); }
+
+    void prepare() {  // SphereStep.java:170-182, 227-235
+        if (grid) {
+            launch_bin(D, stream);
+            launch_sort(D, S, stream);
+        }
+        if (gs) {
+            launch_warm_walls(D, stream);
+            par ^= 1;  // keepContacts: the list becomes the kept one, and the chains start empty
+            set_lists();
+            launch_clear_contacts(D, stream);
+            launch_list(D, S, stream);
+        }
+    }
+
+    void velocity_list() {  // SphereStep.java:150-159
+        launch_velocity(D, stream);
+        launch_bounce(D, grid, stream);
+        launch_bounced(D, stream);
+    }
+
+    void rubs(int iteration, int rounds) {  // SphereStep.java:270-276
+        for (int r = 0; r < rounds; r++) launch_round(D, cycle_round(iteration, position(r), true), stream);
+    }
+
+    // SphereStep.addRounds and the report read back (SphereStep.java:394-402; Spheres.java:1146-1159).
+    int add_rounds(int it, int from, int rounds, int *listed) {
+        for (int k = from; k < from + rounds; k++) launch_round(D, cycle_round(it, position(k), false), stream);
+        int mark = position(from + rounds - 1) % 2;
+        HIP_OK(hipMemcpyAsync(pinned, control, 5 * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        *listed = pinned[LISTED];
+        return pinned[3 + mark];
+    }
+
+    void adapt(int it, int ran) {  // SphereStepper.java:118-126
+        if (ran > opening[it]) {
+            opening[it] = std::min(ran, CYCLE);
+            enough[it] = 0;
+        } else if (++enough[it] >= SHRINK_AFTER && opening[it] > BATCH) {
+            opening[it] -= BATCH;
+            enough[it] = 0;
+        }
+    }
+
+    void step_until_done(vr_step_report *rep) {  // SphereStepper.java:66-103
+        int waits = 0, rounds = 0, most = 0, unlisted = 0, rubbed = 0;
+        int limit = cap + 2 * CYCLE;
+        for (int sub = 0; sub < d.substeps; sub++) {
+            for (int it = 0; it < d.iterations; it++) {
+                int k = opening[it];
+                rubs(it == 0 ? d.iterations - 1 : it - 1, rubbed);
+                if (it > 0) {
+                    launch_walls(D, stream);
+                } else {
+                    if (sub > 0) {
+                        launch_walls(D, stream);
+                        velocity_list();
+                    }
+                    predict();
+                    prepare();
+                }
+                int listed = 0;
+                int open = add_rounds(it, 0, k, &listed);
+                waits++;
+                while (open != 0) {
+                    if (k > limit)
+                        throw StateError("a pass is still open after " + std::to_string(k) +
+                                         " rounds, on a list of " + std::to_string(listed));
+                    open = add_rounds(it, k, BATCH, &listed);
+                    k += BATCH;
+                    waits++;
+                }
+                rubbed = friction ? k : 0;
+                rounds += k;
+                most = std::max(most, k);
+                unlisted += std::max(0, listed - cap);
+                adapt(it, k);
+            }
+        }
+        rubs(d.iterations - 1, rubbed);
+        launch_walls(D, stream);
+        velocity_list();
+        launch_show(D, grid, stream);
+        rep->waits = waits;
+        rep->rounds = rounds;
+        rep->most_rounds = most;
+        rep->unlisted = unlisted;
+    }
+
+    // SphereStep.step(): Jacobi, or Gauss-Seidel in fixed rounds (SphereStep.java:184-208, 245-260).
+    void step_list() {
+        for (int sub = 0; sub < d.substeps; sub++) {
+            predict();
+            prepare();
+            if (!gs) {
+                for (int it = 0; it < d.iterations; it++) {
+                    launch_jacobi_solve(D, grid, stream);
+                    launch_jacobi_apply(D, stream);
+                }
+            } else {
+                int g = 0;
+                for (int it = 0; it < d.iterations; it++) {
+                    for (int k = 0; k <= d.rounds; k++, g++) {
+                        RoundConst rc;
+                        rc.checks = k == 0 ? 0 : 1;
+                        rc.claims = k == d.rounds ? 0 : 1;
+                        rc.pass = it + 1;
+                        rc.dispatch = g;
+                        rc.mark = g % 2;
+                        rc.checksDispatch = g - 1;
+                        rc.rubs = 0;
+                        rc.slot = g % 3;
+                        launch_round(D, rc, stream);
+                    }
+                    launch_walls(D, stream);
+                }
+            }
+            velocity_list();
+        }
+        launch_show(D, grid, stream);
+    }
+
+    void step(int32_t wait, vr_step_report *rep) {
+        vr_step_report local{};
+        vr_step_report *r = rep ? rep : &local;
+        std::memset(r, 0, sizeof(*r));
+        auto w0 = std::chrono::steady_clock::now();
+        HIP_OK(hipEventRecord(t0, stream));
+        if (gs && !d.fixed_rounds) step_until_done(r);
+        else step_list();
+        HIP_OK(hipEventRecord(t1, stream));
+        if (wait) {
+            HIP_OK(hipStreamSynchronize(stream));
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, t0, t1));
+            r->gpu_ns = (int64_t)(ms * 1e6);
+        }
+        auto w1 = std::chrono::steady_clock::now();
+        r->wall_ns = std::chrono::duration_cast<std::chrono::nanoseconds>(w1 - w0).count();
+    }
+
+    ~vr_world() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : allocs) (void)hipFree(p);
+        if (pinned) (void)hipHostFree(pinned);
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int fail(int code, const std::string &msg) {
+    g_error = msg;
+    return code;
+}
+
+template <class F>
+int guard(F fn) {
+    try {
+        fn();
+        return VR_OK;
+    } catch (const ArgError &e) {
+        return fail(VR_ERR_ARGUMENT, e.what());
+    } catch (const StateError &e) {
+        return fail(VR_ERR_STATE, e.what());
+    } catch (const NameError &e) {
+        return fail(VR_ERR_NAME, e.what());
+    } catch (const std::exception &e) {
+        return fail(VR_ERR_DEVICE, e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out) {
+    if (!desc || !out) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (kind != VR_SPHERES) return fail(VR_ERR_ARGUMENT, "unknown world kind " + std::to_string(kind));
+    auto *w = new vr_world();
+    int rc = guard([&] { w->build(*desc); });
+    if (rc != VR_OK) {
+        delete w;
+        return rc;
+    }
+    *out = w;
+    return VR_OK;
+}
+
+int vr_world_clear(vr_world *w) {
+    if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    return guard([&] { w->clear(); });
+}
+
+int vr_world_write(vr_world *w, const char *name, const void *src, int64_t count) {
+    if (!w || !name || (!src && count > 0)) return fail(VR_ERR_ARGUMENT, "null argument");
+    return guard([&] { w->write(name, src, count); });
+}
+
+int vr_world_read(vr_world *w, const char *name, void *dst, int64_t count) {
+    if (!w || !name || (!dst && count > 0)) return fail(VR_ERR_ARGUMENT, "null argument");
+    return guard([&] { w->read(name, dst, count); });
+}
+
+int vr_world_buffer_info(vr_world *w, const char *name, int64_t *count, int32_t *dtype) {
+    if (!w || !name || !count || !dtype) return fail(VR_ERR_ARGUMENT, "null argument");
+    return guard([&] {
+        Buf b = w->find(name);
+        *count = b.count;
+        *dtype = b.dtype;
+    });
+}
+
+int vr_world_params(vr_world *w, const int32_t *words, int32_t n) {
+    if (!w || !words) return fail(VR_ERR_ARGUMENT, "null argument");
+    return guard([&] { w->params(words, n); });
+}
+
+int vr_world_friction(vr_world *w, int32_t on) {
+    if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    w->friction = on != 0;
+    return VR_OK;
+}
+
+int vr_world_step(vr_world *w, int32_t wait, vr_step_report *report) {
+    if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    return guard([&] { w->step(wait, report); });
+}
+
+int vr_world_finish(vr_world *w) {
+    if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    return guard([&] { HIP_OK(hipStreamSynchronize(w->stream)); });
+}
+
+int vr_world_destroy(vr_world *w) {
+    delete w;
+    return VR_OK;
+}
+
+int vr_last_error(char *buf, int32_t len) {
+    if (!buf || len <= 0) return VR_OK;
+    std::strncpy(buf, g_error.c_str(), (size_t)len - 1);
+    buf[len - 1] = 0;
+    return VR_OK;
+}
+
+}  // extern "C"
