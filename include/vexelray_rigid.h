@@ -96,6 +96,20 @@ typedef struct vr_step_report {
     int64_t gpu_ns;
 } vr_step_report;
 
+/*
+ * Measurement (no counterpart in the reference): HIP events on the world's stream bracket every Gauss-Seidel pass,
+ * whose round kernels (k_round_wide ... k_round_tail) are launched back to back, so the time in round kernels is
+ * known without a profiler. listed_sum adds up, over every pass run (parting and rubbing), the contacts listed for
+ * its substep: the C of the algorithmic-byte count 72 C + 104 N a pass is credited with.
+ */
+typedef struct vr_profile {
+    int64_t round_ns;
+    int64_t round_launches;
+    int64_t passes;
+    int64_t listed_sum;
+    int64_t steps;
+} vr_profile;
+
 int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out);
 int vr_world_clear(vr_world *w);
 int vr_world_write(vr_world *w, const char *name, const void *src, int64_t count);
@@ -108,6 +122,9 @@ int vr_world_step(vr_world *w, int32_t wait, vr_step_report *report);
 int vr_world_finish(vr_world *w);
 int vr_world_destroy(vr_world *w);
 int vr_last_error(char *buf, int32_t len);
+/* Switches pass timing on or off and zeroes its counters. */
+int vr_world_profile(vr_world *w, int32_t on);
+int vr_world_profile_read(vr_world *w, vr_profile *out);
 
 #ifdef __cplusplus
 }

@@ -1202,6 +1202,14 @@ int vro_last_error(char *buf, int32_t len) {
     return VR_OK;
 }
 
+// The oracle keeps no pass timing; its profile reads as zeros.
+int vro_world_profile(vro_world *, int32_t) { return VR_OK; }
+
+int vro_world_profile_read(vro_world *, vr_profile *out) {
+    if (out) std::memset(out, 0, sizeof(*out));
+    return VR_OK;
+}
+
 // Opening batch sizes the stepper holds, for the tests of its adaptation (SphereStepper.opening, :115-117).
 int vro_world_opening(vro_world *p, int32_t iteration) { return p->w.opening.at(iteration); }
 

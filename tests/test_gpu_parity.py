@@ -125,3 +125,11 @@ def test_list_overflow_is_counted_the_same():
     assert any(r[3] > 0 for r in a["reports"]), "the list should overflow"
     assert np.array_equal(a["contactCount"], b["contactCount"])
     assert_bits(a, b, "overflow")
+
+
+def test_reports_and_state_over_many_steps_match_oracle():
+    """40 steps of the friction pile: the per-step reports (waits, rounds, most rounds) follow the reference's stepper,
+    its opening batch growing and shrinking (SphereStepper.java:118-126), and the state stays bit-identical."""
+    a, b = run_both(friction_pile, 40, extra=("contactCount",))
+    assert a["reports"] == b["reports"]
+    assert_bits(a, b, "40 steps")

@@ -23,7 +23,8 @@ VR_I32 = 2
 
 # Every entry point the header declares, for the export check.
 ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "world_buffer_info", "world_params",
-                "world_friction", "world_step", "world_finish", "world_destroy", "last_error")
+                "world_friction", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
+                "world_profile_read")
 
 
 class StepDesc(ctypes.Structure):
@@ -37,6 +38,11 @@ class StepDesc(ctypes.Structure):
 class StepReport(ctypes.Structure):
     _fields_ = [("waits", ctypes.c_int32), ("rounds", ctypes.c_int32), ("most_rounds", ctypes.c_int32),
                 ("unlisted", ctypes.c_int32), ("wall_ns", ctypes.c_int64), ("gpu_ns", ctypes.c_int64)]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [("round_ns", ctypes.c_int64), ("round_launches", ctypes.c_int64), ("passes", ctypes.c_int64),
+                ("listed_sum", ctypes.c_int64), ("steps", ctypes.c_int64)]
 
 
 class Abi:
@@ -63,6 +69,8 @@ class Abi:
             "world_finish": ([p]),
             "world_destroy": ([p]),
             "last_error": ([ctypes.c_char_p, ctypes.c_int32]),
+            "world_profile": ([p, ctypes.c_int32]),
+            "world_profile_read": ([p, ctypes.POINTER(Profile)]),
         }
         for name, args in sig.items():
             fn = getattr(self.lib, prefix + name)
@@ -153,6 +161,14 @@ class World:
 
     def finish(self) -> None:
         self._check(self.abi.fn("world_finish")(self.handle))
+
+    def profile(self, on: bool) -> None:
+        self._check(self.abi.fn("world_profile")(self.handle, 1 if on else 0))
+
+    def profile_read(self) -> Profile:
+        out = Profile()
+        self._check(self.abi.fn("world_profile_read")(self.handle, ctypes.byref(out)))
+        return out
 
     def destroy(self) -> None:
         if self.handle:

@@ -47,6 +47,25 @@ struct RoundConst {
     int checks, claims, pass, dispatch, mark, checksDispatch, rubs, slot;
 };
 
+// Device control of the Gauss-Seidel passes run until done (see k_round_wide / k_round_tail). Each pass of a step
+// has a slot of its own, set up once at the start of the step, so no pass needs a kernel to set it up.
+struct PassSlot {
+    int rd[2];   // rd[j & 1] == j: round j may still run as a wide round
+    int cnt[4];  // cnt[j & 3]: contacts open at the start of round j (round 0: every listed contact)
+};
+
+struct PassCtl {
+    int k;  // rounds the last parting pass ran, as the reference's stepper counts them
+    int error, errRounds, errListed;
+    int waits, rounds, most, unlisted;  // SphereStepper.Report, added up over the step
+    long long listedSum;                // contacts listed, added up over every pass (parting and rubbing)
+    int wideRan[64];                    // per pass of a substep: the most rounds run wide this step
+};
+
+struct OpenLists {
+    int32_t *buf[2];  // round j's open contacts are in buf[j & 1] (round 0 takes every listed contact)
+};
+
 struct SortScratch {
     int32_t *keysA, *valsA, *keysB, *hist, *scanTmp;
     int maxBlocks;
@@ -60,6 +79,11 @@ void launch_warm_walls(const SphereDev &D, hipStream_t s);
 void launch_clear_contacts(const SphereDev &D, hipStream_t s);
 void launch_list(const SphereDev &D, const SortScratch &S, hipStream_t s);
 void launch_round(const SphereDev &D, const RoundConst &rc, hipStream_t s);
+// A pass run until done (parting) or for the parting pass's rounds (rubbing), with no host wait: `wide` rounds of
+// the whole chip while many contacts are open, then one workgroup finishes the pass.
+void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLists &L, int32_t *opening,
+                 int32_t *enough, int it, int iterations, bool rub, int wide, hipStream_t s);
+void launch_step_begin(PassCtl *ctl, PassSlot *slots, int nslots, hipStream_t s);
 void launch_walls(const SphereDev &D, hipStream_t s);
 void launch_velocity(const SphereDev &D, hipStream_t s);
 void launch_bounce(const SphereDev &D, bool grid, hipStream_t s);

@@ -426,8 +426,10 @@ __device__ __forceinline__ bool near_pair(const SphereDev &D, const float mine[3
 }
 
 __global__ void k_list_count(SphereDev D) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= D.n) return;
+    // Threads take spheres in cell order, so a wave walks neighbouring cells together.
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.n) return;
+    int a = D.order[t];
     float mine[3] = {D.x[a], D.y[a], D.z[a]};
     float ra = D.r[a];
     int found = 0;
@@ -438,8 +440,10 @@ __global__ void k_list_count(SphereDev D) {
 }
 
 __global__ void k_list_write(SphereDev D) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= D.n) return;
+    // Threads take spheres in cell order, so a wave walks neighbouring cells together.
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.n) return;
+    int a = D.order[t];
     float mine[3] = {D.x[a], D.y[a], D.z[a]};
     float ra = D.r[a];
     int slot = D.cur.off[a];
@@ -472,31 +476,209 @@ __global__ void k_round(SphereDev D, RoundConst rc) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c == 0) D.open[1 - rc.mark] = 0;
     int listed = min(D.count[LISTED], D.cap);
-    if (c >= listed) return;
-    if (D.cur.done[c] == rc.pass) return;
+    bool stillOpen = false;
+    if (c < listed && D.cur.done[c] != rc.pass) {
+        int a = D.cur.a[c], o = D.cur.b[c];
+        const long long *checked = D.claims[(rc.slot + 2) % 3];
+        long long *claimed = D.claims[rc.slot % 3];
+        long long *cleared = D.claims[(rc.slot + 1) % 3];
+        bool won = false;
+        if (rc.checks > 0) {
+            long long last = priority(D, a, o, rc.checksDispatch);
+            won = checked[a] == last && checked[o] == last;
+        }
+        if (won) {
+            contact(D, D.params, c, a, o, rc.pass, rc.rubs);
+            D.cur.done[c] = rc.pass;
+        } else {
+            cleared[a] = 0;
+            cleared[o] = 0;
+            if (rc.claims > 0) {
+                stillOpen = true;
+                long long mine = priority(D, a, o, rc.dispatch);
+                atomicMax(&claimed[a], mine);
+                atomicMax(&claimed[o], mine);
+            } else {
+                atomicAdd(&D.count[MISSED], 1);
+            }
+        }
+    }
+    // open[mark] = 1 from one lane of each wave that has an open contact, not from every contact.
+    if (__ballot(stillOpen) && (threadIdx.x & 63) == 0) D.open[rc.mark] = 1;
+}
+
+// One contact of a round run until done (the body of Spheres.contactRound, Spheres.java:1088-1131, for a contact
+// still open): it solves if it holds both its spheres' claims from the round before, else clears the spheres' third
+// claim array and claims both. Returns whether it is still open.
+__device__ __forceinline__ bool round_contact(const SphereDev &D, const RoundConst &rc, int c) {
     int a = D.cur.a[c], o = D.cur.b[c];
     const long long *checked = D.claims[(rc.slot + 2) % 3];
     long long *claimed = D.claims[rc.slot % 3];
     long long *cleared = D.claims[(rc.slot + 1) % 3];
-    bool won = false;
     if (rc.checks > 0) {
         long long last = priority(D, a, o, rc.checksDispatch);
-        won = checked[a] == last && checked[o] == last;
-    }
-    if (won) {
-        contact(D, D.params, c, a, o, rc.pass, rc.rubs);
-        D.cur.done[c] = rc.pass;
-    } else {
-        cleared[a] = 0;
-        cleared[o] = 0;
-        if (rc.claims > 0) {
-            D.open[rc.mark] = 1;
-            long long mine = priority(D, a, o, rc.dispatch);
-            atomicMax(&claimed[a], mine);
-            atomicMax(&claimed[o], mine);
-        } else {
-            atomicAdd(&D.count[MISSED], 1);
+        if (checked[a] == last && checked[o] == last) {
+            contact(D, D.params, c, a, o, rc.pass, rc.rubs);
+            D.cur.done[c] = rc.pass;
+            return false;
         }
+    }
+    cleared[a] = 0;
+    cleared[o] = 0;
+    long long mine = priority(D, a, o, rc.dispatch);
+    atomicMax(&claimed[a], mine);
+    atomicMax(&claimed[o], mine);
+    return true;
+}
+
+constexpr int CYCLE = 48, BATCH = 8, SHRINK_AFTER = 120;  // SphereStep.java:48-59, SphereStepper.java:49
+
+__device__ __forceinline__ int position(int k) { return k == 0 ? 0 : 1 + (k - 1) % CYCLE; }
+
+// The constants of the round at cycle position p of pass `it` (SphereStep.java:286-315, roundNames :320-327).
+__device__ __forceinline__ RoundConst cycle_round(int it, int p, bool rub, int iterations) {
+    RoundConst rc;
+    rc.checks = p == 0 ? 0 : 1;
+    rc.claims = 1;
+    rc.pass = rub ? iterations + it + 1 : it + 1;
+    rc.dispatch = it * CYCLE + p % CYCLE;
+    rc.mark = p % 2;
+    rc.checksDispatch = it * CYCLE + max(p - 1, 0) % CYCLE;
+    rc.rubs = rub ? 1 : 0;
+    rc.slot = p % 3;
+    return rc;
+}
+
+// Appends `c` to `out` for every lane with `open`, one atomic a wave.
+__device__ __forceinline__ void append_open(bool open, int c, int32_t *out, int *count) {
+    unsigned long long mask = __ballot(open);
+    if (!mask) return;
+    int lane = threadIdx.x & 63;
+    int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (open) out[base + __popcll(mask & ((1ull << lane) - 1ull))] = c;
+}
+
+__global__ void k_step_begin(PassCtl *ctl, PassSlot *slots, int nslots) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nslots) {
+        slots[q].rd[0] = 0;
+        slots[q].rd[1] = -1;
+        for (int k = 0; k < 4; k++) slots[q].cnt[k] = 0;
+    }
+    if (q == 0) {
+        ctl->error = 0;
+        ctl->waits = 0;
+        ctl->rounds = 0;
+        ctl->most = 0;
+        ctl->unlisted = 0;
+        ctl->listedSum = 0;
+        for (int k = 0; k < 64; k++) ctl->wideRan[k] = 0;
+    }
+}
+
+// Round j of a pass over the whole chip, while at least `T` contacts are open and every round before it ran wide;
+// otherwise nothing, and k_round_tail finishes the pass. The open contacts that do not solve are compacted into
+// round j+1's list; which order they land in does not matter, since a round's outcome is the same in any order.
+__global__ void k_round_wide(SphereDev D, const PassCtl *ctl, PassSlot *slot, OpenLists L, int it, int iterations,
+                             int rub, int j, int T) {
+    if (slot->rd[j & 1] != j) return;
+    int nIn = j == 0 ? min(D.count[LISTED], D.cap) : slot->cnt[j & 3];
+    if (nIn == 0 || nIn < T) return;
+    if (rub && j >= ctl->k) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        slot->rd[(j + 1) & 1] = j + 1;
+        slot->cnt[(j + 2) & 3] = 0;
+    }
+    RoundConst rc = cycle_round(it, position(j), rub != 0, iterations);
+    const int32_t *in = L.buf[j & 1];
+    int32_t *out = L.buf[(j + 1) & 1];
+    int *outCount = &slot->cnt[(j + 1) & 3];
+    for (int base = blockIdx.x * blockDim.x; base < nIn; base += gridDim.x * blockDim.x) {
+        int i = base + threadIdx.x;
+        bool open = false;
+        int c = 0;
+        if (i < nIn) {
+            c = j == 0 ? i : in[i];
+            open = round_contact(D, rc, c);
+        }
+        append_open(open, c, out, outCount);
+    }
+}
+
+// The rest of a pass in one workgroup, rounds separated by barriers rather than launches. A parting pass runs until
+// no contact is open, then settles k, the rounds the reference's stepper would have run: its opening batch, then
+// batches of 8, up to the first report that finds nothing open (SphereStepper.java:72-96); the opening batch then
+// adapts as there (:118-126). A rubbing pass runs the parting pass's k rounds, or until nothing is open.
+constexpr int TAIL = 512;
+
+__global__ void __launch_bounds__(TAIL) k_round_tail(SphereDev D, PassCtl *ctl, const PassSlot *slot, OpenLists L,
+                                                     int32_t *opening, int32_t *enough, int it, int iterations,
+                                                     int rub, int limit) {
+    __shared__ int sCount;
+    int j = max(slot->rd[0], slot->rd[1]);
+    int n = j == 0 ? min(D.count[LISTED], D.cap) : slot->cnt[j & 3];
+    int open0 = rub ? 0 : opening[it];
+    int kRub = rub ? ctl->k : 0;
+    int k = 0;
+    bool failed = false;
+    while (true) {
+        if (!rub) {
+            if (n == 0) {
+                k = j <= open0 ? open0 : open0 + BATCH * ((j - open0 + BATCH - 1) / BATCH);
+                break;
+            }
+            if (j >= open0 && (j - open0) % BATCH == 0 && j > limit) {
+                failed = true;
+                k = j;
+                break;
+            }
+        } else if (j >= kRub || n == 0) {
+            break;
+        }
+        RoundConst rc = cycle_round(it, position(j), rub != 0, iterations);
+        const int32_t *in = L.buf[j & 1];
+        int32_t *out = L.buf[(j + 1) & 1];
+        if (threadIdx.x == 0) sCount = 0;
+        __syncthreads();
+        for (int base = 0; base < n; base += TAIL) {
+            int i = base + threadIdx.x;
+            bool open = false;
+            int c = 0;
+            if (i < n) {
+                c = j == 0 ? i : in[i];
+                open = round_contact(D, rc, c);
+            }
+            append_open(open, c, out, &sCount);
+        }
+        __syncthreads();
+        n = sCount;
+        j++;
+        __syncthreads();
+    }
+    // The rounds this pass needed, for the host to launch that many wide next step.
+    int used = (rub ? iterations : 0) + it;
+    if (threadIdx.x == 0 && used < 64) ctl->wideRan[used] = max(ctl->wideRan[used], j);
+    if (threadIdx.x == 0) ctl->listedSum += min(D.count[LISTED], D.cap);
+    if (threadIdx.x != 0 || rub) return;
+    if (failed) {
+        ctl->error = 1;
+        ctl->errRounds = k;
+        ctl->errListed = D.count[LISTED];
+    }
+    ctl->k = k;
+    ctl->waits += 1 + (k > open0 ? (k - open0) / BATCH : 0);
+    ctl->rounds += k;
+    ctl->most = max(ctl->most, k);
+    ctl->unlisted += max(0, D.count[LISTED] - D.cap);
+    if (k > opening[it]) {  // SphereStepper.adapt
+        opening[it] = min(k, CYCLE);
+        enough[it] = 0;
+    } else if (++enough[it] >= SHRINK_AFTER && opening[it] > BATCH) {
+        opening[it] -= BATCH;
+        enough[it] = 0;
     }
 }
 
@@ -578,8 +760,9 @@ __global__ void k_velocity(SphereDev D) {
 // Spheres.bounce / bounce(grid), Spheres.java:1523-1627.
 template <bool GRID>
 __global__ void k_bounce(SphereDev D) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= D.n) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.n) return;
+    int s = GRID ? D.order[t] : t;
     const float *P = D.params;
     float e = P[RESTITUTION];
     float wa = D.im[s];
@@ -640,8 +823,9 @@ __global__ void k_bounced(SphereDev D) {
 // Spheres.solve / solve(grid) with Solving, Spheres.java:302-347, 1245-1298: the Jacobi move into cx, cy, cz.
 template <bool GRID>
 __global__ void k_jacobi_solve(SphereDev D) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= D.n) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.n) return;
+    int s = GRID ? D.order[t] : t;
     float mine[3] = {D.x[s], D.y[s], D.z[s]};
     float rs = D.r[s], ws = D.im[s];
     float sum[3] = {0, 0, 0}, contacts = 0;
@@ -696,8 +880,9 @@ __global__ void k_jacobi_apply(SphereDev D) {
 // Spheres.show / show(grid), Spheres.java:1697-1794.
 template <bool GRID>
 __global__ void k_show(SphereDev D) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= D.n) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.n) return;
+    int s = GRID ? D.order[t] : t;
     const float *P = D.params;
     float ext[3] = {P[SX], P[SY], P[SZ]};
     float *b = D.shown + (size_t)s * SHOWN_STRIDE;
@@ -1012,6 +1197,22 @@ void launch_list(const SphereDev &D, const SortScratch &S, hipStream_t s) {
 void launch_round(const SphereDev &D, const RoundConst &rc, hipStream_t s) {
     k_round<<<blocks_for(D.cap), THREADS, 0, s>>>(D, rc);
     check_launch("contact round");
+}
+
+void launch_step_begin(PassCtl *ctl, PassSlot *slots, int nslots, hipStream_t s) {
+    k_step_begin<<<blocks_for(nslots), THREADS, 0, s>>>(ctl, slots, nslots);
+    check_launch("step begin");
+}
+
+void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLists &L, int32_t *opening,
+                 int32_t *enough, int it, int iterations, bool rub, int wide, hipStream_t s) {
+    constexpr int T = 1;  // every round with an open contact runs wide; the tail finishes what `wide` leaves
+    int blocks = std::min(blocks_for(D.cap), 2048);
+    for (int j = 0; j < wide; j++)
+        k_round_wide<<<blocks, THREADS, 0, s>>>(D, ctl, slot, L, it, iterations, rub ? 1 : 0, j, T);
+    int limit = D.cap + 2 * CYCLE;
+    k_round_tail<<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, rub ? 1 : 0, limit);
+    check_launch("pass");
 }
 
 void launch_walls(const SphereDev &D, hipStream_t s) {

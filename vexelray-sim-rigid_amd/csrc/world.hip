@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -60,7 +61,19 @@ struct vr_world {
     int par = 0;  // lists[par] is this substep's list, lists[1 - par] the one kept from the substep before
     int32_t *control = nullptr;  // device: contactCount[3], open[2]
     int32_t *pinned = nullptr;   // host copy of control
-    std::vector<int> opening, enough;
+    std::vector<int> opening, enough;  // the host stepper's (fixed-rounds and reference-wait modes)
+    PassCtl *ctl = nullptr;            // device: the passes' shared control words and the step's report
+    PassSlot *slots = nullptr;         // device: one per pass of a step
+    int nslots = 0;
+    PassCtl *ctlHost = nullptr;        // pinned copy read after a step
+    OpenLists openLists{};
+    int32_t *devOpening = nullptr, *devEnough = nullptr;  // SphereStepper's opening batches, kept on the device
+    std::vector<int> wide, wideRub;    // wide rounds to launch per pass, from what the last step used
+    bool hostWaits = false;            // run the reference's launch model: host reads each report
+    bool profiling = false;
+    vr_profile prof{};
+    std::vector<hipEvent_t> passEvents;  // a pair a pass, recorded around its round kernels when profiling
+    int launchesThisStep = 0;
 
     template <class T>
     T *alloc(int64_t count) {
@@ -93,6 +106,9 @@ struct vr_world {
         if (hipGetDeviceCount(&devices) != hipSuccess || devices < 1) throw DeviceError("no HIP device");
         n = d.bodies;
         cap = n * d.contacts_per_sphere;
+        // VR_HOST_WAITS=1 runs a Gauss-Seidel pass as the reference's stepper does, the host reading each report.
+        const char *hw = std::getenv("VR_HOST_WAITS");
+        hostWaits = hw && hw[0] == '1';
         HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIP_OK(hipEventCreate(&t0));
         HIP_OK(hipEventCreate(&t1));
@@ -159,6 +175,21 @@ struct vr_world {
         }
         opening.assign(d.iterations, BATCH);
         enough.assign(d.iterations, 0);
+        if (gs) {
+            ctl = alloc<PassCtl>(1);
+            nslots = 2 * d.substeps * d.iterations + 2;
+            slots = alloc<PassSlot>(nslots);
+            HIP_OK(hipHostMalloc((void **)&ctlHost, sizeof(PassCtl), hipHostMallocDefault));
+            openLists.buf[0] = alloc<int32_t>(cap);
+            openLists.buf[1] = alloc<int32_t>(cap);
+            devOpening = alloc<int32_t>(d.iterations);
+            devEnough = alloc<int32_t>(d.iterations);
+            HIP_OK(hipMemcpy(devOpening, opening.data(), sizeof(int32_t) * d.iterations, hipMemcpyHostToDevice));
+            HIP_OK(hipMemset(devEnough, 0, sizeof(int32_t) * d.iterations));
+            HIP_OK(hipMemset(ctl, 0, sizeof(PassCtl)));
+            wide.assign(d.iterations, 4);
+            wideRub.assign(d.iterations, 4);
+        }
         set_lists();
         clear();
     }
@@ -292,6 +323,78 @@ struct vr_world {
         }
     }
 
+    void pass(int q, int it, bool rub, int wideRounds) {
+        if (profiling) {
+            while ((int)passEvents.size() < 2 * (q + 1)) {
+                hipEvent_t e;
+                HIP_OK(hipEventCreate(&e));
+                passEvents.push_back(e);
+            }
+            HIP_OK(hipEventRecord(passEvents[2 * q], stream));
+        }
+        launch_pass(D, ctl, slots + q, openLists, devOpening, devEnough, it, d.iterations, rub, wideRounds, stream);
+        launchesThisStep += wideRounds + 1;
+        if (profiling) HIP_OK(hipEventRecord(passEvents[2 * q + 1], stream));
+    }
+
+    // SphereStepper.step with every decision on the device: each pass is launched as a fixed sequence (a few wide
+    // rounds, then the tail that finishes it), so the step runs with no host wait; the rounds run and the report
+    // are the reference's.
+    void step_device(vr_step_report *rep) {
+        launch_step_begin(ctl, slots, nslots, stream);
+        int q = 0;
+        launchesThisStep = 0;
+        for (int sub = 0; sub < d.substeps; sub++) {
+            for (int it = 0; it < d.iterations; it++) {
+                if (friction && !(sub == 0 && it == 0)) {
+                    int prev = it == 0 ? d.iterations - 1 : it - 1;
+                    pass(q++, prev, true, wideRub[prev]);
+                }
+                if (it > 0) {
+                    launch_walls(D, stream);
+                } else {
+                    if (sub > 0) {
+                        launch_walls(D, stream);
+                        velocity_list();
+                    }
+                    predict();
+                    prepare();
+                }
+                pass(q++, it, false, wide[it]);
+            }
+        }
+        if (friction)
+            pass(q++, d.iterations - 1, true, wideRub[d.iterations - 1]);
+        launch_walls(D, stream);
+        velocity_list();
+        launch_show(D, grid, stream);
+        HIP_OK(hipMemcpyAsync(ctlHost, ctl, sizeof(PassCtl), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        if (ctlHost->error)
+            throw StateError("a pass is still open after " + std::to_string(ctlHost->errRounds) +
+                             " rounds, on a list of " + std::to_string(ctlHost->errListed));
+        if (profiling) {
+            for (int k = 0; k < q; k++) {
+                float ms = 0;
+                HIP_OK(hipEventElapsedTime(&ms, passEvents[2 * k], passEvents[2 * k + 1]));
+                prof.round_ns += (int64_t)((double)ms * 1e6);
+            }
+            prof.round_launches += launchesThisStep;
+            prof.passes += q;
+            prof.listed_sum += ctlHost->listedSum;
+            prof.steps += 1;
+        }
+        rep->waits = ctlHost->waits;
+        rep->rounds = ctlHost->rounds;
+        rep->most_rounds = ctlHost->most;
+        rep->unlisted = ctlHost->unlisted;
+        // Wide rounds for the next step: as many as ran wide in this one, and two more.
+        for (int it = 0; it < d.iterations && it < 32; it++) {
+            wide[it] = std::min(ctlHost->wideRan[it] + 1, 64);
+            wideRub[it] = std::min(ctlHost->wideRan[d.iterations + it] + 1, 64);
+        }
+    }
+
     void step_until_done(vr_step_report *rep) {  // SphereStepper.java:66-103
         int waits = 0, rounds = 0, most = 0, unlisted = 0, rubbed = 0;
         int limit = cap + 2 * CYCLE;
@@ -376,8 +479,12 @@ struct vr_world {
         std::memset(r, 0, sizeof(*r));
         auto w0 = std::chrono::steady_clock::now();
         HIP_OK(hipEventRecord(t0, stream));
-        if (gs && !d.fixed_rounds) step_until_done(r);
-        else step_list();
+        if (gs && !d.fixed_rounds) {
+            if (hostWaits) step_until_done(r);
+            else step_device(r);
+        } else {
+            step_list();
+        }
         HIP_OK(hipEventRecord(t1, stream));
         if (wait) {
             HIP_OK(hipStreamSynchronize(stream));
@@ -393,6 +500,8 @@ struct vr_world {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : allocs) (void)hipFree(p);
         if (pinned) (void)hipHostFree(pinned);
+        if (ctlHost) (void)hipHostFree(ctlHost);
+        for (hipEvent_t e : passEvents) (void)hipEventDestroy(e);
         if (t0) (void)hipEventDestroy(t0);
         if (t1) (void)hipEventDestroy(t1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -486,6 +595,19 @@ int vr_world_finish(vr_world *w) {
 
 int vr_world_destroy(vr_world *w) {
     delete w;
+    return VR_OK;
+}
+
+int vr_world_profile(vr_world *w, int32_t on) {
+    if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    w->profiling = on != 0;
+    w->prof = vr_profile{};
+    return VR_OK;
+}
+
+int vr_world_profile_read(vr_world *w, vr_profile *out) {
+    if (!w || !out) return fail(VR_ERR_ARGUMENT, "null argument");
+    *out = w->prof;
     return VR_OK;
 }
 
