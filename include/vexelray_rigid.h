@@ -12,16 +12,21 @@
  *   vr_world_create   SphereStep(spheres, substeps, iterations, grid, solve, rounds, contactsPerSphere)
  *                     (core/.../sphere/SphereStep.java:115-209) plus the runner it is registered on
  *                     (gui/.../SphereSimulation.java:70-97 -> PassRunner.gpu(...)); SphereGrid(nx, ny, nz, cell)
- *                     (core/.../sphere/SphereGrid.java:11-19).
+ *                     (core/.../sphere/SphereGrid.java:11-19). With VR_BOXES, BoxStep(boxes, substeps, sweeps,
+ *                     symmetric, grid) (core/.../box/BoxStep.java:81-162) and BoxSimulation's runner
+ *                     (gui/.../BoxSimulation.java:34-75).
  *   vr_world_clear    PassRunner.clear(), then SphereStep.constants() written (SphereSimulation.java:154-166,
  *                     SphereStep.java:157-160, 224-226, 419).
  *   vr_world_write    PassRunner.write(name, int[]|float[]) (ctest/.../sphere/Scene.java:127-140).
  *   vr_world_read     PassRunner.floats(name) / read(name) (Scene.java:176-204; SphereSimulation.state() :250-254).
  *   vr_world_params   SphereSimulation.params(int[]) (SphereSimulation.java:182-185): 13 words of raw float bits
- *                     laid out as Spheres.params (core/.../sphere/Spheres.java:173-226).
+ *                     laid out as Spheres.params (core/.../sphere/Spheres.java:173-226); for boxes 18 words laid
+ *                     out as Boxes.params (core/.../box/Boxes.java:212-234), BoxSimulation.params.
  *   vr_world_friction SphereStepper.friction(boolean) (core/.../sphere/SphereStepper.java:110-112).
  *   vr_world_step     SphereStepper.step(Runner) -> Report (SphereStepper.java:66-103) as driven by
- *                     SphereSimulation.step()/stepAndWait() (SphereSimulation.java:190-229).
+ *                     SphereSimulation.step()/stepAndWait() (SphereSimulation.java:190-229); for boxes
+ *                     BoxStepper.step(Runner) -> Report (core/.../box/BoxStepper.java:56-100) as driven by
+ *                     BoxSimulation.stepAndWait() (gui/.../BoxSimulation.java:116-142).
  *   vr_world_finish   SphereSimulation.finish() / PassRunner.finish() (SphereSimulation.java:243-245).
  *   vr_world_destroy  SphereSimulation.close() (SphereSimulation.java:287-295).
  *   vr_last_error     the exception message a Java caller would have seen (IllegalArgumentException for
@@ -47,6 +52,7 @@ typedef struct vr_world vr_world;
 
 /* World kinds. */
 #define VR_SPHERES 1
+#define VR_BOXES 2
 
 /* SphereStep.Solve (SphereStep.java:37-43). */
 #define VR_JACOBI 0
@@ -80,6 +86,10 @@ typedef struct vr_step_desc {
      * the priorities differ, so the colouring does.
      */
     int32_t wide_pairs;
+    /* Boxes (BoxStep(boxes, substeps, sweeps, symmetric, grid), core/.../box/BoxStep.java:81-162): sweeps of the
+     * solve and of the relax a substep makes, and whether every other sweep runs backwards. */
+    int32_t sweeps;
+    int32_t symmetric;
 } vr_step_desc;
 
 /*
@@ -92,6 +102,11 @@ typedef struct vr_step_report {
     int32_t rounds;
     int32_t most_rounds;
     int32_t unlisted;
+    /* Boxes: BoxStepper.Report(touching, points, rounds, unplaced, unlisted) (core/.../box/BoxStepper.java:40). */
+    int32_t touching;
+    int32_t points;
+    int32_t unplaced;
+    int32_t reserved;
     int64_t wall_ns;
     int64_t gpu_ns;
 } vr_step_report;
@@ -122,6 +137,13 @@ int vr_world_step(vr_world *w, int32_t wait, vr_step_report *report);
 int vr_world_finish(vr_world *w);
 int vr_world_destroy(vr_world *w);
 int vr_last_error(char *buf, int32_t len);
+/*
+ * The narrow phase alone: BoxContact.collide (core/.../box/BoxContact.java:136-246) over `npairs` pairs of a box
+ * world's bodies as they are now, pairs[2k] first and pairs[2k+1] second, writing 40 floats a pair to `manifold`
+ * (normal, count, axis, up to 4 points of anchor on the first, anchor on the second, depth). What the reference's
+ * tests run as a collide-only program (ctest/.../box/BoxContactTest.java:441-505).
+ */
+int vr_box_collide(vr_world *w, const int32_t *pairs, int32_t npairs, float *manifold);
 /* Switches pass timing on or off and zeroes its counters. */
 int vr_world_profile(vr_world *w, int32_t on);
 int vr_world_profile_read(vr_world *w, vr_profile *out);

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "../include/vexelray_rigid.h"
+#include "box_oracle.h"
 
 namespace {
 
@@ -1098,15 +1099,52 @@ int guard(F fn) {
 }  // namespace
 
 struct vro_world {
+    int kind = VR_SPHERES;
     World w;
+    vro::BoxWorld *box = nullptr;
+    ~vro_world() {
+        if (box) vro::box_destroy(box);
+    }
 };
+
+namespace {
+
+bool find_buffer(vro_world *p, const std::string &s, void **ptr, int64_t *count, int32_t *dtype) {
+    if (p->kind == VR_BOXES) return vro::box_buffer(p->box, s, ptr, count, dtype);
+    if (p->w.F.count(s)) {
+        *ptr = p->w.F[s].data();
+        *count = (int64_t)p->w.F[s].size();
+        *dtype = VR_F32;
+        return true;
+    }
+    if (p->w.I.count(s)) {
+        *ptr = p->w.I[s].data();
+        *count = (int64_t)p->w.I[s].size();
+        *dtype = VR_I32;
+        return true;
+    }
+    return false;
+}
+
+}  // namespace
 
 extern "C" {
 
 int vro_world_create(int32_t kind, const vr_step_desc *desc, vro_world **out) {
-    if (kind != VR_SPHERES || !desc || !out) return fail(VR_ERR_ARGUMENT, "unknown world kind");
+    if ((kind != VR_SPHERES && kind != VR_BOXES) || !desc || !out) return fail(VR_ERR_ARGUMENT, "unknown world kind");
     auto *p = new vro_world();
-    int rc = guard([&] { p->w.build(*desc); });
+    p->kind = kind;
+    int rc = guard([&] {
+        if (kind == VR_BOXES) {
+            try {
+                p->box = vro::box_create(*desc);
+            } catch (const std::invalid_argument &e) {
+                throw ArgError(e.what());
+            }
+        } else {
+            p->w.build(*desc);
+        }
+    });
     if (rc != VR_OK) {
         delete p;
         return rc;
@@ -1116,61 +1154,48 @@ int vro_world_create(int32_t kind, const vr_step_desc *desc, vro_world **out) {
 }
 
 int vro_world_clear(vro_world *p) {
-    return guard([&] { p->w.clear(); });
+    return guard([&] {
+        if (p->kind == VR_BOXES) vro::box_clear(p->box);
+        else p->w.clear();
+    });
 }
 
 int vro_world_buffer_info(vro_world *p, const char *name, int64_t *count, int32_t *dtype) {
-    std::string s(name);
-    if (p->w.F.count(s)) {
-        *count = (int64_t)p->w.F[s].size();
-        *dtype = VR_F32;
-        return VR_OK;
-    }
-    if (p->w.I.count(s)) {
-        *count = (int64_t)p->w.I[s].size();
-        *dtype = VR_I32;
-        return VR_OK;
-    }
-    return fail(VR_ERR_NAME, "no buffer named " + s);
+    void *ptr;
+    if (find_buffer(p, name, &ptr, count, dtype)) return VR_OK;
+    return fail(VR_ERR_NAME, std::string("no buffer named ") + name);
 }
 
 int vro_world_write(vro_world *p, const char *name, const void *src, int64_t count) {
-    std::string s(name);
-    void *dst = nullptr;
-    size_t size = 0;
-    if (p->w.F.count(s)) {
-        dst = p->w.F[s].data();
-        size = p->w.F[s].size();
-    } else if (p->w.I.count(s)) {
-        dst = p->w.I[s].data();
-        size = p->w.I[s].size();
-    } else {
-        return fail(VR_ERR_NAME, "no buffer named " + s);
-    }
-    if (count < 0 || (size_t)count > size) return fail(VR_ERR_NAME, "buffer " + s + " is shorter than the write");
+    void *dst;
+    int64_t size;
+    int32_t dtype;
+    if (!find_buffer(p, name, &dst, &size, &dtype)) return fail(VR_ERR_NAME, std::string("no buffer named ") + name);
+    if (count < 0 || count > size) return fail(VR_ERR_NAME, std::string("buffer ") + name + " is shorter than the write");
     std::memcpy(dst, src, (size_t)count * 4);
     return VR_OK;
 }
 
 int vro_world_read(vro_world *p, const char *name, void *dst, int64_t count) {
-    std::string s(name);
-    const void *src = nullptr;
-    size_t size = 0;
-    if (p->w.F.count(s)) {
-        src = p->w.F[s].data();
-        size = p->w.F[s].size();
-    } else if (p->w.I.count(s)) {
-        src = p->w.I[s].data();
-        size = p->w.I[s].size();
-    } else {
-        return fail(VR_ERR_NAME, "no buffer named " + s);
-    }
-    if (count < 0 || (size_t)count > size) return fail(VR_ERR_NAME, "buffer " + s + " is shorter than the read");
+    void *src;
+    int64_t size;
+    int32_t dtype;
+    if (!find_buffer(p, name, &src, &size, &dtype)) return fail(VR_ERR_NAME, std::string("no buffer named ") + name);
+    if (count < 0 || count > size) return fail(VR_ERR_NAME, std::string("buffer ") + name + " is shorter than the read");
     std::memcpy(dst, src, (size_t)count * 4);
     return VR_OK;
 }
 
 int vro_world_params(vro_world *p, const int32_t *words, int32_t n) {
+    if (p->kind == VR_BOXES) {
+        return guard([&] {
+            try {
+                vro::box_params(p->box, words, n);
+            } catch (const std::invalid_argument &e) {
+                throw ArgError(e.what());
+            }
+        });
+    }
     if (n != PARAM_COUNT) return fail(VR_ERR_ARGUMENT, "sphere parameters are 13 words");
     std::memcpy(p->w.F["params"].data(), words, sizeof(int32_t) * PARAM_COUNT);
     float *P = p->w.F["params"].data();
@@ -1185,6 +1210,17 @@ int vro_world_friction(vro_world *p, int32_t on) {
 
 int vro_world_step(vro_world *p, int32_t wait, vr_step_report *rep) {
     (void)wait;
+    if (p->kind == VR_BOXES) {
+        return guard([&] {
+            vr_step_report local{};
+            vr_step_report *r = rep ? rep : &local;
+            std::memset(r, 0, sizeof(*r));
+            auto t0 = std::chrono::steady_clock::now();
+            vro::box_step(p->box, r);
+            r->wall_ns = std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0)
+                             .count();
+        });
+    }
     return guard([&] { p->w.step(rep); });
 }
 
@@ -1199,6 +1235,12 @@ int vro_last_error(char *buf, int32_t len) {
     if (len <= 0) return VR_OK;
     std::strncpy(buf, g_error.c_str(), (size_t)len - 1);
     buf[len - 1] = 0;
+    return VR_OK;
+}
+
+int vro_box_collide(vro_world *p, const int32_t *pairs, int32_t npairs, float *manifold) {
+    if (p->kind != VR_BOXES) return fail(VR_ERR_STATE, "not a box world");
+    vro::box_collide(p->box, pairs, npairs, manifold);
     return VR_OK;
 }
 

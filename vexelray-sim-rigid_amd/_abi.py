@@ -18,13 +18,14 @@ VR_ERR_STATE = -2
 VR_ERR_DEVICE = -3
 VR_ERR_NAME = -4
 VR_SPHERES = 1
+VR_BOXES = 2
 VR_F32 = 1
 VR_I32 = 2
 
 # Every entry point the header declares, for the export check.
 ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "world_buffer_info", "world_params",
                 "world_friction", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
-                "world_profile_read")
+                "world_profile_read", "box_collide")
 
 
 class StepDesc(ctypes.Structure):
@@ -32,12 +33,15 @@ class StepDesc(ctypes.Structure):
                 ("has_grid", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
                 ("nz", ctypes.c_int32), ("cell", ctypes.c_float), ("solve", ctypes.c_int32),
                 ("rounds", ctypes.c_int32), ("contacts_per_sphere", ctypes.c_int32),
-                ("fixed_rounds", ctypes.c_int32), ("wide_pairs", ctypes.c_int32)]
+                ("fixed_rounds", ctypes.c_int32), ("wide_pairs", ctypes.c_int32), ("sweeps", ctypes.c_int32),
+                ("symmetric", ctypes.c_int32)]
 
 
 class StepReport(ctypes.Structure):
     _fields_ = [("waits", ctypes.c_int32), ("rounds", ctypes.c_int32), ("most_rounds", ctypes.c_int32),
-                ("unlisted", ctypes.c_int32), ("wall_ns", ctypes.c_int64), ("gpu_ns", ctypes.c_int64)]
+                ("unlisted", ctypes.c_int32), ("touching", ctypes.c_int32), ("points", ctypes.c_int32),
+                ("unplaced", ctypes.c_int32), ("reserved", ctypes.c_int32), ("wall_ns", ctypes.c_int64),
+                ("gpu_ns", ctypes.c_int64)]
 
 
 class Profile(ctypes.Structure):
@@ -71,6 +75,7 @@ class Abi:
             "last_error": ([ctypes.c_char_p, ctypes.c_int32]),
             "world_profile": ([p, ctypes.c_int32]),
             "world_profile_read": ([p, ctypes.POINTER(Profile)]),
+            "box_collide": ([p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
         }
         for name, args in sig.items():
             fn = getattr(self.lib, prefix + name)
@@ -168,6 +173,13 @@ class World:
     def profile_read(self) -> Profile:
         out = Profile()
         self._check(self.abi.fn("world_profile_read")(self.handle, ctypes.byref(out)))
+        return out
+
+    def collide(self, pairs) -> np.ndarray:
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+        out = np.zeros((pairs.size // 2) * 40, dtype=np.float32)
+        self._check(self.abi.fn("box_collide")(self.handle, pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                               pairs.size // 2, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
 
     def destroy(self) -> None:
