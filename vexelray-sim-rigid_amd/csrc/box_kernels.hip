@@ -1,0 +1,472 @@
+// box_kernels.hip — the box world's kernels for gfx950: one per pass of the reference's box step
+// (vexelray-sim-rigid-core/.../box/BoxStep.java:138-153, 199-231), over structure-of-arrays body state and the
+// reference's per-slot pair layout (32 slots a body, BoxBroad.java:46).
+#include "box_kernels.h"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "box_math.h"
+
+namespace vrbox {
+
+namespace {
+
+constexpr int THREADS = 256;
+
+inline int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, (n + THREADS - 1) / THREADS); }
+
+inline void check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+__device__ __forceinline__ Bufs bufs(const BoxDev &D) {
+    Bufs B;
+    for (int a = 0; a < 3; a++) {
+        B.x[a] = D.x[a];
+        B.vel[a] = D.vel[a];
+        B.l[a] = D.l[a];
+        B.half[a] = D.half[a];
+    }
+    for (int k = 0; k < 4; k++) B.q[k] = D.q[k];
+    B.im = D.im;
+    B.shape = D.shape;
+    B.params = D.params;
+    B.lambda = D.lambda;
+    B.rub = D.rub;
+    B.roll = D.roll;
+    return B;
+}
+
+// Boxes.velocities, Boxes.java:276-289.
+__global__ void k_velocities(BoxDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n || !(D.im[s] > 0)) return;
+    const float *P = D.params;
+    float h = P[H];
+    float g[3] = {P[GX], P[GY], P[GZ]};
+    for (int a = 0; a < 3; a++) D.vel[a][s] = D.vel[a][s] + h * g[a];
+}
+
+// Boxes.positions, Boxes.java:299-342: the midpoint turn, with the angular momentum as the state.
+__global__ void k_positions(BoxDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n || !(D.im[s] > 0)) return;
+    Bufs B = bufs(D);
+    float h = D.params[H];
+    for (int a = 0; a < 3; a++) B.x[a][s] = B.x[a][s] + h * B.vel[a][s];
+    Moving m = load_moving(B, s);
+    float cq[4], own[3], omega[3];
+    conjugate(m.q, cq);
+    rotate(cq, m.mom, own);
+    for (int a = 0; a < 3; a++) own[a] = m.inverse[a] * own[a];
+    rotate(m.q, own, omega);
+    float halfH = 0.5f * h;
+    float mid[4], omegaMid[3], end[4], cm[4];
+    turned(m.q, omega, halfH, mid);
+    conjugate(mid, cm);
+    rotate(cm, m.mom, own);
+    for (int a = 0; a < 3; a++) own[a] = m.inverse[a] * own[a];
+    rotate(mid, own, omegaMid);
+    turned(m.q, omegaMid, h, end);
+    for (int k = 0; k < 4; k++) B.q[k][s] = end[k];
+}
+
+// Boxes.walls, Boxes.java:357-416: one sweep of a body's corners (a sphere's nearest point) against the six walls.
+template <int SWEEP, bool REVERSE>
+__global__ void k_walls(BoxDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n || !(D.im[s] > 0)) return;
+    Bufs B = bufs(D);
+    const float *P = D.params;
+    float extent[3] = {P[SX], P[SY], P[SZ]};
+    Moving box = load_moving(B, s);
+    Soft soft(P, true);
+    size_t base = (size_t)s * LAMBDA_WORDS;
+    const Sweep sweep = (Sweep)SWEEP;
+    for (int order = 0; order < WALLS; order++) {
+        int wall = REVERSE ? WALLS - 1 - order : order;
+        int axis = wall / 2;
+        bool high = wall % 2 == 1;
+        int firstSlot = wall * CORNERS;
+        float nrm[3];
+        for (int a = 0; a < 3; a++) nrm[a] = a == axis ? (high ? -1.0f : 1.0f) : 0.0f;
+        if (box.sphere()) {
+            float r[3];
+            for (int a = 0; a < 3; a++) r[a] = -(box.half[0] * nrm[a]);
+            size_t at = base + firstSlot;
+            size_t tangentAt = at * 3;
+            float reach = box.at[axis] + r[axis];
+            float gap = high ? extent[axis] - reach : reach;
+            contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, soft);
+            roll(sweep, nullptr, box, box.half[0], B.lambda, at, B.roll, tangentAt, P);
+        } else {
+            for (int k = 0; k < CORNERS; k++) {
+                int c = REVERSE ? CORNERS - 1 - k : k;
+                float offset[3], r[3];
+                for (int a = 0; a < 3; a++) {
+                    int bit = (c / (1 << a)) % 2;
+                    offset[a] = box.half[a] * (2.0f * (float)bit - 1.0f);
+                }
+                rotate(box.q, offset, r);
+                size_t at = base + firstSlot + c;
+                size_t tangentAt = at * 3;
+                float reach = box.at[axis] + r[axis];
+                float gap = high ? extent[axis] - reach : reach;
+                contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, soft);
+            }
+        }
+    }
+    store_moving(B, box);
+}
+
+// BoxPairs.solve, BoxPairs.java:64-151: the pairs of round r, one thread a place, each its manifold in turn.
+template <int SWEEP, bool REVERSE>
+__global__ void k_solve_round(BoxDev D, int r) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= D.width) return;
+    int pair = D.rounds[(size_t)r * D.width + k];
+    if (pair < 0) return;
+    size_t out = (size_t)pair * MANIFOLD_WORDS;
+    int count = (int)D.manifold[out + COUNT];
+    if (!(count > 0)) return;
+    Bufs B = bufs(D);
+    const float *P = D.params;
+    const Sweep sweep = (Sweep)SWEEP;
+    int ia = D.pairs[2 * (size_t)pair], ib = D.pairs[2 * (size_t)pair + 1];
+    float imA = B.im[ia], imB = B.im[ib];
+    Moving first = load_moving(B, ia), second = load_moving(B, ib);
+    Soft soft(P, false);
+    if (imA == 0 || imB == 0) soft.stiffen(P);
+    float nrm[3] = {D.manifold[out], D.manifold[out + 1], D.manifold[out + 2]};
+    size_t held = (size_t)pair * IMPULSE_WORDS;
+    for (int order = 0; order < MOST_POINTS; order++) {
+        int slot = REVERSE ? MOST_POINTS - 1 - order : order;
+        if (!(slot < count)) continue;
+        size_t where = out + POINTS + slot * POINT_WORDS;
+        float rA[3], rB[3];
+        rotate(first.q, D.manifold + where + ANCHOR_FIRST, rA);
+        rotate(second.q, D.manifold + where + ANCHOR_SECOND, rB);
+        if (first.sphere())
+            for (int a = 0; a < 3; a++) rA[a] = first.half[0] * nrm[a];
+        if (second.sphere())
+            for (int a = 0; a < 3; a++) rB[a] = -(second.half[0] * nrm[a]);
+        float gap = ((second.at[0] + rB[0]) - (first.at[0] + rA[0])) * nrm[0] +
+                    ((second.at[1] + rB[1]) - (first.at[1] + rA[1])) * nrm[1];
+        gap = gap + ((second.at[2] + rB[2]) - (first.at[2] + rA[2])) * nrm[2];
+        size_t normalAt = held + POINT_IMPULSE * slot;
+        contact(sweep, gap, nrm, &first, rA, second, rB, D.impulses, normalAt, D.impulses, normalAt + 1, soft);
+        if (first.sphere() || second.sphere()) {
+            float radius = second.half[0];
+            if (first.sphere()) radius = first.half[0];
+            if (first.sphere() && second.sphere())
+                radius = (first.half[0] * second.half[0]) / (first.half[0] + second.half[0]);
+            roll(sweep, &first, second, radius, D.impulses, normalAt, D.impulses, normalAt + 4, P);
+        }
+    }
+    store_moving(B, first);
+    store_moving(B, second);
+}
+
+// BoxBroad.bin, BoxBroad.java:74-89.
+__global__ void k_bin(BoxDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    int cellsAxis[3] = {D.nx, D.ny, D.nz};
+    int c[3];
+    for (int a = 0; a < 3; a++) {
+        float fl = floorf(D.x[a][s] / D.cell);
+        c[a] = (int)mx(0.f, mn((float)(cellsAxis[a] - 1), fl));
+    }
+    D.keys[s] = c[0] + D.nx * (c[1] + D.ny * c[2]);
+}
+
+__device__ __forceinline__ void reach_of(const BoxDev &D, int s, float *out) {  // BoxBroad.reach, :196-227
+    float x = D.q[0][s], y = D.q[1][s], z = D.q[2][s], w = D.q[3][s];
+    float r[3][3] = {{1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y - z * w), 2.0f * (x * z + y * w)},
+                     {2.0f * (x * y + z * w), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z - x * w)},
+                     {2.0f * (x * z - y * w), 2.0f * (y * z + x * w), 1.0f - 2.0f * (x * x + y * y)}};
+    float h[3] = {D.half[0][s], D.half[1][s], D.half[2][s]};
+    for (int a = 0; a < 3; a++)
+        out[a] = (h[0] * fabsf(r[a][0]) + h[1] * fabsf(r[a][1])) + h[2] * fabsf(r[a][2]);
+    if (D.shape[s] > 0.5f)
+        for (int a = 0; a < 3; a++) out[a] = h[0];
+}
+
+// BoxBroad.list, BoxBroad.java:103-164: a body's partners of higher index, in ascending order, into its 32 slots.
+__global__ void k_list(BoxDev D) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.n) return;
+    int s = D.order[t];  // threads take bodies in cell order, so a wave walks neighbouring cells together
+    size_t base = (size_t)s * SLOTS;
+    int part[SLOTS];
+    float margin = D.params[MARGIN];
+    float centre[3] = {D.x[0][s], D.x[1][s], D.x[2][s]};
+    float rs[3];
+    reach_of(D, s, rs);
+    int found = 0;
+    int key = D.keys[s];
+    int ix = key % D.nx, iy = (key / D.nx) % D.ny, iz = key / (D.nx * D.ny);
+    for (int m = 0; m < 27; m++) {
+        int jx = ix + (m % 3 - 1), jy = iy + ((m / 3) % 3 - 1), jz = iz + (m / 9 - 1);
+        if (jx < 0 || jx >= D.nx || jy < 0 || jy >= D.ny || jz < 0 || jz >= D.nz) continue;
+        int cell = jx + D.nx * (jy + D.ny * jz);
+        int end = D.starts[cell + 1];
+        for (int k = D.starts[cell]; k < end; k++) {
+            int o = D.order[k];
+            if (!(o > s)) continue;
+            float other[3];
+            reach_of(D, o, other);
+            bool close = true;
+            for (int a = 0; a < 3; a++)
+                close = close && !(fabsf(D.x[a][o] - centre[a]) > (rs[a] + other[a]) + margin);
+            if (!close) continue;
+            if (found < SLOTS) {
+                int place = found;
+                while (place > 0 && part[place - 1] > o) {
+                    part[place] = part[place - 1];
+                    place--;
+                }
+                part[place] = o;
+            }
+            found++;
+        }
+    }
+    for (int m = 0; m < SLOTS; m++) {
+        D.pairs[(base + m) * 2] = s;
+        D.pairs[(base + m) * 2 + 1] = m < found ? part[m] : -1;
+    }
+    D.contacts[2 * (size_t)D.slots + s] = found;
+}
+
+// BoxContact.collide, BoxContact.java:136-246, one thread a slot.
+__device__ void collide_pair(const BoxDev &D, int ia, int ib, float *m) {
+    Bufs B = bufs(D);
+    m[COUNT] = 0;
+    Solid first = load_solid(B, ia), second = load_solid(B, ib);
+    float shapeA = B.shape[ia], shapeB = B.shape[ib];
+    float margin = B.params[MARGIN];
+    float dd[3] = {second.centre[0] - first.centre[0], second.centre[1] - first.centre[1],
+                   second.centre[2] - first.centre[2]};
+    float parted = -1e30f;
+    float bestFace[2] = {-1e30f, -1e30f}, bestFaceAxis[2] = {0, 0};
+    for (int side = 0; side < 2; side++) {
+        const Solid &own = side == 0 ? first : second;
+        const Solid &other = side == 0 ? second : first;
+        for (int k = 0; k < 3; k++) {
+            float rch = own.half[k];
+            for (int j = 0; j < 3; j++) rch = rch + other.half[j] * fabsf(dot(own.axes[k], other.axes[j]));
+            float sep = fabsf(dot(dd, own.axes[k])) - rch;
+            parted = mx(parted, sep);
+            if (sep > bestFace[side]) {
+                bestFace[side] = sep;
+                bestFaceAxis[side] = (float)k;
+            }
+        }
+    }
+    float bestEdge = -1e30f;
+    Edge edge;
+    for (int k = 0; k < 3; k++) {
+        for (int j = 0; j < 3; j++) {
+            float across[3];
+            cross(first.axes[k], second.axes[j], across);
+            float length2 = dot(across, across);
+            if (!(length2 > PARALLEL)) continue;
+            float length = sqrtf(length2);
+            float inv = 1.0f / length;
+            float ax[3] = {across[0] * inv, across[1] * inv, across[2] * inv};
+            float rch = 0;
+            for (int mm = 0; mm < 3; mm++)
+                rch = rch + (first.half[mm] * fabsf(dot(first.axes[mm], ax)) +
+                             second.half[mm] * fabsf(dot(second.axes[mm], ax)));
+            float sep = fabsf(dot(dd, ax)) - rch;
+            parted = mx(parted, sep);
+            if (sep > bestEdge) {
+                bestEdge = sep;
+                edge.record(first, second, k, j, ax, dd);
+            }
+        }
+    }
+    float smallest = first.half[0];
+    for (int k = 0; k < 3; k++) smallest = mn(smallest, first.half[k]);
+    for (int k = 0; k < 3; k++) smallest = mn(smallest, second.half[k]);
+    float slack = ABSOLUTE * smallest;
+    float fromSecond = 0;
+    if (bestFace[1] > RELATIVE * bestFace[0] + slack) fromSecond = 1;
+    float faceSep = bestFace[0];
+    if (fromSecond > 0.5f) faceSep = bestFace[1];
+    float byEdge = 0;
+    if (bestEdge > RELATIVE * faceSep + slack) byEdge = 1;
+    int count = 0;
+    if (!(parted > margin)) {
+        if (byEdge > 0.5f) {
+            edge.contact(first, second, bestEdge, m);
+            count = 1;
+        } else {
+            Face face(first, second, fromSecond, bestFaceAxis, margin);
+            face.contact(m, count);
+        }
+    }
+    m[COUNT] = (float)count;
+    if (shapeA > 0.5f || shapeB > 0.5f) sphere_contact(first, second, shapeA, shapeB, margin, m);
+}
+
+__global__ void k_collide(BoxDev D) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.slots) return;
+    float *m = D.manifold + (size_t)p * MANIFOLD_WORDS;
+    int partner = D.pairs[2 * (size_t)p + 1];
+    m[COUNT] = 0;
+    if (partner < 0) return;
+    collide_pair(D, D.pairs[2 * (size_t)p], partner, m);
+}
+
+// The narrow phase alone over given pairs (what the reference's tests run as a collide-only program).
+__global__ void k_collide_pairs(BoxDev D, const int32_t *pairs, int npairs, float *manifold) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    float *m = manifold + (size_t)p * MANIFOLD_WORDS;
+    for (int w = 0; w < MANIFOLD_WORDS; w++) m[w] = 0;
+    collide_pair(D, pairs[2 * p], pairs[2 * p + 1], m);
+}
+
+// BoxPairs.carry, BoxPairs.java:191-267, and BoxBroad.counts, BoxBroad.java:174-188, one thread a slot.
+__global__ void k_carry_counts(BoxDev D) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.slots) return;
+    size_t out = (size_t)p * MANIFOLD_WORDS, held = (size_t)p * IMPULSE_WORDS;
+    int count = (int)D.manifold[out + COUNT];
+    int partner = D.pairs[2 * (size_t)p + 1];
+    int firstSlot = (p / SLOTS) * SLOTS;
+    int wasSlot = -1;
+    for (int look = 0; look < SLOTS; look++) {
+        int then = D.pairsWas[(size_t)(firstSlot + look) * 2 + 1];
+        if (then == partner && !(partner < 0)) wasSlot = firstSlot + look;
+    }
+    int before = 0;
+    size_t outWas = 0, heldWas = 0;
+    if (!(wasSlot < 0)) {
+        outWas = (size_t)wasSlot * MANIFOLD_WORDS;
+        heldWas = (size_t)wasSlot * IMPULSE_WORDS;
+        before = (int)D.manifoldWas[outWas + COUNT];
+    }
+    int ia = D.pairs[2 * (size_t)p];
+    float smallest = mn(D.half[0][ia], mn(D.half[1][ia], D.half[2][ia]));
+    float reachSq = (SAME_POINT * smallest) * (SAME_POINT * smallest);
+    for (int slot = 0; slot < MOST_POINTS; slot++) {
+        float taken[POINT_IMPULSE] = {0, 0, 0, 0, 0, 0, 0};
+        if (slot < count) {
+            const float *anchor = D.manifold + out + POINTS + slot * POINT_WORDS + ANCHOR_FIRST;
+            float nearestSq = reachSq;
+            for (int old = 0; old < MOST_POINTS; old++) {
+                if (!(old < before)) continue;
+                const float *then = D.manifoldWas + outWas + POINTS + old * POINT_WORDS + ANCHOR_FIRST;
+                float gap[3] = {anchor[0] - then[0], anchor[1] - then[1], anchor[2] - then[2]};
+                float d2 = dot(gap, gap);
+                if (d2 < nearestSq) {
+                    nearestSq = d2;
+                    for (int w = 0; w < POINT_IMPULSE; w++)
+                        taken[w] = D.impulsesWas[heldWas + POINT_IMPULSE * old + w];
+                }
+            }
+        }
+        if (slot == 0 && count == 1 && before == 1)
+            for (int w = 0; w < POINT_IMPULSE; w++) taken[w] = D.impulsesWas[heldWas + w];
+        for (int w = 0; w < POINT_IMPULSE; w++) D.impulses[held + POINT_IMPULSE * slot + w] = taken[w];
+    }
+    D.contacts[p] = count > 0 ? partner : -1;
+    D.contacts[(size_t)D.slots + p] = count;
+}
+
+// Boxes.show, Boxes.java:801-820.
+__global__ void k_show(BoxDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    float *b = D.shown + (size_t)s * SHOWN_STRIDE;
+    for (int a = 0; a < 3; a++) {
+        b[a] = b[4 + a];
+        b[4 + a] = D.x[a][s];
+        b[8 + a] = D.half[a][s];
+    }
+    for (int k = 0; k < 4; k++) {
+        b[12 + k] = b[16 + k];
+        b[16 + k] = D.q[k][s];
+    }
+    b[3] = D.shape[s];
+}
+
+}  // namespace
+
+void launch_velocities(const BoxDev &D, hipStream_t s) {
+    k_velocities<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("velocities");
+}
+
+void launch_positions(const BoxDev &D, hipStream_t s) {
+    k_positions<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("positions");
+}
+
+template <int SWEEP, bool REVERSE>
+static void walls_then(const BoxDev &D, hipStream_t s) {
+    k_walls<SWEEP, REVERSE><<<blocks_for(D.n), THREADS, 0, s>>>(D);
+}
+
+void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s) {
+    switch (sweep * 2 + (reverse ? 1 : 0)) {
+        case 0: walls_then<WARM, false>(D, s); break;
+        case 1: walls_then<WARM, true>(D, s); break;
+        case 2: walls_then<SOLVE, false>(D, s); break;
+        case 3: walls_then<SOLVE, true>(D, s); break;
+        case 4: walls_then<RELAX, false>(D, s); break;
+        default: walls_then<RELAX, true>(D, s); break;
+    }
+    check_launch("walls");
+}
+
+void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int r, hipStream_t s) {
+    int blocks = blocks_for(D.width);
+    switch (sweep * 2 + (reverse ? 1 : 0)) {
+        case 0: k_solve_round<WARM, false><<<blocks, THREADS, 0, s>>>(D, r); break;
+        case 1: k_solve_round<WARM, true><<<blocks, THREADS, 0, s>>>(D, r); break;
+        case 2: k_solve_round<SOLVE, false><<<blocks, THREADS, 0, s>>>(D, r); break;
+        case 3: k_solve_round<SOLVE, true><<<blocks, THREADS, 0, s>>>(D, r); break;
+        case 4: k_solve_round<RELAX, false><<<blocks, THREADS, 0, s>>>(D, r); break;
+        default: k_solve_round<RELAX, true><<<blocks, THREADS, 0, s>>>(D, r); break;
+    }
+    check_launch("solve round");
+}
+
+void launch_bin(const BoxDev &D, hipStream_t s) {
+    k_bin<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("box bin");
+}
+
+void launch_list(const BoxDev &D, hipStream_t s) {
+    k_list<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("box list");
+}
+
+void launch_collide(const BoxDev &D, hipStream_t s) {
+    k_collide<<<blocks_for(D.slots), THREADS, 0, s>>>(D);
+    check_launch("box collide");
+}
+
+void launch_collide_pairs(const BoxDev &D, const int32_t *pairs, int npairs, float *manifold, hipStream_t s) {
+    k_collide_pairs<<<blocks_for(npairs), THREADS, 0, s>>>(D, pairs, npairs, manifold);
+    check_launch("box collide pairs");
+}
+
+void launch_carry_counts(const BoxDev &D, hipStream_t s) {
+    k_carry_counts<<<blocks_for(D.slots), THREADS, 0, s>>>(D);
+    check_launch("box carry");
+}
+
+void launch_show(const BoxDev &D, hipStream_t s) {
+    k_show<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("box show");
+}
+
+}  // namespace vrbox

@@ -1,0 +1,29 @@
+// box_world.h — the box world behind the C ABI (world.hip dispatches to it for VR_BOXES).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+#include "vexelray_rigid.h"
+
+namespace vrbox {
+
+struct BoxWorld;
+
+struct BoxBuf {
+    void *p;
+    int64_t count;
+    int dtype;
+};
+
+BoxWorld *box_create(const vr_step_desc &desc);  // std::invalid_argument for the reference's validation
+void box_destroy(BoxWorld *w);
+void box_clear(BoxWorld *w);
+bool box_find(BoxWorld *w, const std::string &name, BoxBuf *out);
+hipStream_t box_stream(BoxWorld *w);
+void box_params(BoxWorld *w, const int32_t *words, int32_t n);
+void box_step(BoxWorld *w, vr_step_report *rep);
+void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold);
+
+}  // namespace vrbox

@@ -1,0 +1,301 @@
+// box_world.hip — a box world behind the C ABI: its buffers by name and the step schedule.
+//
+// The schedule is the reference's BoxStepper.step (core/.../box/BoxStepper.java:56-100) over BoxStep's pass lists
+// (core/.../box/BoxStep.java:138-153, 199-231): keep, bin and sort, list, collide, carry and counts; the touching
+// pairs coloured greedily in slot order into rounds (first fit, at most 32); then each substep velocities, the warm
+// sweep, the solve sweeps, positions and the relax sweeps, each sweep every round of pairs then the walls.
+#include "box_world.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "box_kernels.h"
+#include "box_math.h"
+#include "sphere_kernels.h"
+
+namespace vrbox {
+
+namespace {
+
+#define HIP_OK(expr)                                                                                       \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct BoxWorld {
+    vr_step_desc d{};
+    int n = 0;
+    bool grid = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    std::vector<void *> allocs;
+    std::vector<std::pair<std::string, BoxBuf>> named;
+    BoxDev D{};
+    vr::SphereDev sortView{};  // the sphere kernels' grid sort reused: keys, order, starts over the same cells
+    vr::SortScratch S{};
+    std::vector<int32_t> hostContacts, hostRounds;
+    std::vector<uint8_t> taken;
+
+    template <class T>
+    T *alloc(int64_t count) {
+        void *p = nullptr;
+        HIP_OK(hipMalloc(&p, (size_t)std::max<int64_t>(count, 1) * sizeof(T)));
+        allocs.push_back(p);
+        return (T *)p;
+    }
+
+    void reg(const char *name, void *p, int64_t count, int dtype) { named.push_back({name, BoxBuf{p, count, dtype}}); }
+
+    void build(const vr_step_desc &desc) {
+        d = desc;
+        if (d.bodies < 1 || d.substeps < 1 || d.sweeps < 1)
+            throw std::invalid_argument("a step needs boxes, substeps and sweeps, got " + std::to_string(d.bodies) +
+                                        ", " + std::to_string(d.substeps) + " and " + std::to_string(d.sweeps));
+        n = d.bodies;
+        grid = n > 1 && d.has_grid;
+        if (grid && (d.nx < 1 || d.ny < 1 || d.nz < 1 || !(d.cell > 0)))
+            throw std::invalid_argument("a grid of at least one cell, of positive size");
+        int devices = 0;
+        if (hipGetDeviceCount(&devices) != hipSuccess || devices < 1) throw std::runtime_error("no HIP device");
+        HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_OK(hipEventCreate(&t0));
+        HIP_OK(hipEventCreate(&t1));
+        D.n = n;
+        D.slots = grid ? n * SLOTS : 0;
+        D.width = std::max(1, n / 2);
+        static const char *field[] = {"x",  "y",  "z",  "u",  "v",  "w",  "lx", "ly", "lz",
+                                      "qx", "qy", "qz", "qw", "im", "hx", "hy", "hz", "shape"};
+        float **where[] = {&D.x[0], &D.x[1], &D.x[2], &D.vel[0], &D.vel[1], &D.vel[2], &D.l[0], &D.l[1], &D.l[2],
+                           &D.q[0], &D.q[1], &D.q[2], &D.q[3], &D.im, &D.half[0], &D.half[1], &D.half[2], &D.shape};
+        int64_t stride = ((int64_t)n + 63) / 64 * 64;
+        float *arena = alloc<float>(18 * stride);
+        for (int k = 0; k < 18; k++) {
+            *where[k] = arena + k * stride;
+            reg(field[k], *where[k], n, VR_F32);
+        }
+        float *params = alloc<float>(PARAM_COUNT);
+        D.params = params;
+        reg("params", params, PARAM_COUNT, VR_F32);
+        D.lambda = alloc<float>((int64_t)LAMBDA_WORDS * n);
+        D.rub = alloc<float>((int64_t)RUB_WORDS * n);
+        D.roll = alloc<float>((int64_t)RUB_WORDS * n);
+        D.shown = alloc<float>((int64_t)SHOWN_STRIDE * n);
+        reg("lambda", D.lambda, (int64_t)LAMBDA_WORDS * n, VR_F32);
+        reg("rub", D.rub, (int64_t)RUB_WORDS * n, VR_F32);
+        reg("roll", D.roll, (int64_t)RUB_WORDS * n, VR_F32);
+        reg("shown", D.shown, (int64_t)SHOWN_STRIDE * n, VR_F32);
+        if (grid) {
+            D.nx = d.nx;
+            D.ny = d.ny;
+            D.nz = d.nz;
+            D.cell = d.cell;
+            D.cells = d.nx * d.ny * d.nz;
+            D.keys = alloc<int32_t>(n);
+            D.order = alloc<int32_t>(n);
+            D.starts = alloc<int32_t>((int64_t)D.cells + 1);
+            reg("keys", D.keys, n, VR_I32);
+            reg("order", D.order, n, VR_I32);
+            reg("starts", D.starts, (int64_t)D.cells + 1, VR_I32);
+            int64_t slots = D.slots;
+            D.pairs = alloc<int32_t>(2 * slots);
+            D.pairsWas = alloc<int32_t>(2 * slots);
+            D.manifold = alloc<float>((int64_t)MANIFOLD_WORDS * slots);
+            D.manifoldWas = alloc<float>((int64_t)MANIFOLD_WORDS * slots);
+            D.impulses = alloc<float>((int64_t)IMPULSE_WORDS * slots);
+            D.impulsesWas = alloc<float>((int64_t)IMPULSE_WORDS * slots);
+            D.contacts = alloc<int32_t>(2 * slots + n);
+            D.rounds = alloc<int32_t>((int64_t)MOST_ROUNDS * D.width);
+            reg("contacts", D.contacts, 2 * slots + n, VR_I32);
+            reg("rounds", D.rounds, (int64_t)MOST_ROUNDS * D.width, VR_I32);
+            sortView.n = n;
+            sortView.nx = d.nx;
+            sortView.ny = d.ny;
+            sortView.nz = d.nz;
+            sortView.cells = D.cells;
+            sortView.cell = d.cell;
+            sortView.keys = D.keys;
+            sortView.order = D.order;
+            sortView.starts = D.starts;
+            int nblocks = (n + 2047) / 2048;
+            S.maxBlocks = nblocks;
+            S.keysA = alloc<int32_t>(n);
+            S.valsA = alloc<int32_t>(n);
+            S.keysB = alloc<int32_t>(n);
+            S.hist = alloc<int32_t>(256 * (int64_t)nblocks);
+            S.scanTmp = alloc<int32_t>(std::max(vr::scan_tmp_words(256 * nblocks), vr::scan_tmp_words(n)) + 64);
+            hostContacts.resize((size_t)2 * slots + n);
+            hostRounds.resize((size_t)MOST_ROUNDS * D.width);
+            taken.resize((size_t)MOST_ROUNDS * n);
+        }
+        clear();
+    }
+
+    // Buffers whose address changes from step to step (the kept copies swap with the current ones).
+    bool find(const std::string &name, BoxBuf *out) {
+        for (auto &kv : named)
+            if (kv.first == name) {
+                *out = kv.second;
+                return true;
+            }
+        if (grid) {
+            int64_t slots = D.slots;
+            if (name == "pairs") return *out = BoxBuf{D.pairs, 2 * slots, VR_I32}, true;
+            if (name == "pairsWas") return *out = BoxBuf{D.pairsWas, 2 * slots, VR_I32}, true;
+            if (name == "manifold") return *out = BoxBuf{D.manifold, (int64_t)MANIFOLD_WORDS * slots, VR_F32}, true;
+            if (name == "manifoldWas")
+                return *out = BoxBuf{D.manifoldWas, (int64_t)MANIFOLD_WORDS * slots, VR_F32}, true;
+            if (name == "impulses") return *out = BoxBuf{D.impulses, (int64_t)IMPULSE_WORDS * slots, VR_F32}, true;
+            if (name == "impulsesWas")
+                return *out = BoxBuf{D.impulsesWas, (int64_t)IMPULSE_WORDS * slots, VR_F32}, true;
+        }
+        return false;
+    }
+
+    void clear() {  // PassRunner.clear(), then BoxStep.constants(): qw = 1 (BoxStep.java:103-105)
+        for (auto &kv : named) HIP_OK(hipMemsetAsync(kv.second.p, 0, (size_t)kv.second.count * 4, stream));
+        if (grid) {
+            size_t slots = D.slots;
+            for (void *p : {(void *)D.pairs, (void *)D.pairsWas})
+                HIP_OK(hipMemsetAsync(p, 0, slots * 2 * 4, stream));
+            for (void *p : {(void *)D.manifold, (void *)D.manifoldWas})
+                HIP_OK(hipMemsetAsync(p, 0, slots * MANIFOLD_WORDS * 4, stream));
+            for (void *p : {(void *)D.impulses, (void *)D.impulsesWas})
+                HIP_OK(hipMemsetAsync(p, 0, slots * IMPULSE_WORDS * 4, stream));
+        }
+        vr::launch_fill(D.q[3], 1.0f, n, stream);
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    // BoxStep.sweep, BoxStep.java:217-231.
+    void sweep(int kind, bool reverse, int rounds) {
+        if (reverse) launch_walls(D, kind, reverse, stream);
+        for (int order = 0; order < rounds; order++)
+            launch_solve_round(D, kind, reverse, reverse ? rounds - 1 - order : order, stream);
+        if (!reverse) launch_walls(D, kind, reverse, stream);
+    }
+
+    void step(vr_step_report *rep) {
+        auto w0 = std::chrono::steady_clock::now();
+        HIP_OK(hipEventRecord(t0, stream));
+        int used = 0, touching = 0, points = 0, unplaced = 0, unlisted = 0;
+        if (grid) {
+            // BoxPairs.keep: the step's pairs, manifolds and impulses become the kept ones, which carry reads.
+            std::swap(D.pairs, D.pairsWas);
+            std::swap(D.manifold, D.manifoldWas);
+            std::swap(D.impulses, D.impulsesWas);
+            launch_bin(D, stream);
+            vr::launch_sort(sortView, S, stream);
+            launch_list(D, stream);
+            launch_collide(D, stream);
+            launch_carry_counts(D, stream);
+            // BoxStepper: the contacts read back and coloured on the host, first fit in slot order.
+            HIP_OK(hipMemcpyAsync(hostContacts.data(), D.contacts, hostContacts.size() * 4, hipMemcpyDeviceToHost,
+                                  stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            int slots = D.slots, width = D.width;
+            std::fill(taken.begin(), taken.end(), 0);
+            std::fill(hostRounds.begin(), hostRounds.end(), -1);
+            std::vector<int> filled(MOST_ROUNDS, 0);
+            for (int p = 0; p < slots; p++) {
+                int b = hostContacts[p];
+                if (b < 0) continue;
+                int a = p / SLOTS;
+                touching++;
+                points += hostContacts[(size_t)slots + p];
+                int r = 0;
+                while (r < MOST_ROUNDS && (taken[(size_t)r * n + a] || taken[(size_t)r * n + b])) r++;
+                if (r == MOST_ROUNDS) {
+                    unplaced++;
+                } else {
+                    taken[(size_t)r * n + a] = taken[(size_t)r * n + b] = 1;
+                    hostRounds[(size_t)r * width + filled[r]++] = p;
+                    used = std::max(used, r + 1);
+                }
+            }
+            for (int s = 0; s < n; s++) unlisted += std::max(0, hostContacts[2 * (size_t)slots + s] - SLOTS);
+            if (used > 0)
+                HIP_OK(hipMemcpyAsync(D.rounds, hostRounds.data(), hostRounds.size() * 4, hipMemcpyHostToDevice,
+                                      stream));
+        }
+        for (int sub = 0; sub < d.substeps; sub++) {
+            launch_velocities(D, stream);
+            sweep(WARM, false, used);
+            for (int k = 0; k < d.sweeps; k++) sweep(SOLVE, d.symmetric && k % 2 == 1, used);
+            launch_positions(D, stream);
+            for (int k = 0; k < d.sweeps; k++) sweep(RELAX, d.symmetric && k % 2 == 1, used);
+        }
+        launch_show(D, stream);
+        HIP_OK(hipEventRecord(t1, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, t0, t1));
+        rep->rounds = used;
+        rep->most_rounds = used;
+        rep->touching = touching;
+        rep->points = points;
+        rep->unplaced = unplaced;
+        rep->unlisted = unlisted;
+        rep->gpu_ns = (int64_t)(ms * 1e6);
+        rep->wall_ns =
+            std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - w0).count();
+    }
+
+    void collide_pairs(const int32_t *pairs, int npairs, float *manifold) {
+        int32_t *dp = nullptr;
+        float *dm = nullptr;
+        HIP_OK(hipMalloc(&dp, (size_t)std::max(1, 2 * npairs) * 4));
+        HIP_OK(hipMalloc(&dm, (size_t)std::max(1, npairs) * MANIFOLD_WORDS * 4));
+        HIP_OK(hipMemcpyAsync(dp, pairs, (size_t)2 * npairs * 4, hipMemcpyHostToDevice, stream));
+        if (npairs > 0) launch_collide_pairs(D, dp, npairs, dm, stream);
+        HIP_OK(hipMemcpyAsync(manifold, dm, (size_t)npairs * MANIFOLD_WORDS * 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        HIP_OK(hipFree(dp));
+        HIP_OK(hipFree(dm));
+    }
+
+    ~BoxWorld() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : allocs) (void)hipFree(p);
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+BoxWorld *box_create(const vr_step_desc &desc) {
+    auto *w = new BoxWorld();
+    try {
+        w->build(desc);
+    } catch (...) {
+        delete w;
+        throw;
+    }
+    return w;
+}
+
+void box_destroy(BoxWorld *w) { delete w; }
+void box_clear(BoxWorld *w) { w->clear(); }
+bool box_find(BoxWorld *w, const std::string &name, BoxBuf *out) { return w->find(name, out); }
+hipStream_t box_stream(BoxWorld *w) { return w->stream; }
+
+void box_params(BoxWorld *w, const int32_t *words, int32_t n) {
+    if (n != PARAM_COUNT) throw std::invalid_argument("box parameters are 18 words");
+    HIP_OK(hipMemcpyAsync((void *)w->D.params, words, sizeof(int32_t) * PARAM_COUNT, hipMemcpyHostToDevice,
+                          w->stream));
+    HIP_OK(hipStreamSynchronize(w->stream));
+}
+
+void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
+
+void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold) {
+    w->collide_pairs(pairs, npairs, manifold);
+}
+
+}  // namespace vrbox

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "box_world.h"
 #include "sphere_kernels.h"
 #include "vexelray_rigid.h"
 
@@ -48,6 +49,7 @@ struct Buf {
 }  // namespace
 
 struct vr_world {
+    vrbox::BoxWorld *box = nullptr;  // set for a box world, which the calls below hand on to
     vr_step_desc d{};
     int n = 0, cap = 0, cells = 0;
     bool gs = false, grid = false, friction = true;
@@ -497,6 +499,7 @@ struct vr_world {
     }
 
     ~vr_world() {
+        if (box) vrbox::box_destroy(box);
         if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : allocs) (void)hipFree(p);
         if (pinned) (void)hipHostFree(pinned);
@@ -537,9 +540,20 @@ extern "C" {
 
 int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out) {
     if (!desc || !out) return fail(VR_ERR_ARGUMENT, "null argument");
-    if (kind != VR_SPHERES) return fail(VR_ERR_ARGUMENT, "unknown world kind " + std::to_string(kind));
+    if (kind != VR_SPHERES && kind != VR_BOXES)
+        return fail(VR_ERR_ARGUMENT, "unknown world kind " + std::to_string(kind));
     auto *w = new vr_world();
-    int rc = guard([&] { w->build(*desc); });
+    int rc = guard([&] {
+        if (kind == VR_BOXES) {
+            try {
+                w->box = vrbox::box_create(*desc);
+            } catch (const std::invalid_argument &e) {
+                throw ArgError(e.what());
+            }
+        } else {
+            w->build(*desc);
+        }
+    });
     if (rc != VR_OK) {
         delete w;
         return rc;
@@ -550,23 +564,53 @@ int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out) {
 
 int vr_world_clear(vr_world *w) {
     if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    if (w->box) return guard([&] { vrbox::box_clear(w->box); });
     return guard([&] { w->clear(); });
 }
 
+namespace {
+
+Buf find_any(vr_world *w, const std::string &name) {
+    if (!w->box) return w->find(name);
+    vrbox::BoxBuf b;
+    if (!vrbox::box_find(w->box, name, &b)) throw NameError("no buffer named " + name);
+    return Buf{b.p, b.count, b.dtype};
+}
+
+hipStream_t stream_of(vr_world *w) { return w->box ? vrbox::box_stream(w->box) : w->stream; }
+
+void copy_in(vr_world *w, const std::string &name, const void *src, int64_t count) {
+    Buf b = find_any(w, name);
+    if (count < 0 || count > b.count) throw NameError("buffer " + name + " is shorter than the write");
+    hipStream_t s = stream_of(w);
+    HIP_OK(hipMemcpyAsync(b.p, src, (size_t)count * 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+void copy_out(vr_world *w, const std::string &name, void *dst, int64_t count) {
+    Buf b = find_any(w, name);
+    if (count < 0 || count > b.count) throw NameError("buffer " + name + " is shorter than the read");
+    hipStream_t s = stream_of(w);
+    HIP_OK(hipMemcpyAsync(dst, b.p, (size_t)count * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
 int vr_world_write(vr_world *w, const char *name, const void *src, int64_t count) {
     if (!w || !name || (!src && count > 0)) return fail(VR_ERR_ARGUMENT, "null argument");
-    return guard([&] { w->write(name, src, count); });
+    return guard([&] { copy_in(w, name, src, count); });
 }
 
 int vr_world_read(vr_world *w, const char *name, void *dst, int64_t count) {
     if (!w || !name || (!dst && count > 0)) return fail(VR_ERR_ARGUMENT, "null argument");
-    return guard([&] { w->read(name, dst, count); });
+    return guard([&] { copy_out(w, name, dst, count); });
 }
 
 int vr_world_buffer_info(vr_world *w, const char *name, int64_t *count, int32_t *dtype) {
     if (!w || !name || !count || !dtype) return fail(VR_ERR_ARGUMENT, "null argument");
     return guard([&] {
-        Buf b = w->find(name);
+        Buf b = find_any(w, name);
         *count = b.count;
         *dtype = b.dtype;
     });
@@ -574,6 +618,15 @@ int vr_world_buffer_info(vr_world *w, const char *name, int64_t *count, int32_t 
 
 int vr_world_params(vr_world *w, const int32_t *words, int32_t n) {
     if (!w || !words) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (w->box) {
+        return guard([&] {
+            try {
+                vrbox::box_params(w->box, words, n);
+            } catch (const std::invalid_argument &e) {
+                throw ArgError(e.what());
+            }
+        });
+    }
     return guard([&] { w->params(words, n); });
 }
 
@@ -585,17 +638,31 @@ int vr_world_friction(vr_world *w, int32_t on) {
 
 int vr_world_step(vr_world *w, int32_t wait, vr_step_report *report) {
     if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    if (w->box) {
+        return guard([&] {
+            vr_step_report local{};
+            vr_step_report *r = report ? report : &local;
+            std::memset(r, 0, sizeof(*r));
+            vrbox::box_step(w->box, r);
+        });
+    }
     return guard([&] { w->step(wait, report); });
 }
 
 int vr_world_finish(vr_world *w) {
     if (!w) return fail(VR_ERR_ARGUMENT, "null world");
-    return guard([&] { HIP_OK(hipStreamSynchronize(w->stream)); });
+    return guard([&] { HIP_OK(hipStreamSynchronize(stream_of(w))); });
 }
 
 int vr_world_destroy(vr_world *w) {
     delete w;
     return VR_OK;
+}
+
+int vr_box_collide(vr_world *w, const int32_t *pairs, int32_t npairs, float *manifold) {
+    if (!w || (!pairs && npairs > 0) || (!manifold && npairs > 0)) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (!w->box) return fail(VR_ERR_STATE, "not a box world");
+    return guard([&] { vrbox::box_collide(w->box, pairs, npairs, manifold); });
 }
 
 int vr_world_profile(vr_world *w, int32_t on) {
