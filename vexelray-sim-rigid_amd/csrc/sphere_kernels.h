@@ -48,10 +48,13 @@ struct RoundConst {
 };
 
 // Device control of the Gauss-Seidel passes run until done (see k_round_wide / k_round_tail). Each pass of a step
-// has a slot of its own, set up once at the start of the step, so no pass needs a kernel to set it up.
+// has a slot of its own, cleared once at the start of the step, so no pass needs a kernel to set it up.
+constexpr int MAX_WIDE = 64;  // most wide rounds a pass launches
+constexpr int SEG = 256;      // open-list places a wide round's workgroup owns (its thread count)
+
 struct PassSlot {
-    int rd[2];   // rd[j & 1] == j: round j may still run as a wide round
-    int cnt[4];  // cnt[j & 3]: contacts open at the start of round j (round 0: every listed contact)
+    int tailCount;          // contacts the pass's last wide round left open, compacted for the tail
+    int opened[MAX_WIDE];   // opened[j] != 0: round j had an open contact to take
 };
 
 struct PassCtl {
@@ -59,11 +62,15 @@ struct PassCtl {
     int error, errRounds, errListed;
     int waits, rounds, most, unlisted;  // SphereStepper.Report, added up over the step
     long long listedSum;                // contacts listed, added up over every pass (parting and rubbing)
-    int wideRan[64];                    // per pass of a substep: the most rounds run wide this step
+    int wideRan[MAX_WIDE];              // per pass of a substep: the most rounds run wide this step
 };
 
+// Round j's open contacts, as {c, a, o, 0}: in segment g of buf[j & 1], seg[j & 1][g] of them, for the wide rounds
+// (round 0 takes the listed contacts in place); from place 0 of buf[j & 1] for the tail.
 struct OpenLists {
-    int32_t *buf[2];  // round j's open contacts are in buf[j & 1] (round 0 takes every listed contact)
+    int4 *buf[2];     // segments * SEG entries each
+    int32_t *seg[2];  // segments each
+    int segments;
 };
 
 struct SortScratch {

@@ -41,10 +41,12 @@ __device__ __forceinline__ void cross3(const float *x, const float *y, float *z)
     z[2] = x[0] * y[1] - x[1] * y[0];
 }
 
-// A sphere's position, moves and turns, loaded once and stored once (the reference's Held, Spheres.java:532-569).
+// A sphere's position, moves and turns, loaded once and stored once (the reference's Held, Spheres.java:532-569),
+// with what a pass only reads of it: radius, spin and velocity (a pass changes none of them).
 struct Held {
     int s;
     float at[3], displaced[3], turned[3];
+    float r, spin[3], vel[3];
 };
 
 __device__ __forceinline__ Held hold(const SphereDev &D, int s) {
@@ -53,6 +55,9 @@ __device__ __forceinline__ Held hold(const SphereDev &D, int s) {
     h.at[0] = D.x[s]; h.at[1] = D.y[s]; h.at[2] = D.z[s];
     h.displaced[0] = D.dx[s]; h.displaced[1] = D.dy[s]; h.displaced[2] = D.dz[s];
     h.turned[0] = D.tx[s]; h.turned[1] = D.ty[s]; h.turned[2] = D.tz[s];
+    h.r = D.r[s];
+    h.spin[0] = D.ax[s]; h.spin[1] = D.ay[s]; h.spin[2] = D.az[s];
+    h.vel[0] = D.u[s]; h.vel[1] = D.v[s]; h.vel[2] = D.w[s];
     return h;
 }
 
@@ -60,6 +65,18 @@ __device__ __forceinline__ void store(const SphereDev &D, const Held &h, bool tu
     D.x[h.s] = h.at[0]; D.y[h.s] = h.at[1]; D.z[h.s] = h.at[2];
     D.dx[h.s] = h.displaced[0]; D.dy[h.s] = h.displaced[1]; D.dz[h.s] = h.displaced[2];
     if (turns) { D.tx[h.s] = h.turned[0]; D.ty[h.s] = h.turned[1]; D.tz[h.s] = h.turned[2]; }
+}
+
+// The 13 parameter words (Spheres.java:143-155), loaded once a thread at the start of a kernel.
+struct Params {
+    float w[PARAM_COUNT];
+    __device__ __forceinline__ float operator[](int i) const { return w[i]; }
+};
+
+__device__ __forceinline__ Params load_params(const float *p) {
+    Params P;
+    for (int k = 0; k < PARAM_COUNT; k++) P.w[k] = p[k];
+    return P;
 }
 
 struct Impulse {  // Spheres.java:490-522
@@ -87,14 +104,6 @@ __device__ __forceinline__ float turns_per_impulse(float w, float lever, float r
     return (w * lever) / (INERTIA * (radius * radius));
 }
 
-__device__ __forceinline__ void spin_of(const SphereDev &D, int s, float out[3]) {
-    out[0] = D.ax[s]; out[1] = D.ay[s]; out[2] = D.az[s];
-}
-
-__device__ __forceinline__ void vel_of(const SphereDev &D, int s, float out[3]) {
-    out[0] = D.u[s]; out[1] = D.v[s]; out[2] = D.w[s];
-}
-
 // Spheres.rub, Spheres.java:662-677.
 __device__ __forceinline__ void rub(Held &a, float wa, float turnsA, Held *o, float wo, float turnsO,
                                     const float n[3], const float p[3]) {
@@ -117,20 +126,18 @@ __device__ __forceinline__ void rub(Held &a, float wa, float turnsA, Held *o, fl
 // Spheres.contactTravel, Spheres.java:751-765.
 __device__ __forceinline__ void contact_travel(const SphereDev &D, const Held &s, float h, const float n[3],
                                                float lever, int side, float travel[3]) {
-    float spin[3], vel[3], turn[3], turnN[3];
-    spin_of(D, s.s, spin);
-    vel_of(D, s.s, vel);
-    for (int k = 0; k < 3; k++) turn[k] = h * spin[k] + s.turned[k];
+    float turn[3], turnN[3];
+    for (int k = 0; k < 3; k++) turn[k] = h * s.spin[k] + s.turned[k];
     cross3(turn, n, turnN);
     for (int k = 0; k < 3; k++) {
-        float centre = h * vel[k] + s.displaced[k];
+        float centre = h * s.vel[k] + s.displaced[k];
         float spun = lever * turnN[k];
         travel[k] = side > 0 ? centre + spun : centre - spun;
     }
 }
 
 // Spheres.friction, Spheres.java:595-654; o == nullptr for a wall.
-__device__ void friction_at(const SphereDev &D, const float *P, Held &a, float wa, float leverA, Held *o, float wo,
+__device__ void friction_at(const SphereDev &D, const Params &P, Held &a, float wa, float leverA, Held *o, float wo,
                             float leverO, const float n[3], Impulse &taken) {
     float muS = P[MU_STATIC], muK = P[MU_KINETIC];
     if (!(muS + muK > 0)) return;
@@ -139,7 +146,7 @@ __device__ void friction_at(const SphereDev &D, const float *P, Held &a, float w
     float slip[3];
     contact_travel(D, a, h, n, la, -1, slip);
     float ia = wa;
-    float leverOverIa = turns_per_impulse(ia, la, D.r[a.s]);
+    float leverOverIa = turns_per_impulse(ia, la, a.r);
     float weight = ia + leverOverIa * la;
     float io = 0, lo = 0, leverOverIo = 0;
     if (o) {
@@ -148,7 +155,7 @@ __device__ void friction_at(const SphereDev &D, const float *P, Held &a, float w
         float other[3];
         contact_travel(D, *o, h, n, lo, 1, other);
         for (int k = 0; k < 3; k++) slip[k] = slip[k] - other[k];
-        leverOverIo = turns_per_impulse(io, lo, D.r[o->s]);
+        leverOverIo = turns_per_impulse(io, lo, o->r);
         weight = weight + (io + leverOverIo * lo);
     }
     float along = dot3(slip, n);
@@ -177,23 +184,20 @@ __device__ void friction_at(const SphereDev &D, const float *P, Held &a, float w
 }
 
 // Spheres.roll, Spheres.java:699-741; o == nullptr for a wall.
-__device__ void roll(const SphereDev &D, const float *P, Held &a, float wa, Held *o, float wo, float reach,
+__device__ void roll(const SphereDev &D, const Params &P, Held &a, float wa, Held *o, float wo, float reach,
                      Impulse &taken) {
     float muR = P[MU_ROLLING], h = P[H];
-    float ra = D.r[a.s];
+    float ra = a.r;
     float ia = wa / (INERTIA * (ra * ra));
     float inverse = ia;
     float io = 0;
-    float spinA[3], against[3];
-    spin_of(D, a.s, spinA);
-    for (int k = 0; k < 3; k++) against[k] = h * spinA[k] + a.turned[k];
+    float against[3];
+    for (int k = 0; k < 3; k++) against[k] = h * a.spin[k] + a.turned[k];
     if (o) {
-        float ro = D.r[o->s];
+        float ro = o->r;
         io = wo / (INERTIA * (ro * ro));
         inverse = inverse + io;
-        float spinO[3];
-        spin_of(D, o->s, spinO);
-        for (int k = 0; k < 3; k++) against[k] = against[k] - (h * spinO[k] + o->turned[k]);
+        for (int k = 0; k < 3; k++) against[k] = against[k] - (h * o->spin[k] + o->turned[k]);
     }
     if (!(inverse > 0)) return;
     float whole[3];
@@ -212,7 +216,7 @@ __device__ void roll(const SphereDev &D, const float *P, Held &a, float wa, Held
     }
 }
 
-__device__ __forceinline__ float rubbing(const float *P) {  // Spheres.java:684-687
+__device__ __forceinline__ float rubbing(const Params &P) {  // Spheres.java:684-687
     return P[MU_STATIC] + (P[MU_KINETIC] + P[MU_ROLLING]);
 }
 
@@ -233,7 +237,7 @@ __device__ void warm(const SphereDev &D, Held &a, float wa, Held &o, float wo, c
     float along = dot3(taken.p, n);
     for (int k = 0; k < 3; k++) taken.p[k] = taken.p[k] - along * n[k];
     float share = dist / reach;
-    float ra = D.r[a.s], ro = D.r[o.s];
+    float ra = a.r, ro = o.r;
     float turnsA = turns_per_impulse(wa, share * ra, ra);
     float turnsO = turns_per_impulse(wo, share * ro, ro);
     rub(a, wa, turnsA, &o, wo, turnsO, n, taken.p);
@@ -245,21 +249,68 @@ __device__ void warm(const SphereDev &D, Held &a, float wa, Held &o, float wo, c
     }
 }
 
-// Spheres.contact, Spheres.java:369-439: contact c of spheres a < o, parting (rubs == 0) or rubbing.
-__device__ void contact(const SphereDev &D, const float *P, int c, int a, int o, int pass, int rubs) {
-    Held ha = hold(D, a), ho = hold(D, o);
+// What a contact's solve reads: both spheres and the contact's impulse.
+struct PairLoad {
+    Held a, o;
+    float wa, wo;
+    Impulse taken;
+};
+
+__device__ __forceinline__ PairLoad load_pair(const SphereDev &D, int c, int a, int o) {
+    PairLoad L;
+    L.a = hold(D, a);
+    L.o = hold(D, o);
+    L.wa = D.im[a];
+    L.wo = D.im[o];
+    L.taken = load_impulse(D.cur.impulse + (size_t)c * IMPULSE_WORDS);
+    return L;
+}
+
+// Makes the compiler have `v` in a register here, so a load of it is issued no later than this point.
+__device__ __forceinline__ void issued(float v) { asm volatile("" ::"v"(v)); }
+
+__device__ __forceinline__ void issued(const Held &h, bool rubs) {
+    for (int k = 0; k < 3; k++) {
+        issued(h.at[k]);
+        issued(h.displaced[k]);
+        issued(h.turned[k]);
+        if (rubs) {
+            issued(h.spin[k]);
+            issued(h.vel[k]);
+        }
+    }
+    issued(h.r);
+}
+
+// Issues every load of a contact's solve at once (for a rubbing pass, spins and velocities too).
+template <bool RUBS>
+__device__ __forceinline__ void issued(const PairLoad &L) {
+    issued(L.a, RUBS);
+    issued(L.o, RUBS);
+    issued(L.wa);
+    issued(L.wo);
+    issued(L.taken.normal);
+    for (int k = 0; k < 3; k++) {
+        issued(L.taken.p[k]);
+        issued(L.taken.q[k]);
+    }
+}
+
+// Spheres.contact, Spheres.java:369-439: contact c of spheres a < o, parting (RUBS false) or rubbing, from its loads.
+template <bool RUBS>
+__device__ void contact(const SphereDev &D, const Params &P, int c, PairLoad &L, int pass) {
+    Held &ha = L.a, &ho = L.o;
     float d[3];
     for (int k = 0; k < 3; k++) d[k] = ha.at[k] - ho.at[k];
     float d2 = dot3(d, d);
-    float ra = D.r[a], ro = D.r[o];
+    float ra = ha.r, ro = ho.r;
     float reach = ra + ro;
-    float wa = D.im[a], wo = D.im[o];
+    float wa = L.wa, wo = L.wo;
     float wsum = wa + wo;
     float listed = reach * MARGIN1;
     if (!(d2 < listed * listed) || !(d2 > COINCIDENT) || !(wsum > 0)) return;
-    float *imp = D.cur.impulse + (size_t)c * IMPULSE_WORDS;
-    Impulse taken = load_impulse(imp);
-    if (rubs == 0) {
+    Impulse taken = L.taken;
+    if (!RUBS) {
         if (pass == 1) {
             if (rubbing(P) > 0) warm(D, ha, wa, ho, wo, d, d2, reach, taken);
             else clear_impulse(taken);
@@ -286,9 +337,15 @@ __device__ void contact(const SphereDev &D, const float *P, int c, int a, int o,
         roll(D, P, ha, wa, &ho, wo, pairRadius, taken);
         friction_at(D, P, ha, wa, share * ra, &ho, wo, share * ro, nn, taken);
     }
-    store_impulse(imp, taken);
+    store_impulse(D.cur.impulse + (size_t)c * IMPULSE_WORDS, taken);
     store(D, ha, true);
     store(D, ho, true);
+}
+
+__device__ __forceinline__ void contact(const SphereDev &D, const Params &P, int c, int a, int o, int pass, int rubs) {
+    PairLoad L = load_pair(D, c, a, o);
+    if (rubs) contact<true>(D, P, c, L, pass);
+    else contact<false>(D, P, c, L, pass);
 }
 
 // Spheres.priority, Spheres.java:1166-1168 (31 bits); the same formula in 63 bits when wide.
@@ -338,7 +395,7 @@ __global__ void k_predict(SphereDev D) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= D.n) return;
     if (!(D.im[s] > 0)) return;
-    const float *P = D.params;
+    const Params P = load_params(D.params);
     float h = P[H];
     float v0 = D.u[s] + h * P[GX];
     float v1 = D.v[s] + h * P[GY];
@@ -370,7 +427,7 @@ __global__ void k_warm_walls(SphereDev D) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= D.n) return;
     if (!(D.im[s] > 0)) return;
-    const float *P = D.params;
+    const Params P = load_params(D.params);
     float ext[3] = {P[SX], P[SY], P[SZ]};
     float rb = rubbing(P);
     float rs = D.r[s], w = D.im[s];
@@ -488,7 +545,7 @@ __global__ void k_round(SphereDev D, RoundConst rc) {
             won = checked[a] == last && checked[o] == last;
         }
         if (won) {
-            contact(D, D.params, c, a, o, rc.pass, rc.rubs);
+            contact(D, load_params(D.params), c, a, o, rc.pass, rc.rubs);
             D.cur.done[c] = rc.pass;
         } else {
             cleared[a] = 0;
@@ -509,16 +566,23 @@ __global__ void k_round(SphereDev D, RoundConst rc) {
 
 // One contact of a round run until done (the body of Spheres.contactRound, Spheres.java:1088-1131, for a contact
 // still open): it solves if it holds both its spheres' claims from the round before, else clears the spheres' third
-// claim array and claims both. Returns whether it is still open.
-__device__ __forceinline__ bool round_contact(const SphereDev &D, const RoundConst &rc, int c) {
-    int a = D.cur.a[c], o = D.cur.b[c];
+// claim array and claims both. Returns whether it is still open. The solve's loads are issued with the claim check,
+// not after it: in a round, a contact's spheres are written only by the one contact that holds both their claims,
+// so nothing changes what was loaded before it is used.
+template <bool RUBS>
+__device__ __forceinline__ bool round_contact(const SphereDev &D, const Params &P, const RoundConst &rc, int c, int a,
+                                              int o) {
     const long long *checked = D.claims[(rc.slot + 2) % 3];
     long long *claimed = D.claims[rc.slot % 3];
     long long *cleared = D.claims[(rc.slot + 1) % 3];
     if (rc.checks > 0) {
         long long last = priority(D, a, o, rc.checksDispatch);
-        if (checked[a] == last && checked[o] == last) {
-            contact(D, D.params, c, a, o, rc.pass, rc.rubs);
+        long long ca = checked[a], co = checked[o];
+        PairLoad L = load_pair(D, c, a, o);
+        issued<RUBS>(L);
+        for (int k = 0; k < PARAM_COUNT; k++) issued(P[k]);
+        if (ca == last && co == last) {
+            contact<RUBS>(D, P, c, L, rc.pass);
             D.cur.done[c] = rc.pass;
             return false;
         }
@@ -549,8 +613,13 @@ __device__ __forceinline__ RoundConst cycle_round(int it, int p, bool rub, int i
     return rc;
 }
 
-// Appends `c` to `out` for every lane with `open`, one atomic a wave.
-__device__ __forceinline__ void append_open(bool open, int c, int32_t *out, int *count) {
+// Listed contact c as an open-list entry: {c, a, o, 0}.
+__device__ __forceinline__ int4 listed_entry(const SphereDev &D, int c) {
+    return make_int4(c, D.cur.a[c], D.cur.b[c], 0);
+}
+
+// Appends `e` to `out` for every lane with `open`, one atomic a wave on `count` (in LDS or in global memory).
+__device__ __forceinline__ void append_open(bool open, const int4 &e, int4 *out, int *count) {
     unsigned long long mask = __ballot(open);
     if (!mask) return;
     int lane = threadIdx.x & 63;
@@ -558,16 +627,12 @@ __device__ __forceinline__ void append_open(bool open, int c, int32_t *out, int 
     int base = 0;
     if (lane == leader) base = atomicAdd(count, __popcll(mask));
     base = __shfl(base, leader, 64);
-    if (open) out[base + __popcll(mask & ((1ull << lane) - 1ull))] = c;
+    if (open) out[base + __popcll(mask & ((1ull << lane) - 1ull))] = e;
 }
 
-__global__ void k_step_begin(PassCtl *ctl, PassSlot *slots, int nslots) {
+__global__ void k_step_begin(PassCtl *ctl, PassSlot *slots, int words) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nslots) {
-        slots[q].rd[0] = 0;
-        slots[q].rd[1] = -1;
-        for (int k = 0; k < 4; k++) slots[q].cnt[k] = 0;
-    }
+    if (q < words) reinterpret_cast<int *>(slots)[q] = 0;
     if (q == 0) {
         ctl->error = 0;
         ctl->waits = 0;
@@ -575,57 +640,80 @@ __global__ void k_step_begin(PassCtl *ctl, PassSlot *slots, int nslots) {
         ctl->most = 0;
         ctl->unlisted = 0;
         ctl->listedSum = 0;
-        for (int k = 0; k < 64; k++) ctl->wideRan[k] = 0;
+        for (int k = 0; k < MAX_WIDE; k++) ctl->wideRan[k] = 0;
     }
 }
 
-// Round j of a pass over the whole chip, while at least `T` contacts are open and every round before it ran wide;
-// otherwise nothing, and k_round_tail finishes the pass. The open contacts that do not solve are compacted into
-// round j+1's list; which order they land in does not matter, since a round's outcome is the same in any order.
-__global__ void k_round_wide(SphereDev D, const PassCtl *ctl, PassSlot *slot, OpenLists L, int it, int iterations,
-                             int rub, int j, int T) {
-    if (slot->rd[j & 1] != j) return;
-    int nIn = j == 0 ? min(D.count[LISTED], D.cap) : slot->cnt[j & 3];
-    if (nIn == 0 || nIn < T) return;
-    if (rub && j >= ctl->k) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        slot->rd[(j + 1) & 1] = j + 1;
-        slot->cnt[(j + 2) & 3] = 0;
+// Round j of a pass over the whole chip, a workgroup per segment of SEG list places. In round 0, workgroup g takes
+// the listed contacts g*SEG onwards; in each later round it takes the open contacts it kept in the round before. It
+// compacts the contacts that stay open into its own segment of the other list, counting in LDS, so no round has a
+// device-wide counter; the pass's last wide round (`last`) compacts into one list instead, for k_round_tail. Which
+// order open contacts land in does not matter: a round's outcome is the same in any order.
+template <bool RUB>
+__global__ void __launch_bounds__(THREADS) k_round_wide(SphereDev D, const PassCtl *ctl, PassSlot *slot,
+                                                        OpenLists L, int it, int iterations, int j, int last) {
+    if (RUB && j >= ctl->k) return;
+    __shared__ int sCount;
+    int g = blockIdx.x;
+    int nIn = j == 0 ? min(SEG, min(D.count[LISTED], D.cap) - g * SEG) : L.seg[j & 1][g];
+    if (nIn <= 0) {
+        if (!last && threadIdx.x == 0) L.seg[(j + 1) & 1][g] = 0;
+        return;
     }
-    RoundConst rc = cycle_round(it, position(j), rub != 0, iterations);
-    const int32_t *in = L.buf[j & 1];
-    int32_t *out = L.buf[(j + 1) & 1];
-    int *outCount = &slot->cnt[(j + 1) & 3];
-    for (int base = blockIdx.x * blockDim.x; base < nIn; base += gridDim.x * blockDim.x) {
-        int i = base + threadIdx.x;
-        bool open = false;
-        int c = 0;
-        if (i < nIn) {
-            c = j == 0 ? i : in[i];
-            open = round_contact(D, rc, c);
-        }
-        append_open(open, c, out, outCount);
+    if (threadIdx.x == 0) {
+        slot->opened[j] = 1;
+        sCount = 0;
+    }
+    __syncthreads();
+    RoundConst rc = cycle_round(it, position(j), RUB, iterations);
+    const Params P = load_params(D.params);
+    int i = threadIdx.x;
+    bool open = false;
+    int4 e = make_int4(0, 0, 0, 0);
+    if (i < nIn) {
+        e = j == 0 ? listed_entry(D, g * SEG + i) : L.buf[j & 1][(size_t)g * SEG + i];
+        open = round_contact<RUB>(D, P, rc, e.x, e.y, e.z);
+    }
+    if (last) {
+        append_open(open, e, L.buf[(j + 1) & 1], &slot->tailCount);
+    } else {
+        append_open(open, e, L.buf[(j + 1) & 1] + (size_t)g * SEG, &sCount);
+        __syncthreads();
+        if (threadIdx.x == 0) L.seg[(j + 1) & 1][g] = sCount;
     }
 }
 
-// The rest of a pass in one workgroup, rounds separated by barriers rather than launches. A parting pass runs until
-// no contact is open, then settles k, the rounds the reference's stepper would have run: its opening batch, then
-// batches of 8, up to the first report that finds nothing open (SphereStepper.java:72-96); the opening batch then
-// adapts as there (:118-126). A rubbing pass runs the parting pass's k rounds, or until nothing is open.
+// The rest of a pass in one workgroup, rounds separated by barriers rather than launches. It starts at the first
+// round the wide rounds found empty, or else at round `wide` on the list the last of them compacted. A parting pass
+// runs until no contact is open, then settles k, the rounds the reference's stepper would have run: its opening
+// batch, then batches of 8, up to the first report that finds nothing open (SphereStepper.java:72-96); the opening
+// batch then adapts as there (:118-126). A rubbing pass runs the parting pass's k rounds, or until nothing is open.
 constexpr int TAIL = 512;
 
+template <bool RUB>
 __global__ void __launch_bounds__(TAIL) k_round_tail(SphereDev D, PassCtl *ctl, const PassSlot *slot, OpenLists L,
                                                      int32_t *opening, int32_t *enough, int it, int iterations,
-                                                     int rub, int limit) {
+                                                     int wide, int limit) {
     __shared__ int sCount;
-    int j = max(slot->rd[0], slot->rd[1]);
-    int n = j == 0 ? min(D.count[LISTED], D.cap) : slot->cnt[j & 3];
-    int open0 = rub ? 0 : opening[it];
-    int kRub = rub ? ctl->k : 0;
+    int listed = min(D.count[LISTED], D.cap);
+    int j = 0, n = listed;
+    if (wide > 0) {
+        j = wide;
+        for (int r = 0; r < wide; r++) {
+            if (!slot->opened[r]) {
+                j = r;
+                break;
+            }
+        }
+        n = j < wide ? 0 : slot->tailCount;
+    }
+    int open0 = RUB ? 0 : opening[it];
+    int kRub = RUB ? ctl->k : 0;
+    const Params P = load_params(D.params);
     int k = 0;
     bool failed = false;
     while (true) {
-        if (!rub) {
+        if (!RUB) {
             if (n == 0) {
                 k = j <= open0 ? open0 : open0 + BATCH * ((j - open0 + BATCH - 1) / BATCH);
                 break;
@@ -638,20 +726,20 @@ __global__ void __launch_bounds__(TAIL) k_round_tail(SphereDev D, PassCtl *ctl, 
         } else if (j >= kRub || n == 0) {
             break;
         }
-        RoundConst rc = cycle_round(it, position(j), rub != 0, iterations);
-        const int32_t *in = L.buf[j & 1];
-        int32_t *out = L.buf[(j + 1) & 1];
+        RoundConst rc = cycle_round(it, position(j), RUB, iterations);
+        const int4 *in = L.buf[j & 1];
+        int4 *out = L.buf[(j + 1) & 1];
         if (threadIdx.x == 0) sCount = 0;
         __syncthreads();
         for (int base = 0; base < n; base += TAIL) {
             int i = base + threadIdx.x;
             bool open = false;
-            int c = 0;
+            int4 e = make_int4(0, 0, 0, 0);
             if (i < n) {
-                c = j == 0 ? i : in[i];
-                open = round_contact(D, rc, c);
+                e = j == 0 ? listed_entry(D, i) : in[i];
+                open = round_contact<RUB>(D, P, rc, e.x, e.y, e.z);
             }
-            append_open(open, c, out, &sCount);
+            append_open(open, e, out, &sCount);
         }
         __syncthreads();
         n = sCount;
@@ -659,10 +747,10 @@ __global__ void __launch_bounds__(TAIL) k_round_tail(SphereDev D, PassCtl *ctl, 
         __syncthreads();
     }
     // The rounds this pass needed, for the host to launch that many wide next step.
-    int used = (rub ? iterations : 0) + it;
-    if (threadIdx.x == 0 && used < 64) ctl->wideRan[used] = max(ctl->wideRan[used], j);
-    if (threadIdx.x == 0) ctl->listedSum += min(D.count[LISTED], D.cap);
-    if (threadIdx.x != 0 || rub) return;
+    int used = (RUB ? iterations : 0) + it;
+    if (threadIdx.x == 0 && used < MAX_WIDE) ctl->wideRan[used] = max(ctl->wideRan[used], j);
+    if (threadIdx.x == 0) ctl->listedSum += listed;
+    if (threadIdx.x != 0 || RUB) return;
     if (failed) {
         ctl->error = 1;
         ctl->errRounds = k;
@@ -687,7 +775,7 @@ __global__ void k_walls(SphereDev D) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= D.n) return;
     if (!(D.im[s] > 0)) return;
-    const float *P = D.params;
+    const Params P = load_params(D.params);
     float ext[3] = {P[SX], P[SY], P[SZ]};
     float rs = D.r[s];
     Held held = hold(D, s);
@@ -763,7 +851,7 @@ __global__ void k_bounce(SphereDev D) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= D.n) return;
     int s = GRID ? D.order[t] : t;
-    const float *P = D.params;
+    const Params P = load_params(D.params);
     float e = P[RESTITUTION];
     float wa = D.im[s];
     if (!(e > 0 && wa > 0)) return;
@@ -846,7 +934,7 @@ __global__ void k_jacobi_solve(SphereDev D) {
     } else {
         for (int o = 0; o < D.n; o++) against(o);
     }
-    const float *P = D.params;
+    const Params P = load_params(D.params);
     float omega = P[OMEGA];
     float scale = omega;
     if (P[AVERAGED] > 0) scale = omega / fmax_(contacts, 1.0f);
@@ -860,7 +948,7 @@ __global__ void k_jacobi_apply(SphereDev D) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= D.n) return;
     if (!(D.im[s] > 0)) return;
-    const float *P = D.params;
+    const Params P = load_params(D.params);
     float ext[3] = {P[SX], P[SY], P[SZ]};
     float rs = D.r[s];
     float cor[3] = {D.cx[s], D.cy[s], D.cz[s]};
@@ -883,7 +971,7 @@ __global__ void k_show(SphereDev D) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= D.n) return;
     int s = GRID ? D.order[t] : t;
-    const float *P = D.params;
+    const Params P = load_params(D.params);
     float ext[3] = {P[SX], P[SY], P[SZ]};
     float *b = D.shown + (size_t)s * SHOWN_STRIDE;
     float mine[3] = {D.x[s], D.y[s], D.z[s]};
@@ -1200,18 +1288,23 @@ void launch_round(const SphereDev &D, const RoundConst &rc, hipStream_t s) {
 }
 
 void launch_step_begin(PassCtl *ctl, PassSlot *slots, int nslots, hipStream_t s) {
-    k_step_begin<<<blocks_for(nslots), THREADS, 0, s>>>(ctl, slots, nslots);
+    int words = nslots * (int)(sizeof(PassSlot) / sizeof(int));
+    k_step_begin<<<blocks_for(words), THREADS, 0, s>>>(ctl, slots, words);
     check_launch("step begin");
 }
 
 void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLists &L, int32_t *opening,
                  int32_t *enough, int it, int iterations, bool rub, int wide, hipStream_t s) {
-    constexpr int T = 1;  // every round with an open contact runs wide; the tail finishes what `wide` leaves
-    int blocks = std::min(blocks_for(D.cap), 2048);
-    for (int j = 0; j < wide; j++)
-        k_round_wide<<<blocks, THREADS, 0, s>>>(D, ctl, slot, L, it, iterations, rub ? 1 : 0, j, T);
+    if (wide < 0 || wide > MAX_WIDE) throw std::runtime_error("pass: wide rounds out of range");
+    if ((int64_t)L.segments * SEG < D.cap) throw std::runtime_error("pass: open lists shorter than the contact list");
     int limit = D.cap + 2 * CYCLE;
-    k_round_tail<<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, rub ? 1 : 0, limit);
+    for (int j = 0; j < wide; j++) {
+        int last = j == wide - 1 ? 1 : 0;
+        if (rub) k_round_wide<true><<<L.segments, THREADS, 0, s>>>(D, ctl, slot, L, it, iterations, j, last);
+        else k_round_wide<false><<<L.segments, THREADS, 0, s>>>(D, ctl, slot, L, it, iterations, j, last);
+    }
+    if (rub) k_round_tail<true><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide, limit);
+    else k_round_tail<false><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide, limit);
     check_launch("pass");
 }
 

@@ -182,8 +182,11 @@ struct vr_world {
             nslots = 2 * d.substeps * d.iterations + 2;
             slots = alloc<PassSlot>(nslots);
             HIP_OK(hipHostMalloc((void **)&ctlHost, sizeof(PassCtl), hipHostMallocDefault));
-            openLists.buf[0] = alloc<int32_t>(cap);
-            openLists.buf[1] = alloc<int32_t>(cap);
+            openLists.segments = (cap + SEG - 1) / SEG;
+            for (int k = 0; k < 2; k++) {
+                openLists.buf[k] = alloc<int4>((int64_t)openLists.segments * SEG);
+                openLists.seg[k] = alloc<int32_t>(openLists.segments);
+            }
             devOpening = alloc<int32_t>(d.iterations);
             devEnough = alloc<int32_t>(d.iterations);
             HIP_OK(hipMemcpy(devOpening, opening.data(), sizeof(int32_t) * d.iterations, hipMemcpyHostToDevice));
@@ -390,10 +393,11 @@ struct vr_world {
         rep->rounds = ctlHost->rounds;
         rep->most_rounds = ctlHost->most;
         rep->unlisted = ctlHost->unlisted;
-        // Wide rounds for the next step: as many as ran wide in this one, and two more.
-        for (int it = 0; it < d.iterations && it < 32; it++) {
-            wide[it] = std::min(ctlHost->wideRan[it] + 1, 64);
-            wideRub[it] = std::min(ctlHost->wideRan[d.iterations + it] + 1, 64);
+        // Wide rounds for the next step: as many as this one needed, and one more.
+        for (int it = 0; it < d.iterations; it++) {
+            int parted = it, rubbed = d.iterations + it;
+            if (parted < MAX_WIDE) wide[it] = std::min(ctlHost->wideRan[parted] + 1, MAX_WIDE);
+            if (rubbed < MAX_WIDE) wideRub[it] = std::min(ctlHost->wideRan[rubbed] + 1, MAX_WIDE);
         }
     }
 
