@@ -577,7 +577,7 @@ struct World {
         cnt[LISTED] = 0;
     }
 
-    // Spheres.listContacts, Spheres.java:961-1022, one invocation a sphere.
+    // Spheres.listContacts, Spheres.java:961-1022. Invocations in order a = 0, 1, ...
     void list_contacts() {
         float *at[3] = {F_x, F_y, F_z}, *r = F_r;
         int32_t *keys = I_keys, *cnt = I_contactCount;
@@ -585,10 +585,7 @@ struct World {
                 *cH = I_contactHead, *kB = I_keptB, *kN = I_keptNext, *kH = I_keptHead;
         float *ci = F_contactImpulse, *ki = F_keptImpulse;
         float m = one_plus_margin();
-        // Spheres in cell order (the sort's order): the reference appends contacts as its invocations meet them, in
-        // no fixed order; this restatement fixes one, the order the HIP path lays its rows out in.
-        for (int t = 0; t < n; t++) {
-            int a = I_order[t];
+        for (int a = 0; a < n; a++) {
             float mine[3] = {at[0][a], at[1][a], at[2][a]};
             float ra = r[a];
             around(keys[a], [&](int o) {

@@ -39,17 +39,13 @@ def test_listed_pairs_are_the_brute_force_set(backend, seed):
     listed = int(cnt[0])
     a = s.ints("contactA")[:listed]
     b = s.ints("contactB")[:listed]
-    keys = s.ints("keys")
     got = list(zip(a.tolist(), b.tolist()))
     s.close()
     want = brute_force(x, y, z, r)
     assert len(got) == len(set(got)), "a pair listed twice"
     assert set(got) == want
-    assert np.all(a < b)
-    # Rows by lower sphere: each sphere's contacts in one run; the rows in the spheres' cell order.
-    runs = 1 + int(np.count_nonzero(np.diff(a)))
-    assert runs == len(np.unique(a))
-    assert np.all(np.diff(keys[a]) >= 0)
+    # Rows by lower sphere, each in the 27-cell walk's order: a is non-decreasing.
+    assert np.all(np.diff(a) >= 0)
 
 
 @pytest.mark.parametrize("backend", BACKENDS)

@@ -3,8 +3,7 @@
 // State is structure-of-arrays in HBM: one f32 array per per-sphere field (x, y, z, u, v, w, r, im, cx..cz, dx..dz,
 // ax..az, tx..tz, qx..qw), so every per-sphere pass reads and writes whole cache lines with coalesced loads.
 // The contact list is compressed rows by lower sphere: sphere a's contacts occupy slots off[a] .. off[a]+cnt[a]-1,
-// in the order the 27-cell walk meets them, and the rows follow the spheres' cell order. Two such lists are kept,
-// the substep's and the one before it, and swap.
+// in the order the 27-cell walk meets them. Two such lists are kept, the substep's and the one before it, and swap.
 #pragma once
 
 #include <hip/hip_runtime.h>

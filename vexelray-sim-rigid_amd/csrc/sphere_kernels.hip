@@ -482,10 +482,8 @@ __device__ __forceinline__ bool near_pair(const SphereDev &D, const float mine[3
     return d2 < near * near;
 }
 
-// Threads take spheres in cell order, so a wave walks neighbouring cells together; the rows are laid out in that
-// order too (byCell[t]: the count, then the first slot, of the t-th sphere in cell order), so contacts that share a
-// sphere sit near each other in the list.
-__global__ void k_list_count(SphereDev D, int32_t *byCell) {
+__global__ void k_list_count(SphereDev D) {
+    // Threads take spheres in cell order, so a wave walks neighbouring cells together.
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= D.n) return;
     int a = D.order[t];
@@ -496,17 +494,16 @@ __global__ void k_list_count(SphereDev D, int32_t *byCell) {
         if (o > a && near_pair(D, mine, ra, o)) found++;
     });
     D.cur.cnt[a] = found;
-    byCell[t] = found;
 }
 
-__global__ void k_list_write(SphereDev D, const int32_t *byCell) {
+__global__ void k_list_write(SphereDev D) {
+    // Threads take spheres in cell order, so a wave walks neighbouring cells together.
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= D.n) return;
     int a = D.order[t];
     float mine[3] = {D.x[a], D.y[a], D.z[a]};
     float ra = D.r[a];
-    int slot = byCell[t];
-    D.cur.off[a] = slot;
+    int slot = D.cur.off[a];
     int cap = D.cap;
     int kStart = D.kept.off[a];
     int kEnd = min(kStart + D.kept.cnt[a], cap);
@@ -1278,12 +1275,10 @@ void launch_clear_contacts(const SphereDev &D, hipStream_t s) {
 }
 
 void launch_list(const SphereDev &D, const SortScratch &S, hipStream_t s) {
-    // The sort's key buffer is free once the cells are sorted: it holds the rows' counts, then their first slots.
-    int32_t *byCell = S.keysA;
-    k_list_count<<<blocks_for(D.n), THREADS, 0, s>>>(D, byCell);
+    k_list_count<<<blocks_for(D.n), THREADS, 0, s>>>(D);
     check_launch("list count");
-    scan_exclusive(byCell, byCell, D.n, D.count + LISTED, S.scanTmp, s);
-    k_list_write<<<blocks_for(D.n), THREADS, 0, s>>>(D, byCell);
+    scan_exclusive(D.cur.cnt, D.cur.off, D.n, D.count + LISTED, S.scanTmp, s);
+    k_list_write<<<blocks_for(D.n), THREADS, 0, s>>>(D);
     check_launch("list write");
 }
 
