@@ -448,3 +448,42 @@ def test_ball_rests_on_a_box(backend):
         assert abs(0.15 - s.y[1]) <= 5e-4
         assert math.hypot(s.x[1] - 0.5, s.z[1] - 0.5) < 1e-4
         assert st.kinetic < 1e-6
+
+
+def crowded_body(n=80):
+    """n small balls resting on one big ball, the big ball last: each small ball lists the big one as its partner,
+    so the big ball is in n touching pairs, one listed by each. First fit in slot order gives the first 32 of them
+    rounds 0..31 and leaves the rest unplaced (BoxStepper.java:71-90)."""
+    s = BoxScene(n + 1)
+    s.gy = 0.0
+    s.sx = s.sy = s.sz = 2.0
+    big = n
+    s.sphere(big, 0.5)
+    s.x[big] = s.y[big] = s.z[big] = 1.0
+    golden = math.pi * (3 - math.sqrt(5))
+    for k in range(n):
+        s.sphere(k, 0.06)
+        yk = 1 - 2 * (k + 0.5) / n
+        rk = math.sqrt(1 - yk * yk)
+        dist = 0.5 + 0.06 - 0.002
+        s.x[k] = np.float32(1.0 + dist * rk * math.cos(golden * k))
+        s.y[k] = np.float32(1.0 + dist * yk)
+        s.z[k] = np.float32(1.0 + dist * rk * math.sin(golden * k))
+    return s
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_crowded_body_fills_every_round_and_leaves_the_rest_unplaced(backend):
+    with crowded_body() as s:
+        s.substeps = 1
+        s.start(backend).advance(1)
+        r = s.last
+        assert r.touching == 80
+        assert r.rounds == 32
+        assert r.unplaced == 80 - 32
+        rounds = s.ints("rounds")
+        width = len(rounds) // 32
+        firsts = [int(rounds[k * width]) for k in range(32)]
+        # Round k holds the k-th small ball's pair with the big one (slot k * 32), and nothing else.
+        assert firsts == [k * 32 for k in range(32)]
+        assert all(int(rounds[k * width + 1]) == -1 for k in range(32))

@@ -81,17 +81,19 @@ def run_both(make, steps):
         for _ in range(steps):
             r = s.sim.stepAndWait()
             reps.append((r.touching, r.points, r.rounds, r.unplaced, r.unlisted))
-        out[backend] = ({k: s.floats(k).copy() for k in STATE}, reps, s.ints("contacts").copy())
+        out[backend] = ({k: s.floats(k).copy() for k in STATE}, reps, s.ints("contacts").copy(),
+                        s.ints("rounds").copy())
         s.close()
     return out["oracle"], out["gpu"]
 
 
 @pytest.mark.parametrize("spheres", [False, True])
 def test_mixed_pile_matches_oracle(spheres):
-    (sa, ra, ca), (sb, rb, cb) = run_both(lambda: mixed_pile(spheres=spheres), 40)
+    (sa, ra, ca, qa), (sb, rb, cb, qb) = run_both(lambda: mixed_pile(spheres=spheres), 40)
     assert ra == rb
     assert any(r[0] > 0 for r in ra), "the pile should touch"
     assert np.array_equal(ca, cb)
+    assert np.array_equal(qa, qb), "the rounds (colouring) differ"
     for k in STATE:
         scale = np.maximum(np.abs(sa[k]), 1e-3)
         assert float(np.max(np.abs(sa[k] - sb[k]) / scale)) <= 1e-4, k
@@ -104,7 +106,22 @@ def test_symmetric_sweeps_match_oracle():
         s.symmetric = True
         s.passes = 3
         return s
-    (sa, ra, _), (sb, rb, _) = run_both(make, 20)
+    (sa, ra, _, qa), (sb, rb, _, qb) = run_both(make, 20)
+    assert np.array_equal(qa, qb)
     assert ra == rb
+    for k in STATE:
+        assert np.array_equal(sa[k].view(np.int32), sb[k].view(np.int32)), k
+
+
+def test_big_pile_colouring_matches_oracle():
+    """A denser pile of 600 bodies over 10 steps: the colouring (rounds buffer) and the counts, step by step."""
+    def make():
+        s = mixed_pile(600, seed=11)
+        s.sx = s.sz = 2.0
+        s.sy = 3.0
+        return s
+    (sa, ra, ca, qa), (sb, rb, cb, qb) = run_both(make, 10)
+    assert ra == rb
+    assert np.array_equal(qa, qb)
     for k in STATE:
         assert np.array_equal(sa[k].view(np.int32), sb[k].view(np.int32)), k

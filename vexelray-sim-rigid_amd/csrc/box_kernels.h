@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "sphere_kernels.h"
+
 namespace vrbox {
 
 struct BoxDev {
@@ -20,6 +22,34 @@ struct BoxDev {
     int32_t *pairs, *pairsWas, *contacts, *rounds;
     float *manifold, *manifoldWas, *impulses, *impulsesWas;
 };
+
+// The device's colouring of the touching pairs into rounds (box_colour.hip).
+constexpr int COLOUR_FLAGS = 64;
+enum { REPORT_POINTS = 0, REPORT_UNLISTED = 1, REPORT_WORDS = 2 };
+
+struct Colouring {
+    int most;                // slots: the most touching pairs there can be
+    int32_t *colour;         // slots: a touching pair's round, MOST_ROUNDS if unplaced
+    int32_t *decided;        // slots: the step of the colouring that decided it, INT_MAX while open
+    int32_t *touch;          // slots: the touching pairs in slot order
+    int32_t *touchSeg;       // slots / 256 + 1: touching pairs a workgroup of slots, scanned to where they go
+    int32_t *touching;       // 1: how many touch
+    int32_t *incDeg, *incPut;  // n: touching pairs at a body, and how many of them are placed
+    int32_t *incOff;         // n + 1
+    int32_t *inc;            // 2 * slots: each body's touching pairs
+    int32_t *flags;          // COLOUR_FLAGS: flags[i % COLOUR_FLAGS] set when step i left a pair open
+    int32_t *keys, *order;   // slots: the sort of the touching list by round
+    int32_t *starts;         // MOST_ROUNDS + 2: where each round's pairs begin in that order
+    int32_t *report;         // REPORT_WORDS: points, unlisted
+    int32_t *sumTmp;         // colour_tmp_words(slots)
+    vr::SortScratch sort;    // sized for slots keys
+};
+
+int64_t colour_tmp_words(int64_t slots);
+void launch_colour_prepare(const BoxDev &D, const Colouring &C, const vr::SortScratch &S, hipStream_t s);
+void launch_colour_steps(const BoxDev &D, const Colouring &C, int touchingPairs, int from, int count,
+                         hipStream_t s);
+void launch_colour_finish(const BoxDev &D, const Colouring &C, int touchingPairs, hipStream_t s);
 
 void launch_velocities(const BoxDev &D, hipStream_t s);
 void launch_positions(const BoxDev &D, hipStream_t s);

@@ -2,8 +2,9 @@
 //
 // The schedule is the reference's BoxStepper.step (core/.../box/BoxStepper.java:56-100) over BoxStep's pass lists
 // (core/.../box/BoxStep.java:138-153, 199-231): keep, bin and sort, list, collide, carry and counts; the touching
-// pairs coloured greedily in slot order into rounds (first fit, at most 32); then each substep velocities, the warm
-// sweep, the solve sweeps, positions and the relax sweeps, each sweep every round of pairs then the walls.
+// pairs coloured into rounds as the reference's host colours them (first fit in slot order, at most 32; here on the
+// device, box_colour.hip); then each substep velocities, the warm sweep, the solve sweeps, positions and the relax
+// sweeps, each sweep every round of pairs then the walls.
 #include "box_world.h"
 
 #include <algorithm>
@@ -40,8 +41,9 @@ struct BoxWorld {
     BoxDev D{};
     vr::SphereDev sortView{};  // the sphere kernels' grid sort reused: keys, order, starts over the same cells
     vr::SortScratch S{};
-    std::vector<int32_t> hostContacts, hostRounds;
-    std::vector<uint8_t> taken;
+    Colouring C{};
+    int32_t *pinned = nullptr;  // host: the colouring's counts read back
+    int colourSteps = 1;        // steps of the colouring the last step needed
 
     template <class T>
     T *alloc(int64_t count) {
@@ -130,9 +132,30 @@ struct BoxWorld {
             S.keysB = alloc<int32_t>(n);
             S.hist = alloc<int32_t>(256 * (int64_t)nblocks);
             S.scanTmp = alloc<int32_t>(std::max(vr::scan_tmp_words(256 * nblocks), vr::scan_tmp_words(n)) + 64);
-            hostContacts.resize((size_t)2 * slots + n);
-            hostRounds.resize((size_t)MOST_ROUNDS * D.width);
-            taken.resize((size_t)MOST_ROUNDS * n);
+            C.most = (int)slots;
+            C.colour = alloc<int32_t>(slots);
+            C.decided = alloc<int32_t>(slots);
+            C.touch = alloc<int32_t>(slots);
+            C.touchSeg = alloc<int32_t>(slots / 256 + 2);
+            C.touching = alloc<int32_t>(1);
+            C.incDeg = alloc<int32_t>(n);
+            C.incPut = alloc<int32_t>(n);
+            C.incOff = alloc<int32_t>((int64_t)n + 1);
+            C.inc = alloc<int32_t>(2 * slots);
+            C.flags = alloc<int32_t>(COLOUR_FLAGS);
+            C.keys = alloc<int32_t>(slots);
+            C.order = alloc<int32_t>(slots);
+            C.starts = alloc<int32_t>(MOST_ROUNDS + 2);
+            C.report = alloc<int32_t>(REPORT_WORDS);
+            C.sumTmp = alloc<int32_t>(colour_tmp_words(slots));
+            int sortBlocks = (int)((slots + 2047) / 2048);
+            C.sort.maxBlocks = sortBlocks;
+            C.sort.keysA = alloc<int32_t>(slots);
+            C.sort.valsA = alloc<int32_t>(slots);
+            C.sort.keysB = alloc<int32_t>(slots);
+            C.sort.hist = alloc<int32_t>(256 * (int64_t)sortBlocks);
+            C.sort.scanTmp = alloc<int32_t>(std::max(vr::scan_tmp_words(256 * sortBlocks), vr::scan_tmp_words((int)slots)) + 64);
+            HIP_OK(hipHostMalloc((void **)&pinned, 64 * sizeof(int32_t), hipHostMallocDefault));
         }
         clear();
     }
@@ -173,6 +196,39 @@ struct BoxWorld {
         HIP_OK(hipStreamSynchronize(stream));
     }
 
+    int32_t read_word(const int32_t *p) {
+        HIP_OK(hipMemcpyAsync(pinned, p, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return pinned[0];
+    }
+
+    // The touching pairs coloured into rounds, run until every pair has its round: the steps the last step needed,
+    // then a few more at a time, reading back whether the last left a pair open.
+    void colour(int *used, int *touching, int *points, int *unplaced, int *unlisted) {
+        launch_colour_prepare(D, C, S, stream);
+        int pairs = read_word(C.touching);
+        int done = 0, batch = std::max(colourSteps, 1);
+        while (pairs > 0) {
+            launch_colour_steps(D, C, pairs, done, batch, stream);
+            done += batch;
+            if (read_word(C.flags + (done - 1) % COLOUR_FLAGS) == 0) break;
+            if (done > pairs) throw std::runtime_error("colouring: pairs still open after a step for every pair");
+            batch = 4;
+        }
+        colourSteps = std::max(done, 1);
+        launch_colour_finish(D, C, pairs, stream);
+        HIP_OK(hipMemcpyAsync(pinned, C.starts, (MOST_ROUNDS + 2) * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipMemcpyAsync(pinned + 40, C.report, REPORT_WORDS * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        *used = 0;
+        for (int r = 0; r < MOST_ROUNDS; r++)
+            if (pinned[r + 1] > pinned[r]) *used = r + 1;
+        *touching = pairs;
+        *unplaced = pinned[MOST_ROUNDS + 1] - pinned[MOST_ROUNDS];
+        *points = pinned[40 + REPORT_POINTS];
+        *unlisted = pinned[40 + REPORT_UNLISTED];
+    }
+
     // BoxStep.sweep, BoxStep.java:217-231.
     void sweep(int kind, bool reverse, int rounds) {
         if (reverse) launch_walls(D, kind, reverse, stream);
@@ -195,34 +251,8 @@ struct BoxWorld {
             launch_list(D, stream);
             launch_collide(D, stream);
             launch_carry_counts(D, stream);
-            // BoxStepper: the contacts read back and coloured on the host, first fit in slot order.
-            HIP_OK(hipMemcpyAsync(hostContacts.data(), D.contacts, hostContacts.size() * 4, hipMemcpyDeviceToHost,
-                                  stream));
-            HIP_OK(hipStreamSynchronize(stream));
-            int slots = D.slots, width = D.width;
-            std::fill(taken.begin(), taken.end(), 0);
-            std::fill(hostRounds.begin(), hostRounds.end(), -1);
-            std::vector<int> filled(MOST_ROUNDS, 0);
-            for (int p = 0; p < slots; p++) {
-                int b = hostContacts[p];
-                if (b < 0) continue;
-                int a = p / SLOTS;
-                touching++;
-                points += hostContacts[(size_t)slots + p];
-                int r = 0;
-                while (r < MOST_ROUNDS && (taken[(size_t)r * n + a] || taken[(size_t)r * n + b])) r++;
-                if (r == MOST_ROUNDS) {
-                    unplaced++;
-                } else {
-                    taken[(size_t)r * n + a] = taken[(size_t)r * n + b] = 1;
-                    hostRounds[(size_t)r * width + filled[r]++] = p;
-                    used = std::max(used, r + 1);
-                }
-            }
-            for (int s = 0; s < n; s++) unlisted += std::max(0, hostContacts[2 * (size_t)slots + s] - SLOTS);
-            if (used > 0)
-                HIP_OK(hipMemcpyAsync(D.rounds, hostRounds.data(), hostRounds.size() * 4, hipMemcpyHostToDevice,
-                                      stream));
+            // BoxStepper's colouring (first fit in slot order), on the device; the host reads back the counts.
+            colour(&used, &touching, &points, &unplaced, &unlisted);
         }
         for (int sub = 0; sub < d.substeps; sub++) {
             launch_velocities(D, stream);
@@ -262,6 +292,7 @@ struct BoxWorld {
 
     ~BoxWorld() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (pinned) (void)hipHostFree(pinned);
         for (void *p : allocs) (void)hipFree(p);
         if (t0) (void)hipEventDestroy(t0);
         if (t1) (void)hipEventDestroy(t1);

@@ -9,7 +9,14 @@ Each builder returns a SceneSpec: the SphereStep to make, the initial state, and
   walls), velocity N(0, 1) m/s per axis; mu = 0.4, e = 0.1; dt = 1/60 s in 8 substeps, 10 passes, 1 m cells. More
   spheres than the reference's 31-bit contact priorities order, so the step uses 63-bit ones (wide pairs).
 
-Draw order per sphere: radius (if drawn), x, y, z, then the three velocity components.
+* mixed_32k: the mixed-shape pile, on the box path. 16,384 oriented boxes (half-extents uniform in [0.2, 1.0] m per
+  axis, orientation a normalised Gaussian quaternion), 16,384 spheres (r uniform in [0.2, 0.6] m), density
+  1000 kg/m^3, dropped at rest from heights 22-70 m into a 128 x 72 x 128 m walled box, beside 32 towers of 20
+  stacked unit cubes (half-extent 0.5 m) standing on the floor in an 8 x 4 grid 16 m apart; mu = 0.6, margin
+  0.005 m; dt = 1/60 s in 8 substeps of 20 sweeps, broad-phase cells of the widest bounding sphere plus margin.
+
+Draw order per sphere: radius (if drawn), x, y, z, then the three velocity components. Per body of mixed_32k:
+shape (box, then sphere, alternating), extents (3 halves, or the radius), quaternion (boxes: 4 Gaussians), x, y, z.
 """
 
 import math
@@ -18,6 +25,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from .javarandom import JavaRandom
+from .boxes import BoxBroad, Boxes, BoxStep
 from .spheres import Solve, SphereGrid, SphereStep, Spheres
 
 
@@ -85,4 +93,78 @@ def pile_64k(seed: int = 2, n: int = 65_536) -> SceneSpec:
                      v32[:, 0].copy(), v32[:, 1].copy(), v32[:, 2].copy(), r32, im, box, 600, dt, seed)
 
 
-SCENES = {"pile_1k": pile_1k, "pile_64k": pile_64k}
+@dataclass
+class BoxSceneSpec:
+    """A box-path workload: the BoxStep to make, the initial state (BoxSimulation.start's arguments), the params."""
+    name: str
+    step: BoxStep
+    params: np.ndarray
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    spin: np.ndarray   # 3 x n angular velocity
+    q: np.ndarray      # 4 x n
+    half: np.ndarray   # 3 x n
+    shape: np.ndarray
+    im: np.ndarray
+    box: tuple
+    steps: int
+    dt: float
+    seed: int
+    towers: int
+
+
+def mixed_32k(seed: int = 3, boxes: int = 16_384, spheres: int = 16_384, towers: int = 32, tall: int = 20,
+              box=(128.0, 72.0, 128.0), substeps: int = 8, sweeps: int = 20) -> BoxSceneSpec:
+    rnd = JavaRandom(seed)
+    loose = boxes + spheres
+    n = loose + towers * tall
+    half = np.zeros((3, n), dtype=np.float64)
+    q = np.zeros((4, n), dtype=np.float64)
+    q[3] = 1.0
+    pos = np.zeros((3, n), dtype=np.float64)
+    shape = np.zeros(n, dtype=np.float32)
+    mass = np.zeros(n, dtype=np.float64)
+    b = s = 0
+    for k in range(loose):
+        sphere = (k % 2 == 1 and s < spheres) or b >= boxes
+        if sphere:
+            s += 1
+            r = 0.2 + 0.4 * rnd.nextDouble()
+            half[:, k] = r
+            shape[k] = Boxes.SPHERE_SHAPE
+            mass[k] = _mass(r)
+        else:
+            b += 1
+            for a in range(3):
+                half[a, k] = 0.2 + 0.8 * rnd.nextDouble()
+            g = [rnd.nextGaussian() for _ in range(4)]
+            length = math.sqrt(sum(v * v for v in g))
+            q[:, k] = [v / length for v in g]
+            mass[k] = 1000.0 * 8.0 * half[0, k] * half[1, k] * half[2, k]
+        pos[0, k] = 1.0 + (box[0] - 2.0) * rnd.nextDouble()
+        pos[1, k] = 22.0 + 48.0 * rnd.nextDouble()
+        pos[2, k] = 1.0 + (box[2] - 2.0) * rnd.nextDouble()
+    for t in range(towers):
+        cx, cz = 8.0 + 16.0 * (t % 8), 8.0 + 16.0 * (t // 8)
+        for level in range(tall):
+            k = loose + t * tall + level
+            half[:, k] = 0.5
+            pos[:, k] = (cx, 0.5 + level, cz)
+            mass[k] = 1000.0
+    widest = float(np.max(np.sqrt((half ** 2).sum(axis=0))))
+    margin = 0.005
+    step = BoxStep(n, substeps, sweeps, False, BoxBroad.grid(*box, widest, margin))
+    dt = 1.0 / 60
+    params = Boxes.params(dt / substeps, 0, -9.81, 0, *box, 0.6, 0.6, margin)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    zeros = np.zeros(n, dtype=np.float32)
+    return BoxSceneSpec("mixed_32k", step, params, f32(pos[0]), f32(pos[1]), f32(pos[2]), zeros, zeros.copy(),
+                        zeros.copy(), np.zeros((3, n), dtype=np.float32), f32(q), f32(half), shape,
+                        f32(1.0 / mass), box, 600, dt, seed, towers)
+
+
+SCENES = {"pile_1k": pile_1k, "pile_64k": pile_64k, "mixed_32k": mixed_32k}
