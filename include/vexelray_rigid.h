@@ -126,6 +126,17 @@ typedef struct vr_profile {
 } vr_profile;
 
 int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out);
+/*
+ * A batch (no counterpart in the reference, which steps one scene a world): `scenes` independent box scenes in one
+ * world, stepped together. Scene s holds bodies[s] bodies, consecutive from the sum of the counts before it, in a
+ * walled box extents[3s .. 3s+2] (the reference's box world's params sx, sy, sz, which a batch then ignores), with
+ * a broad-phase grid of grids[3s .. 3s+2] cells of cells[s] metres (BoxBroad.grid). desc holds the step
+ * (bodies = the total, substeps, sweeps, symmetric; has_grid = 1). Buffers are the box world's, over all bodies,
+ * and one set of params serves every scene. Pairs never join two scenes and the rounds are coloured in slot order,
+ * so each scene steps exactly as it would alone; the report adds the scenes' counts up (rounds: the most).
+ */
+int vr_batch_create(const vr_step_desc *desc, int32_t scenes, const int32_t *bodies, const float *extents,
+                    const int32_t *grids, const float *cells, vr_world **out);
 int vr_world_clear(vr_world *w);
 int vr_world_write(vr_world *w, const char *name, const void *src, int64_t count);
 int vr_world_read(vr_world *w, const char *name, void *dst, int64_t count);

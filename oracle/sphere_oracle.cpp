@@ -1102,8 +1102,13 @@ struct vro_world {
     int kind = VR_SPHERES;
     World w;
     vro::BoxWorld *box = nullptr;
+    // A batch: each scene its own box world, as the reference would step it alone, walled by its own box.
+    std::vector<vro::BoxWorld *> batch;
+    std::vector<int64_t> batchFirst;  // first body of each scene, and the total after the last
+    std::vector<float> batchBox;      // 3 a scene
     ~vro_world() {
         if (box) vro::box_destroy(box);
+        for (auto *b : batch) vro::box_destroy(b);
     }
 };
 
@@ -1153,20 +1158,92 @@ int vro_world_create(int32_t kind, const vr_step_desc *desc, vro_world **out) {
     return VR_OK;
 }
 
+int vro_batch_create(const vr_step_desc *desc, int32_t scenes, const int32_t *bodies, const float *extents,
+                     const int32_t *grids, const float *cells, vro_world **out) {
+    if (!desc || !out || !bodies || !extents || !grids || !cells) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (scenes < 1) return fail(VR_ERR_ARGUMENT, "a batch needs a scene");
+    auto *p = new vro_world();
+    p->kind = VR_BOXES;
+    int rc = guard([&] {
+        int64_t total = 0;
+        p->batchFirst.push_back(0);
+        for (int k = 0; k < scenes; k++) {
+            vr_step_desc one = *desc;
+            one.bodies = bodies[k];
+            one.has_grid = 1;
+            one.nx = grids[3 * k];
+            one.ny = grids[3 * k + 1];
+            one.nz = grids[3 * k + 2];
+            one.cell = cells[k];
+            try {
+                p->batch.push_back(vro::box_create(one));
+            } catch (const std::invalid_argument &e) {
+                throw ArgError(e.what());
+            }
+            total += bodies[k];
+            p->batchFirst.push_back(total);
+            for (int a = 0; a < 3; a++) p->batchBox.push_back(extents[3 * k + a]);
+        }
+        if (total != desc->bodies) throw ArgError("the scenes' bodies do not add up to the step's");
+    });
+    if (rc != VR_OK) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return VR_OK;
+}
+
 int vro_world_clear(vro_world *p) {
     return guard([&] {
-        if (p->kind == VR_BOXES) vro::box_clear(p->box);
+        if (!p->batch.empty())
+            for (auto *b : p->batch) vro::box_clear(b);
+        else if (p->kind == VR_BOXES) vro::box_clear(p->box);
         else p->w.clear();
     });
 }
 
 int vro_world_buffer_info(vro_world *p, const char *name, int64_t *count, int32_t *dtype) {
     void *ptr;
+    if (!p->batch.empty()) {  // a batch's per-body buffers, over all its bodies
+        int64_t size;
+        int64_t first = p->batchFirst[1] - p->batchFirst[0];
+        if (vro::box_buffer(p->batch[0], name, &ptr, &size, dtype) && size == first) {
+            *count = p->batchFirst.back();
+            return VR_OK;
+        }
+        return fail(VR_ERR_NAME, std::string("a batch has no per-body buffer named ") + name);
+    }
     if (find_buffer(p, name, &ptr, count, dtype)) return VR_OK;
     return fail(VR_ERR_NAME, std::string("no buffer named ") + name);
 }
 
+namespace {
+
+// A batch's per-body buffer, split over (write) or gathered from (read) its scenes' worlds.
+int batch_copy(vro_world *p, const char *name, void *host, int64_t count, bool write) {
+    int64_t total = p->batchFirst.back();
+    if (count < 0 || count > total) return fail(VR_ERR_NAME, std::string("buffer ") + name + " is shorter");
+    for (size_t k = 0; k < p->batch.size(); k++) {
+        void *ptr;
+        int64_t size;
+        int32_t dtype;
+        if (!vro::box_buffer(p->batch[k], name, &ptr, &size, &dtype) ||
+            size != p->batchFirst[k + 1] - p->batchFirst[k])
+            return fail(VR_ERR_NAME, std::string("a batch has no per-body buffer named ") + name);
+        int64_t from = p->batchFirst[k], to = std::min(p->batchFirst[k + 1], count);
+        if (to <= from) continue;
+        char *h = (char *)host + from * 4;
+        if (write) std::memcpy(ptr, h, (size_t)(to - from) * 4);
+        else std::memcpy(h, ptr, (size_t)(to - from) * 4);
+    }
+    return VR_OK;
+}
+
+}  // namespace
+
 int vro_world_write(vro_world *p, const char *name, const void *src, int64_t count) {
+    if (!p->batch.empty()) return batch_copy(p, name, const_cast<void *>(src), count, true);
     void *dst;
     int64_t size;
     int32_t dtype;
@@ -1177,6 +1254,7 @@ int vro_world_write(vro_world *p, const char *name, const void *src, int64_t cou
 }
 
 int vro_world_read(vro_world *p, const char *name, void *dst, int64_t count) {
+    if (!p->batch.empty()) return batch_copy(p, name, dst, count, false);
     void *src;
     int64_t size;
     int32_t dtype;
@@ -1187,6 +1265,20 @@ int vro_world_read(vro_world *p, const char *name, void *dst, int64_t count) {
 }
 
 int vro_world_params(vro_world *p, const int32_t *words, int32_t n) {
+    if (!p->batch.empty()) {
+        if (n != 18) return fail(VR_ERR_ARGUMENT, "box parameters are 18 words");
+        return guard([&] {
+            for (size_t k = 0; k < p->batch.size(); k++) {
+                std::vector<int32_t> own(words, words + n);
+                for (int a = 0; a < 3; a++) std::memcpy(&own[4 + a], &p->batchBox[3 * k + a], 4);  // SX, SY, SZ
+                try {
+                    vro::box_params(p->batch[k], own.data(), n);
+                } catch (const std::invalid_argument &e) {
+                    throw ArgError(e.what());
+                }
+            }
+        });
+    }
     if (p->kind == VR_BOXES) {
         return guard([&] {
             try {
@@ -1210,6 +1302,23 @@ int vro_world_friction(vro_world *p, int32_t on) {
 
 int vro_world_step(vro_world *p, int32_t wait, vr_step_report *rep) {
     (void)wait;
+    if (!p->batch.empty()) {
+        return guard([&] {
+            vr_step_report local{};
+            vr_step_report *r = rep ? rep : &local;
+            std::memset(r, 0, sizeof(*r));
+            for (auto *b : p->batch) {
+                vr_step_report one{};
+                vro::box_step(b, &one);
+                r->touching += one.touching;
+                r->points += one.points;
+                r->unplaced += one.unplaced;
+                r->unlisted += one.unlisted;
+                r->rounds = std::max(r->rounds, one.rounds);
+                r->most_rounds = std::max(r->most_rounds, one.most_rounds);
+            }
+        });
+    }
     if (p->kind == VR_BOXES) {
         return guard([&] {
             vr_step_report local{};
@@ -1239,7 +1348,7 @@ int vro_last_error(char *buf, int32_t len) {
 }
 
 int vro_box_collide(vro_world *p, const int32_t *pairs, int32_t npairs, float *manifold) {
-    if (p->kind != VR_BOXES) return fail(VR_ERR_STATE, "not a box world");
+    if (p->kind != VR_BOXES || !p->box) return fail(VR_ERR_STATE, "not a box world of one scene");
     vro::box_collide(p->box, pairs, npairs, manifold);
     return VR_OK;
 }

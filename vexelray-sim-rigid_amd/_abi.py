@@ -25,7 +25,7 @@ VR_I32 = 2
 # Every entry point the header declares, for the export check.
 ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "world_buffer_info", "world_params",
                 "world_friction", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
-                "world_profile_read", "box_collide")
+                "world_profile_read", "box_collide", "batch_create")
 
 
 class StepDesc(ctypes.Structure):
@@ -76,6 +76,9 @@ class Abi:
             "world_profile": ([p, ctypes.c_int32]),
             "world_profile_read": ([p, ctypes.POINTER(Profile)]),
             "box_collide": ([p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
+            "batch_create": ([ctypes.POINTER(StepDesc), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                              ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32),
+                              ctypes.POINTER(ctypes.c_float), ctypes.POINTER(p)]),
         }
         for name, args in sig.items():
             fn = getattr(self.lib, prefix + name)
@@ -96,6 +99,20 @@ class Abi:
     def create(self, kind, desc, arg_error, state_error) -> "World":
         handle = ctypes.c_void_p()
         rc = self.fn("world_create")(kind, ctypes.byref(desc), ctypes.byref(handle))
+        World.check(self, rc, arg_error, state_error)
+        return World(self, handle, arg_error, state_error)
+
+
+    def create_batch(self, desc, bodies, extents, grids, cells, arg_error, state_error) -> "World":
+        bodies = np.ascontiguousarray(bodies, dtype=np.int32)
+        extents = np.ascontiguousarray(extents, dtype=np.float32).reshape(-1)
+        grids = np.ascontiguousarray(grids, dtype=np.int32).reshape(-1)
+        cells = np.ascontiguousarray(cells, dtype=np.float32)
+        i32, f32 = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)
+        handle = ctypes.c_void_p()
+        rc = self.fn("batch_create")(ctypes.byref(desc), len(bodies), bodies.ctypes.data_as(i32),
+                                     extents.ctypes.data_as(f32), grids.ctypes.data_as(i32),
+                                     cells.ctypes.data_as(f32), ctypes.byref(handle))
         World.check(self, rc, arg_error, state_error)
         return World(self, handle, arg_error, state_error)
 

@@ -177,7 +177,7 @@ class BoxSimulation:
         shape = np.asarray(shape, dtype=np.float32)
         im = np.asarray(im, dtype=np.float32)
         l = np.zeros((3, n), dtype=np.float32)
-        for s in range(n):
+        for s in (np.flatnonzero(np.any(spin != 0, axis=0)) if n else []):  # no spin, no angular momentum
             if im[s] <= 0:
                 continue
             moment = Boxes.inertia(shape[s], 1 / float(im[s]), half[0][s], half[1][s], half[2][s])
@@ -233,6 +233,35 @@ class BoxSimulation:
             self.close()
         except Exception:
             pass
+
+
+class BoxBatch(BoxSimulation):
+    """Many independent box scenes stepped together in one world (vr_batch_create; the reference steps one scene
+    a world). Scene k holds bodies[k] consecutive bodies in a walled box extents[k] = (sx, sy, sz) with broad-phase
+    grid grids[k] (a SphereGrid, as BoxBroad.grid makes for the scene). Buffers, start(), state() and the report
+    are the box world's over all bodies; params() serves every scene, its box ignored for each scene's own."""
+
+    def __init__(self, bodies, extents, grids, substeps: int, sweeps: int = Boxes.WALL_PASSES,
+                 symmetric: bool = False, abi: Optional["_abi.Abi"] = None):
+        self._abi = abi if abi is not None else _abi.product()
+        bodies = [int(b) for b in bodies]
+        if len(bodies) != len(extents) or len(bodies) != len(grids):
+            raise IllegalArgumentException("a batch needs a box and a grid for every scene")
+        self.scenes = len(bodies)
+        self.firsts = np.concatenate([[0], np.cumsum(bodies)]).astype(np.int64)
+        self.step_ = BoxStep(int(self.firsts[-1]), substeps, sweeps, symmetric, SphereGrid(1, 1, 1, 1.0))
+        cells = [[g.nx, g.ny, g.nz] for g in grids]
+        sizes = [g.cell for g in grids]
+        self._world = self._abi.create_batch(self.step_.desc(), bodies, extents, cells, sizes,
+                                             IllegalArgumentException, IllegalStateException)
+        self._last = BoxReport()
+
+    def scene(self, k: int) -> slice:
+        """The bodies of scene k, as a slice of the world's buffers."""
+        return slice(int(self.firsts[k]), int(self.firsts[k + 1]))
+
+    def collide(self, pairs) -> np.ndarray:
+        raise IllegalStateException("the narrow phase alone runs on a world of one scene")
 
 
 @dataclass

@@ -12,9 +12,19 @@
 
 namespace vrbox {
 
+// One scene of a world: its walled box and its broad-phase grid, whose cells are numbered from cellBase in the
+// world's cell order. A world of one scene is the reference's box world; a batch holds many, their bodies
+// consecutive, stepped together (pairs never join bodies of two scenes).
+struct SceneDev {
+    int nx, ny, nz, cellBase;
+    float cell, sx, sy, sz;
+};
+
 struct BoxDev {
-    int n, slots, width, nx, ny, nz, cells;
-    float cell;
+    int n, slots, width, cells;
+    int scenes;
+    const int32_t *scene;       // n: each body's scene
+    const SceneDev *sceneInfo;  // scenes
     float *x[3], *vel[3], *l[3], *q[4], *im, *half[3], *shape;
     const float *params;
     float *lambda, *rub, *roll, *shown;
@@ -52,6 +62,8 @@ void launch_colour_steps(const BoxDev &D, const Colouring &C, int touchingPairs,
 void launch_colour_finish(const BoxDev &D, const Colouring &C, int touchingPairs, hipStream_t s);
 
 void launch_velocities(const BoxDev &D, hipStream_t s);
+// A world of one scene walls it with the params' box: copied into the scene before each step.
+void launch_scene_from_params(const BoxDev &D, SceneDev *scene, hipStream_t s);
 void launch_positions(const BoxDev &D, hipStream_t s);
 void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s);
 void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int r, hipStream_t s);

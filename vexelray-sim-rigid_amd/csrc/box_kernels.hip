@@ -81,7 +81,8 @@ __global__ void k_walls(BoxDev D) {
     if (s >= D.n || !(D.im[s] > 0)) return;
     Bufs B = bufs(D);
     const float *P = D.params;
-    float extent[3] = {P[SX], P[SY], P[SZ]};
+    const SceneDev sc = D.sceneInfo[D.scene[s]];
+    float extent[3] = {sc.sx, sc.sy, sc.sz};
     Moving box = load_moving(B, s);
     Soft soft(P, true);
     size_t base = (size_t)s * LAMBDA_WORDS;
@@ -170,17 +171,18 @@ __global__ void k_solve_round(BoxDev D, int r) {
     store_moving(B, second);
 }
 
-// BoxBroad.bin, BoxBroad.java:74-89.
+// BoxBroad.bin, BoxBroad.java:74-89, in the body's scene's grid.
 __global__ void k_bin(BoxDev D) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= D.n) return;
-    int cellsAxis[3] = {D.nx, D.ny, D.nz};
+    const SceneDev sc = D.sceneInfo[D.scene[s]];
+    int cellsAxis[3] = {sc.nx, sc.ny, sc.nz};
     int c[3];
     for (int a = 0; a < 3; a++) {
-        float fl = floorf(D.x[a][s] / D.cell);
+        float fl = floorf(D.x[a][s] / sc.cell);
         c[a] = (int)mx(0.f, mn((float)(cellsAxis[a] - 1), fl));
     }
-    D.keys[s] = c[0] + D.nx * (c[1] + D.ny * c[2]);
+    D.keys[s] = sc.cellBase + (c[0] + sc.nx * (c[1] + sc.ny * c[2]));
 }
 
 __device__ __forceinline__ void reach_of(const BoxDev &D, int s, float *out) {  // BoxBroad.reach, :196-227
@@ -207,12 +209,13 @@ __global__ void k_list(BoxDev D) {
     float rs[3];
     reach_of(D, s, rs);
     int found = 0;
-    int key = D.keys[s];
-    int ix = key % D.nx, iy = (key / D.nx) % D.ny, iz = key / (D.nx * D.ny);
+    const SceneDev sc = D.sceneInfo[D.scene[s]];
+    int key = D.keys[s] - sc.cellBase;
+    int ix = key % sc.nx, iy = (key / sc.nx) % sc.ny, iz = key / (sc.nx * sc.ny);
     for (int m = 0; m < 27; m++) {
         int jx = ix + (m % 3 - 1), jy = iy + ((m / 3) % 3 - 1), jz = iz + (m / 9 - 1);
-        if (jx < 0 || jx >= D.nx || jy < 0 || jy >= D.ny || jz < 0 || jz >= D.nz) continue;
-        int cell = jx + D.nx * (jy + D.ny * jz);
+        if (jx < 0 || jx >= sc.nx || jy < 0 || jy >= sc.ny || jz < 0 || jz >= sc.nz) continue;
+        int cell = sc.cellBase + (jx + sc.nx * (jy + sc.ny * jz));
         int end = D.starts[cell + 1];
         for (int k = D.starts[cell]; k < end; k++) {
             int o = D.order[k];
@@ -398,6 +401,17 @@ __global__ void k_show(BoxDev D) {
 }
 
 }  // namespace
+
+__global__ void k_scene_from_params(const float *P, SceneDev *scene) {
+    scene->sx = P[SX];
+    scene->sy = P[SY];
+    scene->sz = P[SZ];
+}
+
+void launch_scene_from_params(const BoxDev &D, SceneDev *scene, hipStream_t s) {
+    k_scene_from_params<<<1, 1, 0, s>>>(D.params, scene);
+    check_launch("scene from params");
+}
 
 void launch_velocities(const BoxDev &D, hipStream_t s) {
     k_velocities<<<blocks_for(D.n), THREADS, 0, s>>>(D);

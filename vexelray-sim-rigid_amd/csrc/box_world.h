@@ -18,6 +18,9 @@ struct BoxBuf {
 };
 
 BoxWorld *box_create(const vr_step_desc &desc);  // std::invalid_argument for the reference's validation
+// Many independent scenes stepped together (vr_batch_create).
+BoxWorld *box_create_batch(const vr_step_desc &desc, int32_t scenes, const int32_t *bodies, const float *extents,
+                           const int32_t *grids, const float *cells);
 void box_destroy(BoxWorld *w);
 void box_clear(BoxWorld *w);
 bool box_find(BoxWorld *w, const std::string &name, BoxBuf *out);

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -41,6 +42,10 @@ struct BoxWorld {
     BoxDev D{};
     vr::SphereDev sortView{};  // the sphere kernels' grid sort reused: keys, order, starts over the same cells
     vr::SortScratch S{};
+    std::vector<int32_t> sceneBodies;  // bodies a scene, consecutive
+    std::vector<SceneDev> scenes;      // walls and grid a scene (cellBase worked out in build)
+    bool wallsFromParams = true;       // one scene, walled by the params' box (the reference's box world)
+    SceneDev *sceneDev = nullptr;
     Colouring C{};
     int32_t *pinned = nullptr;  // host: the colouring's counts read back
     int colourSteps = 1;        // steps of the colouring the last step needed
@@ -55,22 +60,45 @@ struct BoxWorld {
 
     void reg(const char *name, void *p, int64_t count, int dtype) { named.push_back({name, BoxBuf{p, count, dtype}}); }
 
-    void build(const vr_step_desc &desc) {
+    void build(const vr_step_desc &desc, std::vector<int32_t> bodies, std::vector<SceneDev> sceneList,
+               bool fromParams) {
         d = desc;
+        sceneBodies = std::move(bodies);
+        scenes = std::move(sceneList);
+        wallsFromParams = fromParams;
         if (d.bodies < 1 || d.substeps < 1 || d.sweeps < 1)
             throw std::invalid_argument("a step needs boxes, substeps and sweeps, got " + std::to_string(d.bodies) +
                                         ", " + std::to_string(d.substeps) + " and " + std::to_string(d.sweeps));
         n = d.bodies;
         grid = n > 1 && d.has_grid;
-        if (grid && (d.nx < 1 || d.ny < 1 || d.nz < 1 || !(d.cell > 0)))
-            throw std::invalid_argument("a grid of at least one cell, of positive size");
+        for (const SceneDev &sc : scenes)
+            if (grid && (sc.nx < 1 || sc.ny < 1 || sc.nz < 1 || !(sc.cell > 0)))
+                throw std::invalid_argument("a grid of at least one cell, of positive size");
         int devices = 0;
         if (hipGetDeviceCount(&devices) != hipSuccess || devices < 1) throw std::runtime_error("no HIP device");
         HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIP_OK(hipEventCreate(&t0));
         HIP_OK(hipEventCreate(&t1));
         D.n = n;
+        if ((int64_t)n * SLOTS > INT32_MAX) throw std::invalid_argument("too many bodies for 32 slots each");
         D.slots = grid ? n * SLOTS : 0;
+        // The scenes: each body's scene, and each scene's cells numbered after the scenes before it.
+        int64_t cellTotal = 0;
+        for (auto &sc : scenes) {
+            sc.cellBase = (int)cellTotal;
+            if (grid) cellTotal += (int64_t)sc.nx * sc.ny * sc.nz;
+            if (cellTotal > INT32_MAX / 2) throw std::invalid_argument("too many broad-phase cells");
+        }
+        std::vector<int32_t> sceneOf(n);
+        for (int k = 0, at = 0; k < (int)sceneBodies.size(); k++)
+            for (int b = 0; b < sceneBodies[k]; b++) sceneOf[at++] = k;
+        int32_t *sceneIdx = alloc<int32_t>(n);
+        sceneDev = alloc<SceneDev>((int64_t)scenes.size());
+        HIP_OK(hipMemcpy(sceneIdx, sceneOf.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(sceneDev, scenes.data(), sizeof(SceneDev) * scenes.size(), hipMemcpyHostToDevice));
+        D.scenes = (int)scenes.size();
+        D.scene = sceneIdx;
+        D.sceneInfo = sceneDev;
         D.width = std::max(1, n / 2);
         static const char *field[] = {"x",  "y",  "z",  "u",  "v",  "w",  "lx", "ly", "lz",
                                       "qx", "qy", "qz", "qw", "im", "hx", "hy", "hz", "shape"};
@@ -94,11 +122,7 @@ struct BoxWorld {
         reg("roll", D.roll, (int64_t)RUB_WORDS * n, VR_F32);
         reg("shown", D.shown, (int64_t)SHOWN_STRIDE * n, VR_F32);
         if (grid) {
-            D.nx = d.nx;
-            D.ny = d.ny;
-            D.nz = d.nz;
-            D.cell = d.cell;
-            D.cells = d.nx * d.ny * d.nz;
+            D.cells = (int)cellTotal;
             D.keys = alloc<int32_t>(n);
             D.order = alloc<int32_t>(n);
             D.starts = alloc<int32_t>((int64_t)D.cells + 1);
@@ -117,11 +141,7 @@ struct BoxWorld {
             reg("contacts", D.contacts, 2 * slots + n, VR_I32);
             reg("rounds", D.rounds, (int64_t)MOST_ROUNDS * D.width, VR_I32);
             sortView.n = n;
-            sortView.nx = d.nx;
-            sortView.ny = d.ny;
-            sortView.nz = d.nz;
             sortView.cells = D.cells;
-            sortView.cell = d.cell;
             sortView.keys = D.keys;
             sortView.order = D.order;
             sortView.starts = D.starts;
@@ -240,6 +260,7 @@ struct BoxWorld {
     void step(vr_step_report *rep) {
         auto w0 = std::chrono::steady_clock::now();
         HIP_OK(hipEventRecord(t0, stream));
+        if (wallsFromParams) launch_scene_from_params(D, sceneDev, stream);
         int used = 0, touching = 0, points = 0, unplaced = 0, unlisted = 0;
         if (grid) {
             // BoxPairs.keep: the step's pairs, manifolds and impulses become the kept ones, which carry reads.
@@ -301,9 +322,41 @@ struct BoxWorld {
 };
 
 BoxWorld *box_create(const vr_step_desc &desc) {
+    SceneDev one{desc.nx, desc.ny, desc.nz, 0, desc.cell, 1.0f, 1.0f, 1.0f};
     auto *w = new BoxWorld();
     try {
-        w->build(desc);
+        w->build(desc, {desc.bodies}, {one}, true);
+    } catch (...) {
+        delete w;
+        throw;
+    }
+    return w;
+}
+
+BoxWorld *box_create_batch(const vr_step_desc &desc, int32_t scenes, const int32_t *bodies, const float *extents,
+                           const int32_t *grids, const float *cells) {
+    if (scenes < 1) throw std::invalid_argument("a batch needs a scene, got " + std::to_string(scenes));
+    if (!desc.has_grid) throw std::invalid_argument("a batch steps with its scenes' grids");
+    std::vector<int32_t> counts(bodies, bodies + scenes);
+    std::vector<SceneDev> list(scenes);
+    int64_t total = 0;
+    for (int k = 0; k < scenes; k++) {
+        if (counts[k] < 1) throw std::invalid_argument("scene " + std::to_string(k) + " has no bodies");
+        const float *e = extents + 3 * k;
+        const int32_t *g = grids + 3 * k;
+        if (!(e[0] > 0 && e[1] > 0 && e[2] > 0))
+            throw std::invalid_argument("scene " + std::to_string(k) + " needs a box of positive size");
+        if (g[0] < 1 || g[1] < 1 || g[2] < 1 || !(cells[k] > 0))
+            throw std::invalid_argument("scene " + std::to_string(k) + " needs a grid of at least one cell");
+        list[k] = SceneDev{g[0], g[1], g[2], 0, cells[k], e[0], e[1], e[2]};
+        total += counts[k];
+    }
+    if (total != desc.bodies)
+        throw std::invalid_argument("the scenes hold " + std::to_string(total) + " bodies, the step " +
+                                    std::to_string(desc.bodies));
+    auto *w = new BoxWorld();
+    try {
+        w->build(desc, std::move(counts), std::move(list), false);
     } catch (...) {
         delete w;
         throw;

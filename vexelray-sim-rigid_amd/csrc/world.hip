@@ -566,6 +566,25 @@ int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out) {
     return VR_OK;
 }
 
+int vr_batch_create(const vr_step_desc *desc, int32_t scenes, const int32_t *bodies, const float *extents,
+                    const int32_t *grids, const float *cells, vr_world **out) {
+    if (!desc || !out || !bodies || !extents || !grids || !cells) return fail(VR_ERR_ARGUMENT, "null argument");
+    auto *w = new vr_world();
+    int rc = guard([&] {
+        try {
+            w->box = vrbox::box_create_batch(*desc, scenes, bodies, extents, grids, cells);
+        } catch (const std::invalid_argument &e) {
+            throw ArgError(e.what());
+        }
+    });
+    if (rc != VR_OK) {
+        delete w;
+        return rc;
+    }
+    *out = w;
+    return VR_OK;
+}
+
 int vr_world_clear(vr_world *w) {
     if (!w) return fail(VR_ERR_ARGUMENT, "null world");
     if (w->box) return guard([&] { vrbox::box_clear(w->box); });

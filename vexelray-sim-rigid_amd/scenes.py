@@ -15,6 +15,12 @@ Each builder returns a SceneSpec: the SphereStep to make, the initial state, and
   stacked unit cubes (half-extent 0.5 m) standing on the floor in an 8 x 4 grid 16 m apart; mu = 0.6, margin
   0.005 m; dt = 1/60 s in 8 substeps of 20 sweeps, broad-phase cells of the widest bounding sphere plus margin.
 
+* batched_2k: 2,048 independent scenes stepped together as one batch. Scene i holds N_i = round(exp(U[ln 16,
+  ln 4096])) bodies (about 1.5 M in all), half boxes and half spheres drawn as in mixed_32k, in a walled cube of
+  side 2 N_i^(1/3) m, on a lattice of that cube with jitter; mu = 0.6, margin 0.005 m; dt = 1/60 s in 8 substeps
+  of 10 sweeps. Drawn with numpy's PCG64 from the seed (the reference has no such scene, and a million-body
+  java.util.Random draw is too slow to make in Python).
+
 Draw order per sphere: radius (if drawn), x, y, z, then the three velocity components. Per body of mixed_32k:
 shape (box, then sphere, alternating), extents (3 halves, or the radius), quaternion (boxes: 4 Gaussians), x, y, z.
 """
@@ -167,4 +173,73 @@ def mixed_32k(seed: int = 3, boxes: int = 16_384, spheres: int = 16_384, towers:
                         f32(1.0 / mass), box, 600, dt, seed, towers)
 
 
-SCENES = {"pile_1k": pile_1k, "pile_64k": pile_64k, "mixed_32k": mixed_32k}
+@dataclass
+class BatchSpec:
+    """A batched box-path workload: per scene its body count, walled box and grid; the state over all bodies."""
+    name: str
+    bodies: np.ndarray       # per scene
+    extents: np.ndarray      # scenes x 3
+    grids: list              # per scene, a SphereGrid
+    substeps: int
+    sweeps: int
+    params: np.ndarray
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    spin: np.ndarray
+    q: np.ndarray
+    half: np.ndarray
+    shape: np.ndarray
+    im: np.ndarray
+    steps: int
+    dt: float
+    seed: int
+
+
+def batched_2k(seed: int = 4, scenes: int = 2048, fewest: int = 16, most: int = 4096, substeps: int = 8,
+               sweeps: int = 10) -> BatchSpec:
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(math.log(fewest), math.log(most), scenes))).astype(np.int64)
+    n = int(counts.sum())
+    sphere = rng.random(n) < 0.5
+    half = np.where(sphere, rng.uniform(0.2, 0.6, n), 0.0)[None, :].repeat(3, axis=0)
+    boxes = ~sphere
+    half[:, boxes] = rng.uniform(0.2, 1.0, (3, int(boxes.sum())))
+    q = np.zeros((4, n))
+    q[3] = 1.0
+    g = rng.standard_normal((4, int(boxes.sum())))
+    q[:, boxes] = g / np.sqrt((g * g).sum(axis=0))
+    mass = np.where(sphere, 1000.0 * 4.0 / 3.0 * math.pi * half[0] ** 3, 1000.0 * 8.0 * half[0] * half[1] * half[2])
+    pos = np.zeros((3, n))
+    extents = np.zeros((scenes, 3))
+    grids = []
+    margin = 0.005
+    first = 0
+    for k in range(scenes):
+        m = int(counts[k])
+        side = 2.0 * m ** (1.0 / 3.0)
+        per = int(math.ceil(m ** (1.0 / 3.0) - 1e-9))
+        spacing = side / per
+        cell = rng.permutation(per ** 3)[:m]
+        at = np.stack([cell % per, (cell // per) % per, cell // (per * per)]).astype(np.float64)
+        jitter = rng.uniform(-0.1, 0.1, (3, m)) * spacing
+        pos[:, first:first + m] = (at + 0.5) * spacing + jitter
+        extents[k] = side
+        widest = float(np.max(np.sqrt((half[:, first:first + m] ** 2).sum(axis=0))))
+        grids.append(BoxBroad.grid(side, side, side, widest, margin))
+        first += m
+    dt = 1.0 / 60
+    params = Boxes.params(dt / substeps, 0, -9.81, 0, 1.0, 1.0, 1.0, 0.6, 0.6, margin)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    zeros = np.zeros(n, dtype=np.float32)
+    return BatchSpec("batched_2k", counts.astype(np.int32), extents.astype(np.float32), grids, substeps, sweeps,
+                     params, f32(pos[0]), f32(pos[1]), f32(pos[2]), zeros, zeros.copy(), zeros.copy(),
+                     np.zeros((3, n), dtype=np.float32), f32(q), f32(half),
+                     np.where(sphere, Boxes.SPHERE_SHAPE, Boxes.BOX_SHAPE).astype(np.float32), f32(1.0 / mass),
+                     300, dt, seed)
+
+
+SCENES = {"pile_1k": pile_1k, "pile_64k": pile_64k, "mixed_32k": mixed_32k, "batched_2k": batched_2k}
