@@ -17,6 +17,7 @@ from test_boxes_known import broad_scene, random_pairs
 pkg = sc.pkg
 pytestmark = pytest.mark.gpu
 STATE = ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw")
+WALL_IMPULSES = ("lambda", "rub", "roll")  # read back in the reference's body-by-body layout
 
 
 def test_random_pair_manifolds_match_bitwise():
@@ -81,7 +82,7 @@ def run_both(make, steps):
         for _ in range(steps):
             r = s.sim.stepAndWait()
             reps.append((r.touching, r.points, r.rounds, r.unplaced, r.unlisted))
-        out[backend] = ({k: s.floats(k).copy() for k in STATE}, reps, s.ints("contacts").copy(),
+        out[backend] = ({k: s.floats(k).copy() for k in STATE + WALL_IMPULSES}, reps, s.ints("contacts").copy(),
                         s.ints("rounds").copy())
         s.close()
     return out["oracle"], out["gpu"]
@@ -94,6 +95,9 @@ def test_mixed_pile_matches_oracle(spheres):
     assert any(r[0] > 0 for r in ra), "the pile should touch"
     assert np.array_equal(ca, cb)
     assert np.array_equal(qa, qb), "the rounds (colouring) differ"
+    for k in WALL_IMPULSES:
+        assert np.any(sa[k] != 0) or (k == "roll" and not spheres), k  # rolling resistance is for balls
+        assert np.array_equal(sa[k].view(np.int32), sb[k].view(np.int32)), k
     for k in STATE:
         scale = np.maximum(np.abs(sa[k]), 1e-3)
         assert float(np.max(np.abs(sa[k] - sb[k]) / scale)) <= 1e-4, k

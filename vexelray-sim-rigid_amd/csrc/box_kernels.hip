@@ -85,7 +85,8 @@ __global__ void k_walls(BoxDev D) {
     float extent[3] = {sc.sx, sc.sy, sc.sz};
     Moving box = load_moving(B, s);
     Soft soft(P, true);
-    size_t base = (size_t)s * LAMBDA_WORDS;
+    // The wall impulses are slot-major: word w of body s at w * n + s, so a wave's loads of one word coalesce.
+    size_t n = (size_t)D.n;
     const Sweep sweep = (Sweep)SWEEP;
     for (int order = 0; order < WALLS; order++) {
         int wall = REVERSE ? WALLS - 1 - order : order;
@@ -97,12 +98,12 @@ __global__ void k_walls(BoxDev D) {
         if (box.sphere()) {
             float r[3];
             for (int a = 0; a < 3; a++) r[a] = -(box.half[0] * nrm[a]);
-            size_t at = base + firstSlot;
-            size_t tangentAt = at * 3;
+            size_t at = (size_t)firstSlot * n + s;
+            size_t tangentAt = (size_t)firstSlot * 3 * n + s;
             float reach = box.at[axis] + r[axis];
             float gap = high ? extent[axis] - reach : reach;
-            contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, soft);
-            roll(sweep, nullptr, box, box.half[0], B.lambda, at, B.roll, tangentAt, P);
+            contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft);
+            roll(sweep, nullptr, box, box.half[0], B.lambda, at, B.roll, tangentAt, n, P);
         } else {
             for (int k = 0; k < CORNERS; k++) {
                 int c = REVERSE ? CORNERS - 1 - k : k;
@@ -112,11 +113,11 @@ __global__ void k_walls(BoxDev D) {
                     offset[a] = box.half[a] * (2.0f * (float)bit - 1.0f);
                 }
                 rotate(box.q, offset, r);
-                size_t at = base + firstSlot + c;
-                size_t tangentAt = at * 3;
+                size_t at = (size_t)(firstSlot + c) * n + s;
+                size_t tangentAt = (size_t)(firstSlot + c) * 3 * n + s;
                 float reach = box.at[axis] + r[axis];
                 float gap = high ? extent[axis] - reach : reach;
-                contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, soft);
+                contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft);
             }
         }
     }
@@ -158,13 +159,13 @@ __global__ void k_solve_round(BoxDev D, int r) {
                     ((second.at[1] + rB[1]) - (first.at[1] + rA[1])) * nrm[1];
         gap = gap + ((second.at[2] + rB[2]) - (first.at[2] + rA[2])) * nrm[2];
         size_t normalAt = held + POINT_IMPULSE * slot;
-        contact(sweep, gap, nrm, &first, rA, second, rB, D.impulses, normalAt, D.impulses, normalAt + 1, soft);
+        contact(sweep, gap, nrm, &first, rA, second, rB, D.impulses, normalAt, D.impulses, normalAt + 1, 1, soft);
         if (first.sphere() || second.sphere()) {
             float radius = second.half[0];
             if (first.sphere()) radius = first.half[0];
             if (first.sphere() && second.sphere())
                 radius = (first.half[0] * second.half[0]) / (first.half[0] + second.half[0]);
-            roll(sweep, &first, second, radius, D.impulses, normalAt, D.impulses, normalAt + 4, P);
+            roll(sweep, &first, second, radius, D.impulses, normalAt, D.impulses, normalAt + 4, 1, P);
         }
     }
     store_moving(B, first);

@@ -225,10 +225,12 @@ __device__ float mass_along(const Moving *first, const float *rA, const Moving &
 }
 
 // Boxes.contact, Boxes.java:462-542: one point, one sweep.
+// The tangent impulse's three words are `stride` apart (1 for a pair's point; the body count for a wall's, whose
+// impulses are laid out word by word over the bodies).
 __device__ void contact(Sweep sweep, float gap, const float *n, Moving *first, const float *rA, Moving &second, const float *rB,
-             float *normals, size_t normalAt, float *tangents, size_t tangentAt, const Soft &soft) {
+             float *normals, size_t normalAt, float *tangents, size_t tangentAt, size_t stride, const Soft &soft) {
     float normal = normals[normalAt];
-    float tangent[3] = {tangents[tangentAt], tangents[tangentAt + 1], tangents[tangentAt + 2]};
+    float tangent[3] = {tangents[tangentAt], tangents[tangentAt + stride], tangents[tangentAt + 2 * stride]};
     bool holding = normal > 0 || dot(tangent, tangent) > 0;
     if (sweep == WARM) {
         if (holding) {
@@ -277,7 +279,7 @@ __device__ void contact(Sweep sweep, float gap, const float *n, Moving *first, c
     }
     float rubbed[3];
     for (int a = 0; a < 3; a++) {
-        tangents[tangentAt + a] = whole[a];
+        tangents[tangentAt + a * stride] = whole[a];
         rubbed[a] = whole[a] - tangent[a];
     }
     apply_pair(first, rA, second, rB, rubbed);
@@ -293,8 +295,8 @@ __device__ void apply_angular(Moving *first, Moving &second, const float *j) {  
 
 // Boxes.roll, Boxes.java:553-600.
 __device__ void roll(Sweep sweep, Moving *first, Moving &second, float radius, const float *normals, size_t normalAt,
-          float *rolls, size_t rollAt, const float *P) {
-    float old[3] = {rolls[rollAt], rolls[rollAt + 1], rolls[rollAt + 2]};
+          float *rolls, size_t rollAt, size_t stride, const float *P) {
+    float old[3] = {rolls[rollAt], rolls[rollAt + stride], rolls[rollAt + 2 * stride]};
     if (sweep == WARM) {
         apply_angular(first, second, old);
         return;
@@ -328,7 +330,7 @@ __device__ void roll(Sweep sweep, Moving *first, Moving &second, float radius, c
     }
     float change[3];
     for (int a = 0; a < 3; a++) {
-        rolls[rollAt + a] = whole[a];
+        rolls[rollAt + a * stride] = whole[a];
         change[a] = whole[a] - old[a];
     }
     apply_angular(first, second, change);

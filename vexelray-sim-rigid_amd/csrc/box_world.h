@@ -15,6 +15,9 @@ struct BoxBuf {
     void *p;
     int64_t count;
     int dtype;
+    // Words a body (> 0) for a buffer the device keeps slot-major (word w of body b at w * bodies + b) while the
+    // buffer's name reads and writes it body by body, as the reference lays it out; 0 when the two agree.
+    int bodyWords = 0;
 };
 
 BoxWorld *box_create(const vr_step_desc &desc);  // std::invalid_argument for the reference's validation

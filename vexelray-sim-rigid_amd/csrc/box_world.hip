@@ -58,7 +58,9 @@ struct BoxWorld {
         return (T *)p;
     }
 
-    void reg(const char *name, void *p, int64_t count, int dtype) { named.push_back({name, BoxBuf{p, count, dtype}}); }
+    void reg(const char *name, void *p, int64_t count, int dtype, int bodyWords = 0) {
+        named.push_back({name, BoxBuf{p, count, dtype, bodyWords}});
+    }
 
     void build(const vr_step_desc &desc, std::vector<int32_t> bodies, std::vector<SceneDev> sceneList,
                bool fromParams) {
@@ -117,9 +119,9 @@ struct BoxWorld {
         D.rub = alloc<float>((int64_t)RUB_WORDS * n);
         D.roll = alloc<float>((int64_t)RUB_WORDS * n);
         D.shown = alloc<float>((int64_t)SHOWN_STRIDE * n);
-        reg("lambda", D.lambda, (int64_t)LAMBDA_WORDS * n, VR_F32);
-        reg("rub", D.rub, (int64_t)RUB_WORDS * n, VR_F32);
-        reg("roll", D.roll, (int64_t)RUB_WORDS * n, VR_F32);
+        reg("lambda", D.lambda, (int64_t)LAMBDA_WORDS * n, VR_F32, LAMBDA_WORDS);
+        reg("rub", D.rub, (int64_t)RUB_WORDS * n, VR_F32, RUB_WORDS);
+        reg("roll", D.roll, (int64_t)RUB_WORDS * n, VR_F32, RUB_WORDS);
         reg("shown", D.shown, (int64_t)SHOWN_STRIDE * n, VR_F32);
         if (grid) {
             D.cells = (int)cellTotal;

@@ -593,29 +593,62 @@ int vr_world_clear(vr_world *w) {
 
 namespace {
 
-Buf find_any(vr_world *w, const std::string &name) {
+Buf find_any(vr_world *w, const std::string &name, int *bodyWords = nullptr) {
+    if (bodyWords) *bodyWords = 0;
     if (!w->box) return w->find(name);
     vrbox::BoxBuf b;
     if (!vrbox::box_find(w->box, name, &b)) throw NameError("no buffer named " + name);
+    if (bodyWords) *bodyWords = b.bodyWords;
     return Buf{b.p, b.count, b.dtype};
 }
 
 hipStream_t stream_of(vr_world *w) { return w->box ? vrbox::box_stream(w->box) : w->stream; }
 
+// A slot-major device buffer to the reference's body-by-body order (toBodies) or back, the whole buffer.
+void reorder(const int32_t *from, int32_t *to, int64_t count, int words, bool toBodies) {
+    int64_t bodies = count / words;
+    for (int64_t b = 0; b < bodies; b++)
+        for (int k = 0; k < words; k++) {
+            int64_t bodyOrder = b * words + k, slotOrder = (int64_t)k * bodies + b;
+            if (toBodies) to[bodyOrder] = from[slotOrder];
+            else to[slotOrder] = from[bodyOrder];
+        }
+}
+
 void copy_in(vr_world *w, const std::string &name, const void *src, int64_t count) {
-    Buf b = find_any(w, name);
+    int words = 0;
+    Buf b = find_any(w, name, &words);
     if (count < 0 || count > b.count) throw NameError("buffer " + name + " is shorter than the write");
     hipStream_t s = stream_of(w);
-    HIP_OK(hipMemcpyAsync(b.p, src, (size_t)count * 4, hipMemcpyHostToDevice, s));
+    if (words > 0) {  // a part written: the rest kept, so the whole buffer goes through the host
+        std::vector<int32_t> dev((size_t)b.count), host((size_t)b.count);
+        HIP_OK(hipMemcpyAsync(dev.data(), b.p, (size_t)b.count * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        reorder(dev.data(), host.data(), b.count, words, true);
+        std::memcpy(host.data(), src, (size_t)count * 4);
+        reorder(host.data(), dev.data(), b.count, words, false);
+        HIP_OK(hipMemcpyAsync(b.p, dev.data(), (size_t)b.count * 4, hipMemcpyHostToDevice, s));
+    } else {
+        HIP_OK(hipMemcpyAsync(b.p, src, (size_t)count * 4, hipMemcpyHostToDevice, s));
+    }
     HIP_OK(hipStreamSynchronize(s));
 }
 
 void copy_out(vr_world *w, const std::string &name, void *dst, int64_t count) {
-    Buf b = find_any(w, name);
+    int words = 0;
+    Buf b = find_any(w, name, &words);
     if (count < 0 || count > b.count) throw NameError("buffer " + name + " is shorter than the read");
     hipStream_t s = stream_of(w);
-    HIP_OK(hipMemcpyAsync(dst, b.p, (size_t)count * 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    if (words > 0) {
+        std::vector<int32_t> dev((size_t)b.count), host((size_t)b.count);
+        HIP_OK(hipMemcpyAsync(dev.data(), b.p, (size_t)b.count * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        reorder(dev.data(), host.data(), b.count, words, true);
+        std::memcpy(dst, host.data(), (size_t)count * 4);
+    } else {
+        HIP_OK(hipMemcpyAsync(dst, b.p, (size_t)count * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
 }
 
 }  // namespace
