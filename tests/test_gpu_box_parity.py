@@ -129,3 +129,21 @@ def test_big_pile_colouring_matches_oracle():
     assert np.array_equal(qa, qb)
     for k in STATE:
         assert np.array_equal(sa[k].view(np.int32), sb[k].view(np.int32)), k
+
+
+def test_wall_impulses_written_through_the_abi_round_trip():
+    """Wall impulses read and written back body by body (the reference's layout) leave the stepping unchanged."""
+    out = {}
+    for backend in ("oracle", "gpu"):
+        s = mixed_pile(80, seed=5)
+        s.start(backend)
+        for _ in range(6):
+            s.sim.stepAndWait()
+        for name in WALL_IMPULSES:
+            s.sim.write(name, s.floats(name).copy())
+        for _ in range(6):
+            s.sim.stepAndWait()
+        out[backend] = {k: s.floats(k).copy() for k in STATE + WALL_IMPULSES}
+        s.close()
+    for k in STATE + WALL_IMPULSES:
+        assert np.array_equal(out["oracle"][k].view(np.int32), out["gpu"][k].view(np.int32)), k

@@ -28,6 +28,7 @@ struct BoxDev {
     float *x[3], *vel[3], *l[3], *q[4], *im, *half[3], *shape;
     const float *params;
     float *lambda, *rub, *roll, *shown;
+    int32_t *wallHeld;  // n: bit w set when a corner of the body holds an impulse at wall w (boxes; all set = unknown)
     int32_t *keys, *order, *starts;
     int32_t *pairs, *pairsWas, *contacts, *rounds;
     float *manifold, *manifoldWas, *impulses, *impulsesWas;

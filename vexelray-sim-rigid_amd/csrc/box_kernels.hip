@@ -88,6 +88,7 @@ __global__ void k_walls(BoxDev D) {
     // The wall impulses are slot-major: word w of body s at w * n + s, so a wave's loads of one word coalesce.
     size_t n = (size_t)D.n;
     const Sweep sweep = (Sweep)SWEEP;
+    int held = D.wallHeld[s], nowHeld = 0;  // walls at which some corner holds an impulse
     for (int order = 0; order < WALLS; order++) {
         int wall = REVERSE ? WALLS - 1 - order : order;
         int axis = wall / 2;
@@ -105,6 +106,11 @@ __global__ void k_walls(BoxDev D) {
             contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft);
             roll(sweep, nullptr, box, box.half[0], B.lambda, at, B.roll, tangentAt, n, P);
         } else {
+            // A corner that holds no impulse and is out of the margin's reach returns at once in Boxes.contact;
+            // a wall none of whose corners holds one (its bit clear in wallHeld) needs none of its loads for that.
+            bool live = (held >> wall) & 1;
+            if (sweep == WARM && !live) continue;
+            bool holds = false;
             for (int k = 0; k < CORNERS; k++) {
                 int c = REVERSE ? CORNERS - 1 - k : k;
                 float offset[3], r[3];
@@ -113,15 +119,19 @@ __global__ void k_walls(BoxDev D) {
                     offset[a] = box.half[a] * (2.0f * (float)bit - 1.0f);
                 }
                 rotate(box.q, offset, r);
-                size_t at = (size_t)(firstSlot + c) * n + s;
-                size_t tangentAt = (size_t)(firstSlot + c) * 3 * n + s;
                 float reach = box.at[axis] + r[axis];
                 float gap = high ? extent[axis] - reach : reach;
-                contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft);
+                if (!live && !(gap < soft.margin)) continue;
+                size_t at = (size_t)(firstSlot + c) * n + s;
+                size_t tangentAt = (size_t)(firstSlot + c) * 3 * n + s;
+                holds = contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft) ||
+                        holds;
             }
+            if (holds) nowHeld |= 1 << wall;
         }
     }
     store_moving(B, box);
+    if (!box.sphere() && sweep != WARM) D.wallHeld[s] = nowHeld;
 }
 
 // BoxPairs.solve, BoxPairs.java:64-151: the pairs of round r, one thread a place, each its manifold in turn.

@@ -226,8 +226,8 @@ __device__ float mass_along(const Moving *first, const float *rA, const Moving &
 
 // Boxes.contact, Boxes.java:462-542: one point, one sweep.
 // The tangent impulse's three words are `stride` apart (1 for a pair's point; the body count for a wall's, whose
-// impulses are laid out word by word over the bodies).
-__device__ void contact(Sweep sweep, float gap, const float *n, Moving *first, const float *rA, Moving &second, const float *rB,
+// impulses are laid out word by word over the bodies). Returns whether the point holds an impulse afterwards.
+__device__ bool contact(Sweep sweep, float gap, const float *n, Moving *first, const float *rA, Moving &second, const float *rB,
              float *normals, size_t normalAt, float *tangents, size_t tangentAt, size_t stride, const Soft &soft) {
     float normal = normals[normalAt];
     float tangent[3] = {tangents[tangentAt], tangents[tangentAt + stride], tangents[tangentAt + 2 * stride]};
@@ -238,9 +238,9 @@ __device__ void contact(Sweep sweep, float gap, const float *n, Moving *first, c
             for (int a = 0; a < 3; a++) p[a] = normal * n[a] + tangent[a];
             apply_pair(first, rA, second, rB, p);
         }
-        return;
+        return holding;
     }
-    if (!(gap < soft.margin || holding)) return;
+    if (!(gap < soft.margin || holding)) return false;
     float k = mass_along(first, rA, second, rB, n);
     float rel[3];
     relative(first, rA, second, rB, rel);
@@ -283,6 +283,7 @@ __device__ void contact(Sweep sweep, float gap, const float *n, Moving *first, c
         rubbed[a] = whole[a] - tangent[a];
     }
     apply_pair(first, rA, second, rB, rubbed);
+    return total > 0 || dot(whole, whole) > 0;
 }
 
 __device__ void apply_angular(Moving *first, Moving &second, const float *j) {  // Boxes.java:603-612

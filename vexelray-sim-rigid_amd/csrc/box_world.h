@@ -30,6 +30,7 @@ bool box_find(BoxWorld *w, const std::string &name, BoxBuf *out);
 hipStream_t box_stream(BoxWorld *w);
 void box_params(BoxWorld *w, const int32_t *words, int32_t n);
 void box_step(BoxWorld *w, vr_step_report *rep);
+void box_wall_impulses_written(BoxWorld *w);  // after a write of lambda or rub through the ABI
 void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold);
 
 }  // namespace vrbox

@@ -119,6 +119,7 @@ struct BoxWorld {
         D.rub = alloc<float>((int64_t)RUB_WORDS * n);
         D.roll = alloc<float>((int64_t)RUB_WORDS * n);
         D.shown = alloc<float>((int64_t)SHOWN_STRIDE * n);
+        D.wallHeld = alloc<int32_t>(n);
         reg("lambda", D.lambda, (int64_t)LAMBDA_WORDS * n, VR_F32, LAMBDA_WORDS);
         reg("rub", D.rub, (int64_t)RUB_WORDS * n, VR_F32, RUB_WORDS);
         reg("roll", D.roll, (int64_t)RUB_WORDS * n, VR_F32, RUB_WORDS);
@@ -214,7 +215,14 @@ struct BoxWorld {
             for (void *p : {(void *)D.impulses, (void *)D.impulsesWas})
                 HIP_OK(hipMemsetAsync(p, 0, slots * IMPULSE_WORDS * 4, stream));
         }
+        HIP_OK(hipMemsetAsync(D.wallHeld, 0, (size_t)n * 4, stream));  // no impulse anywhere
         vr::launch_fill(D.q[3], 1.0f, n, stream);
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    // Wall impulses written from outside: which walls hold one is not known, so every wall is looked at.
+    void wall_impulses_written() {
+        vr::launch_fill_i32(D.wallHeld, (1 << WALLS) - 1, n, stream);
         HIP_OK(hipStreamSynchronize(stream));
     }
 
@@ -379,6 +387,7 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n) {
 }
 
 void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
+void box_wall_impulses_written(BoxWorld *w) { w->wall_impulses_written(); }
 
 void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold) {
     w->collide_pairs(pairs, npairs, manifold);

@@ -628,6 +628,8 @@ void copy_in(vr_world *w, const std::string &name, const void *src, int64_t coun
         std::memcpy(host.data(), src, (size_t)count * 4);
         reorder(host.data(), dev.data(), b.count, words, false);
         HIP_OK(hipMemcpyAsync(b.p, dev.data(), (size_t)b.count * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipStreamSynchronize(s));
+        vrbox::box_wall_impulses_written(w->box);
     } else {
         HIP_OK(hipMemcpyAsync(b.p, src, (size_t)count * 4, hipMemcpyHostToDevice, s));
     }
