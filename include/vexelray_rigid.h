@@ -159,6 +159,29 @@ int vr_box_collide(vr_world *w, const int32_t *pairs, int32_t npairs, float *man
 int vr_world_profile(vr_world *w, int32_t on);
 int vr_world_profile_read(vr_world *w, vr_profile *out);
 
+/*
+ * The learned warm start's network (no counterpart in the reference): a per-contact MLP, 16 features -> 64 -> 64
+ * -> 64 -> 3 outputs, SiLU on the hidden layers. 9,603 f32 parameters in this order: W1[16][64], b1[64],
+ * W2[64][64], b2[64], W3[64][64], b3[64], W4[64][3], b4[3]. Inference is bf16 on the matrix cores with f32
+ * accumulation; training is f32 with the bf16-rounded weights, mean squared error, Adam. All arrays are host arrays,
+ * row-major (n x 16 features, n x 3 outputs). Errors: vr_mlp_last_error.
+ */
+typedef struct vr_mlp vr_mlp;
+#define VR_MLP_FEATURES 16
+#define VR_MLP_OUTPUTS 3
+#define VR_MLP_PARAMS 9603
+int vr_mlp_create(uint64_t seed, vr_mlp **out);
+int vr_mlp_destroy(vr_mlp *m);
+int vr_mlp_read(vr_mlp *m, float *params);
+int vr_mlp_write(vr_mlp *m, const float *params);
+int vr_mlp_forward(vr_mlp *m, const float *x, int64_t n, float *y);
+/* The gradient of the mean squared error over the n points, and the loss, at the current parameters. */
+int vr_mlp_gradient(vr_mlp *m, const float *x, const float *t, int64_t n, float *grad, float *loss);
+/* Epochs of Adam over shuffled batches; *loss the last epoch's mean batch loss. */
+int vr_mlp_train(vr_mlp *m, const float *x, const float *t, int64_t n, int32_t epochs, int32_t batch, float rate,
+                 uint64_t seed, float *loss);
+int vr_mlp_last_error(char *buf, int32_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,7 +25,8 @@ VR_I32 = 2
 # Every entry point the header declares, for the export check.
 ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "world_buffer_info", "world_params",
                 "world_friction", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
-                "world_profile_read", "box_collide", "batch_create")
+                "world_profile_read", "box_collide", "batch_create", "mlp_create", "mlp_destroy", "mlp_read",
+                "mlp_write", "mlp_forward", "mlp_gradient", "mlp_train", "mlp_last_error")
 
 
 class StepDesc(ctypes.Structure):
@@ -76,6 +77,17 @@ class Abi:
             "world_profile": ([p, ctypes.c_int32]),
             "world_profile_read": ([p, ctypes.POINTER(Profile)]),
             "box_collide": ([p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
+            "mlp_create": ([ctypes.c_uint64, ctypes.POINTER(p)]),
+            "mlp_destroy": ([p]),
+            "mlp_read": ([p, ctypes.POINTER(ctypes.c_float)]),
+            "mlp_write": ([p, ctypes.POINTER(ctypes.c_float)]),
+            "mlp_forward": ([p, ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.POINTER(ctypes.c_float)]),
+            "mlp_gradient": ([p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int64,
+                              ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+            "mlp_train": ([p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int64,
+                           ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64,
+                           ctypes.POINTER(ctypes.c_float)]),
+            "mlp_last_error": ([ctypes.c_char_p, ctypes.c_int32]),
             "batch_create": ([ctypes.POINTER(StepDesc), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(p)]),
