@@ -181,6 +181,23 @@ int vr_mlp_gradient(vr_mlp *m, const float *x, const float *t, int64_t n, float 
 int vr_mlp_train(vr_mlp *m, const float *x, const float *t, int64_t n, int32_t epochs, int32_t batch, float rate,
                  uint64_t seed, float *loss);
 int vr_mlp_last_error(char *buf, int32_t len);
+/*
+ * The learned warm start in a box world (or batch), no counterpart in the reference, whose warm start is the carry
+ * of the last step's impulses (core/.../box/BoxPairs.java:191-267). Per manifold point of each touching pair, 16
+ * features are read after the carry: normal (3), relative velocity at the point (3), depth, the carried impulse as
+ * a velocity (normal, and tangent in a basis of the contact plane: 3), the first body's share of the pair's
+ * inverse mass, the pair's shapes one-hot (box-box, box-ball, ball-ball), the closing speed along the normal, and
+ * the manifold's point count / 4. The 3 targets are the impulse the point holds after the step (what the next
+ * step's carry would start it from), in those units.
+ * Impulses (features 7-9, targets) are given as sign(v) log(1 + |v|) of the velocity v.
+ * vr_box_log starts (or stops) logging (features, targets) of every step, up to `capacity` samples, on the device;
+ * vr_box_samples copies up to `most` of them out. vr_box_learned sets the network (null: none) whose prediction
+ * replaces the carried impulse at each point where it is usable (finite, normal part not negative); the report's
+ * `reserved` word then counts the points it replaced.
+ */
+int vr_box_log(vr_world *w, int32_t on, int64_t capacity);
+int vr_box_samples(vr_world *w, float *x, float *t, int64_t most, int64_t *count);
+int vr_box_learned(vr_world *w, vr_mlp *m);
 
 #ifdef __cplusplus
 }

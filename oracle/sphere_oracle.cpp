@@ -1361,6 +1361,13 @@ int vro_world_profile_read(vro_world *, vr_profile *out) {
     return VR_OK;
 }
 
+// The learned warm start runs on the device only: the oracle's box worlds keep the reference's carry.
+int vro_box_log(vro_world *, int32_t, int64_t) { return fail(VR_ERR_STATE, "the oracle keeps no sample log"); }
+int vro_box_samples(vro_world *, float *, float *, int64_t, int64_t *) {
+    return fail(VR_ERR_STATE, "the oracle keeps no sample log");
+}
+int vro_box_learned(vro_world *, struct vro_mlp *) { return fail(VR_ERR_STATE, "the oracle has no learned start"); }
+
 // Opening batch sizes the stepper holds, for the tests of its adaptation (SphereStepper.opening, :115-117).
 int vro_world_opening(vro_world *p, int32_t iteration) { return p->w.opening.at(iteration); }
 

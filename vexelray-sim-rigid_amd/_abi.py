@@ -26,7 +26,8 @@ VR_I32 = 2
 ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "world_buffer_info", "world_params",
                 "world_friction", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
                 "world_profile_read", "box_collide", "batch_create", "mlp_create", "mlp_destroy", "mlp_read",
-                "mlp_write", "mlp_forward", "mlp_gradient", "mlp_train", "mlp_last_error")
+                "mlp_write", "mlp_forward", "mlp_gradient", "mlp_train", "mlp_last_error", "box_log",
+                "box_samples", "box_learned")
 
 
 class StepDesc(ctypes.Structure):
@@ -88,6 +89,10 @@ class Abi:
                            ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64,
                            ctypes.POINTER(ctypes.c_float)]),
             "mlp_last_error": ([ctypes.c_char_p, ctypes.c_int32]),
+            "box_log": ([p, ctypes.c_int32, ctypes.c_int64]),
+            "box_samples": ([p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int64,
+                             ctypes.POINTER(ctypes.c_int64)]),
+            "box_learned": ([p, p]),
             "batch_create": ([ctypes.POINTER(StepDesc), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(p)]),

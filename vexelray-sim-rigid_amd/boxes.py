@@ -11,6 +11,7 @@
 Bodies are boxes or spheres (``shape`` 1, radius in ``hx``); every step runs on the device through the HIP library.
 """
 
+import ctypes
 import math
 import struct
 from dataclasses import dataclass
@@ -198,6 +199,7 @@ class BoxSimulation:
         rep = self._world.step(True)
         self._last = BoxReport(rep.touching, rep.points, rep.rounds, rep.unplaced, rep.unlisted)
         self.wall_ns, self.gpu_ns = rep.wall_ns, rep.gpu_ns
+        self.learnedPoints = rep.reserved
         return self._last
 
     def last(self) -> BoxReport:
@@ -206,6 +208,28 @@ class BoxSimulation:
     def collide(self, pairs) -> np.ndarray:
         """The narrow phase alone over ``pairs`` (first, second, first, second, ...)."""
         return self._world.collide(pairs)
+
+    # The learned warm start (no counterpart in the reference): vr_box_log / vr_box_samples / vr_box_learned.
+    FEATURES, TARGETS = 16, 3
+
+    def log(self, on: bool, capacity: int = 1 << 20) -> None:
+        """Starts (clearing it) or stops the device log of per-point (features, targets) of every step."""
+        self._world._check(self._abi.fn("box_log")(self._world.handle, 1 if on else 0, int(capacity)))
+
+    def samples(self, most: int = 1 << 20):
+        """Up to `most` logged samples: features (n x 16) and targets (n x 3)."""
+        x = np.empty((most, self.FEATURES), dtype=np.float32)
+        t = np.empty((most, self.TARGETS), dtype=np.float32)
+        count = ctypes.c_int64()
+        f32 = ctypes.POINTER(ctypes.c_float)
+        self._world._check(self._abi.fn("box_samples")(self._world.handle, x.ctypes.data_as(f32),
+                                                        t.ctypes.data_as(f32), most, ctypes.byref(count)))
+        return x[:count.value], t[:count.value]
+
+    def learned(self, net) -> None:
+        """Starts each step's impulses from `net`'s prediction (an Mlp of the same library), or from the carry
+        (None), as the reference does."""
+        self._world._check(self._abi.fn("box_learned")(self._world.handle, net._h if net is not None else None))
 
     def state(self) -> List[np.ndarray]:
         """``BoxSimulation.state`` (BoxSimulation.java:150-160): x, y, z, u, v, w, lx, ly, lz, qx, qy, qz, qw."""

@@ -56,6 +56,25 @@ struct Colouring {
     vr::SortScratch sort;    // sized for slots keys
 };
 
+// The learned warm start's per-point data (box_kernels.hip): each touching pair's manifold points, listed in the
+// touching list's order (pointBase[t] the first of pair touch[t]), with 16 features read at the step's start (after
+// the carry) and 3 targets, the impulses the point ends the step with. Impulses are scaled to velocities by the
+// pair's inverse-mass sum and the tangent one written in a basis (t1, t2) of the plane across the normal.
+constexpr int LEARN_FEATURES = 16, LEARN_OUTPUTS = 3;
+
+struct Learned {
+    int32_t *pointBase;  // slots + 1
+    float *x;            // 4 * slots * LEARN_FEATURES
+    float *y;            // 4 * slots * LEARN_OUTPUTS: the network's prediction
+    float *t;            // 4 * slots * LEARN_OUTPUTS: the targets
+    int32_t *used;       // 1: points whose carried impulse the prediction replaced
+};
+
+void launch_point_features(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs, hipStream_t s);
+void launch_point_targets(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs, hipStream_t s);
+void launch_apply_prediction(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs,
+                             hipStream_t s);
+
 int64_t colour_tmp_words(int64_t slots);
 void launch_colour_prepare(const BoxDev &D, const Colouring &C, const vr::SortScratch &S, hipStream_t s);
 void launch_colour_steps(const BoxDev &D, const Colouring &C, int touchingPairs, int from, int count,

@@ -413,10 +413,170 @@ __global__ void k_show(BoxDev D) {
 
 }  // namespace
 
+// ---- the learned warm start's points (no counterpart in the reference) ----
+
+// A basis of the plane across unit normal n: t1 = n x e / |n x e| (e the axis least along n), t2 = n x t1.
+__device__ __forceinline__ void plane_basis(const float *n, float *t1, float *t2) {
+    float e[3] = {0, 0, 0};
+    e[fabsf(n[0]) < 0.57f ? 0 : (fabsf(n[1]) < 0.57f ? 1 : 2)] = 1;
+    float c[3];
+    cross(n, e, c);
+    float len = sqrtf(dot(c, c));
+    for (int a = 0; a < 3; a++) t1[a] = c[a] / len;
+    cross(n, t1, t2);
+}
+
+// Impulses (as velocities) in the network's scale, sign(v) log(1 + |v|), which keeps the rare large ones from
+// ruling the squared error; and back.
+__device__ __forceinline__ float squash(float v) { return copysignf(log1pf(fabsf(v)), v); }
+__device__ __forceinline__ float unsquash(float v) { return copysignf(expm1f(fabsf(v)), v); }
+
+// Where point k of touching pair t's manifold sits, and what the solve sees there (BoxPairs.solve's arms).
+struct PointAt {
+    int p, count;
+    float n[3], rA[3], rB[3], wsum, share;
+    Moving first, second;
+};
+
+__device__ __forceinline__ PointAt point_at(const BoxDev &D, const Bufs &B, int p, int k) {
+    PointAt P;
+    P.p = p;
+    size_t out = (size_t)p * MANIFOLD_WORDS;
+    P.count = (int)D.manifold[out + COUNT];
+    for (int a = 0; a < 3; a++) P.n[a] = D.manifold[out + a];
+    int ia = p / SLOTS, ib = D.contacts[p];
+    P.first = load_moving(B, ia);
+    P.second = load_moving(B, ib);
+    size_t where = out + POINTS + k * POINT_WORDS;
+    rotate(P.first.q, D.manifold + where + ANCHOR_FIRST, P.rA);
+    rotate(P.second.q, D.manifold + where + ANCHOR_SECOND, P.rB);
+    if (P.first.sphere())
+        for (int a = 0; a < 3; a++) P.rA[a] = P.first.half[0] * P.n[a];
+    if (P.second.sphere())
+        for (int a = 0; a < 3; a++) P.rB[a] = -(P.second.half[0] * P.n[a]);
+    float sum = P.first.im + P.second.im;
+    P.wsum = sum > 0 ? sum : 1.0f;
+    P.share = sum > 0 ? P.first.im / sum : 0.5f;
+    return P;
+}
+
+__global__ void k_point_counts(BoxDev D, Colouring C, Learned L) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *C.touching) return;
+    L.pointBase[t] = (int)D.manifold[(size_t)C.touch[t] * MANIFOLD_WORDS + COUNT];
+}
+
+// Features: normal (3), relative velocity at the point (3), depth, the carried impulse (normal, two tangent),
+// the first body's share of the inverse mass, the pair's shapes (box-box, box-ball, ball-ball), closing speed along
+// the normal, and the manifold's point count / 4.
+__global__ void k_point_features(BoxDev D, Colouring C, Learned L) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *C.touching) return;
+    Bufs B = bufs(D);
+    int p = C.touch[t];
+    int count = (int)D.manifold[(size_t)p * MANIFOLD_WORDS + COUNT];
+    for (int k = 0; k < count; k++) {
+        PointAt P = point_at(D, B, p, k);
+        float va[3], vb[3], rel[3], t1[3], t2[3];
+        P.first.pointVelocity(P.rA, va);
+        P.second.pointVelocity(P.rB, vb);
+        for (int a = 0; a < 3; a++) rel[a] = vb[a] - va[a];
+        plane_basis(P.n, t1, t2);
+        const float *held = D.impulses + (size_t)p * IMPULSE_WORDS + POINT_IMPULSE * k;
+        float tangent[3] = {held[1], held[2], held[3]};
+        int balls = (P.first.sphere() ? 1 : 0) + (P.second.sphere() ? 1 : 0);
+        float *f = L.x + (size_t)(L.pointBase[t] + k) * LEARN_FEATURES;
+        f[0] = P.n[0];
+        f[1] = P.n[1];
+        f[2] = P.n[2];
+        f[3] = rel[0];
+        f[4] = rel[1];
+        f[5] = rel[2];
+        f[6] = D.manifold[(size_t)p * MANIFOLD_WORDS + POINTS + k * POINT_WORDS + DEPTH];
+        f[7] = squash(held[0] * P.wsum);
+        f[8] = squash(dot(tangent, t1) * P.wsum);
+        f[9] = squash(dot(tangent, t2) * P.wsum);
+        f[10] = P.share;
+        f[11] = balls == 0 ? 1.f : 0.f;
+        f[12] = balls == 1 ? 1.f : 0.f;
+        f[13] = balls == 2 ? 1.f : 0.f;
+        f[14] = dot(rel, P.n);
+        f[15] = 0.25f * (float)count;
+    }
+}
+
+// Targets: the impulse each point holds when the step is done, in the features' units and basis.
+__global__ void k_point_targets(BoxDev D, Colouring C, Learned L) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *C.touching) return;
+    Bufs B = bufs(D);
+    int p = C.touch[t];
+    int count = (int)D.manifold[(size_t)p * MANIFOLD_WORDS + COUNT];
+    for (int k = 0; k < count; k++) {
+        PointAt P = point_at(D, B, p, k);
+        float t1[3], t2[3];
+        plane_basis(P.n, t1, t2);
+        const float *held = D.impulses + (size_t)p * IMPULSE_WORDS + POINT_IMPULSE * k;
+        float tangent[3] = {held[1], held[2], held[3]};
+        float *o = L.t + (size_t)(L.pointBase[t] + k) * LEARN_OUTPUTS;
+        o[0] = squash(held[0] * P.wsum);
+        o[1] = squash(dot(tangent, t1) * P.wsum);
+        o[2] = squash(dot(tangent, t2) * P.wsum);
+    }
+}
+
+// The prediction as the step's starting impulse, where it is usable (finite, the normal one not negative, a pair
+// with mass to push); elsewhere the carried impulse stays. The rolling words are left as carried.
+__global__ void k_apply_prediction(BoxDev D, Colouring C, Learned L) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *C.touching) return;
+    Bufs B = bufs(D);
+    int p = C.touch[t];
+    int count = (int)D.manifold[(size_t)p * MANIFOLD_WORDS + COUNT];
+    int taken = 0;
+    for (int k = 0; k < count; k++) {
+        PointAt P = point_at(D, B, p, k);
+        const float *y = L.y + (size_t)(L.pointBase[t] + k) * LEARN_OUTPUTS;
+        bool usable = isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2]) && y[0] >= 0 &&
+                      P.first.im + P.second.im > 0;
+        if (!usable) continue;
+        float t1[3], t2[3];
+        plane_basis(P.n, t1, t2);
+        float *held = D.impulses + (size_t)p * IMPULSE_WORDS + POINT_IMPULSE * k;
+        float along1 = unsquash(y[1]), along2 = unsquash(y[2]);
+        held[0] = unsquash(y[0]) / P.wsum;
+        for (int a = 0; a < 3; a++) held[1 + a] = (along1 * t1[a] + along2 * t2[a]) / P.wsum;
+        taken++;
+    }
+    if (taken) atomicAdd(L.used, taken);
+}
+
 __global__ void k_scene_from_params(const float *P, SceneDev *scene) {
     scene->sx = P[SX];
     scene->sy = P[SY];
     scene->sz = P[SZ];
+}
+
+void launch_point_features(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs, hipStream_t s) {
+    if (touchingPairs <= 0) return;
+    k_point_counts<<<blocks_for(touchingPairs), THREADS, 0, s>>>(D, C, L);
+    check_launch("point counts");
+    vr::scan_exclusive(L.pointBase, L.pointBase, touchingPairs, nullptr, C.sort.scanTmp, s);
+    k_point_features<<<blocks_for(touchingPairs), THREADS, 0, s>>>(D, C, L);
+    check_launch("point features");
+}
+
+void launch_point_targets(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs, hipStream_t s) {
+    if (touchingPairs <= 0) return;
+    k_point_targets<<<blocks_for(touchingPairs), THREADS, 0, s>>>(D, C, L);
+    check_launch("point targets");
+}
+
+void launch_apply_prediction(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs,
+                             hipStream_t s) {
+    if (touchingPairs <= 0) return;
+    k_apply_prediction<<<blocks_for(touchingPairs), THREADS, 0, s>>>(D, C, L);
+    check_launch("apply prediction");
 }
 
 void launch_scene_from_params(const BoxDev &D, SceneDev *scene, hipStream_t s) {

@@ -7,6 +7,10 @@
 
 #include "vexelray_rigid.h"
 
+namespace vrmlp {
+struct Net;
+}
+
 namespace vrbox {
 
 struct BoxWorld;
@@ -31,6 +35,10 @@ hipStream_t box_stream(BoxWorld *w);
 void box_params(BoxWorld *w, const int32_t *words, int32_t n);
 void box_step(BoxWorld *w, vr_step_report *rep);
 void box_wall_impulses_written(BoxWorld *w);  // after a write of lambda or rub through the ABI
+// The learned warm start: a log of per-point (features, targets), and the network that starts each step.
+void box_log(BoxWorld *w, bool on, int64_t capacity);
+int64_t box_samples(BoxWorld *w, float *x, float *t, int64_t most);
+void box_learned(BoxWorld *w, vrmlp::Net *net);
 void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold);
 
 }  // namespace vrbox

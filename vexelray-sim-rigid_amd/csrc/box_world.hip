@@ -17,6 +17,7 @@
 
 #include "box_kernels.h"
 #include "box_math.h"
+#include "mlp.h"
 #include "sphere_kernels.h"
 
 namespace vrbox {
@@ -47,6 +48,15 @@ struct BoxWorld {
     bool wallsFromParams = true;       // one scene, walled by the params' box (the reference's box world)
     SceneDev *sceneDev = nullptr;
     Colouring C{};
+    // The learned warm start (no counterpart in the reference): per-point data, a log of (features, targets), and
+    // the network whose prediction starts each step's impulses, when one is set.
+    Learned L{};
+    int64_t pointCap = 0;
+    bool logging = false;
+    float *logX = nullptr, *logT = nullptr;
+    int64_t logCap = 0, logCount = 0;
+    vrmlp::Net *net = nullptr;
+    int32_t *learnedUsed = nullptr;
     int32_t *pinned = nullptr;  // host: the colouring's counts read back
     int colourSteps = 1;        // steps of the colouring the last step needed
 
@@ -220,6 +230,45 @@ struct BoxWorld {
         HIP_OK(hipStreamSynchronize(stream));
     }
 
+    // Room for this step's points (at most 4 a touching pair), grown as needed.
+    void points_room(int64_t points) {
+        if (points <= pointCap) return;
+        int64_t cap = std::max<int64_t>(points + points / 4, 1024);
+        for (void *p : {(void *)L.pointBase, (void *)L.x, (void *)L.y, (void *)L.t})
+            if (p) HIP_OK(hipFree(p));
+        HIP_OK(hipMalloc((void **)&L.pointBase, sizeof(int32_t) * ((size_t)D.slots + 1)));
+        HIP_OK(hipMalloc((void **)&L.x, sizeof(float) * cap * LEARN_FEATURES));
+        HIP_OK(hipMalloc((void **)&L.y, sizeof(float) * cap * LEARN_OUTPUTS));
+        HIP_OK(hipMalloc((void **)&L.t, sizeof(float) * cap * LEARN_OUTPUTS));
+        if (!L.used) {
+            HIP_OK(hipMalloc((void **)&L.used, sizeof(int32_t)));
+            HIP_OK(hipMemset(L.used, 0, sizeof(int32_t)));
+        }
+        pointCap = cap;
+    }
+
+    void log(bool on, int64_t capacity) {
+        logging = on;
+        logCount = 0;
+        if (on && capacity > logCap) {
+            if (logX) HIP_OK(hipFree(logX));
+            if (logT) HIP_OK(hipFree(logT));
+            HIP_OK(hipMalloc((void **)&logX, sizeof(float) * capacity * LEARN_FEATURES));
+            HIP_OK(hipMalloc((void **)&logT, sizeof(float) * capacity * LEARN_OUTPUTS));
+            logCap = capacity;
+        }
+    }
+
+    int64_t samples(float *x, float *t, int64_t most) {
+        int64_t take = std::min(most, logCount);
+        if (take > 0) {
+            HIP_OK(hipMemcpyAsync(x, logX, (size_t)take * LEARN_FEATURES * 4, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipMemcpyAsync(t, logT, (size_t)take * LEARN_OUTPUTS * 4, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+        }
+        return take;
+    }
+
     // Wall impulses written from outside: which walls hold one is not known, so every wall is looked at.
     void wall_impulses_written() {
         vr::launch_fill_i32(D.wallHeld, (1 << WALLS) - 1, n, stream);
@@ -284,6 +333,14 @@ struct BoxWorld {
             launch_carry_counts(D, stream);
             // BoxStepper's colouring (first fit in slot order), on the device; the host reads back the counts.
             colour(&used, &touching, &points, &unplaced, &unlisted);
+            if ((net || logging) && touching > 0) {
+                points_room(points);
+                launch_point_features(D, C, L, touching, stream);
+                if (net) {
+                    net->forward(L.x, points, L.y, stream);
+                    launch_apply_prediction(D, C, L, touching, stream);
+                }
+            }
         }
         for (int sub = 0; sub < d.substeps; sub++) {
             launch_velocities(D, stream);
@@ -292,11 +349,29 @@ struct BoxWorld {
             launch_positions(D, stream);
             for (int k = 0; k < d.sweeps; k++) sweep(RELAX, d.symmetric && k % 2 == 1, used);
         }
+        if (logging && grid && touching > 0) {
+            launch_point_targets(D, C, L, touching, stream);  // what the next step's carry would start from
+            int64_t take = std::min<int64_t>(points, logCap - logCount);
+            if (take > 0) {
+                HIP_OK(hipMemcpyAsync(logX + logCount * LEARN_FEATURES, L.x, (size_t)take * LEARN_FEATURES * 4,
+                                      hipMemcpyDeviceToDevice, stream));
+                HIP_OK(hipMemcpyAsync(logT + logCount * LEARN_OUTPUTS, L.t, (size_t)take * LEARN_OUTPUTS * 4,
+                                      hipMemcpyDeviceToDevice, stream));
+                logCount += take;
+            }
+        }
         launch_show(D, stream);
         HIP_OK(hipEventRecord(t1, stream));
         HIP_OK(hipStreamSynchronize(stream));
         float ms = 0;
         HIP_OK(hipEventElapsedTime(&ms, t0, t1));
+        rep->reserved = 0;
+        if (net && grid && touching > 0) {
+            HIP_OK(hipMemcpyAsync(pinned + 48, L.used, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipMemsetAsync(L.used, 0, sizeof(int32_t), stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            rep->reserved = pinned[48];
+        }
         rep->rounds = used;
         rep->most_rounds = used;
         rep->touching = touching;
@@ -324,6 +399,9 @@ struct BoxWorld {
     ~BoxWorld() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (pinned) (void)hipHostFree(pinned);
+        for (void *p : {(void *)L.pointBase, (void *)L.x, (void *)L.y, (void *)L.t, (void *)L.used, (void *)logX,
+                        (void *)logT})
+            if (p) (void)hipFree(p);
         for (void *p : allocs) (void)hipFree(p);
         if (t0) (void)hipEventDestroy(t0);
         if (t1) (void)hipEventDestroy(t1);
@@ -388,6 +466,9 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n) {
 
 void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
 void box_wall_impulses_written(BoxWorld *w) { w->wall_impulses_written(); }
+void box_log(BoxWorld *w, bool on, int64_t capacity) { w->log(on, capacity); }
+int64_t box_samples(BoxWorld *w, float *x, float *t, int64_t most) { return w->samples(x, t, most); }
+void box_learned(BoxWorld *w, vrmlp::Net *net) { w->net = net; }
 
 void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold) {
     w->collide_pairs(pairs, npairs, manifold);

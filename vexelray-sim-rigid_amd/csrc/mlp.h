@@ -29,3 +29,8 @@ struct Net {
 };
 
 }  // namespace vrmlp
+
+struct vr_mlp;
+namespace vrmlp {
+Net *net_of(vr_mlp *m);  // the network behind a handle of the C ABI
+}  // namespace vrmlp

@@ -335,6 +335,8 @@ struct vr_mlp {
 
 namespace vrmlp {
 
+Net *net_of(vr_mlp *m) { return m ? &m->net : nullptr; }
+
 // Weights uniform with variance 1/fan-in, biases zero, from a seeded generator (std::mt19937_64).
 std::vector<float> initial_params(uint64_t seed) {
     std::vector<float> p(PARAMS, 0.f);

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "box_world.h"
+#include "mlp.h"
 #include "sphere_kernels.h"
 #include "vexelray_rigid.h"
 
@@ -721,6 +722,24 @@ int vr_box_collide(vr_world *w, const int32_t *pairs, int32_t npairs, float *man
     if (!w || (!pairs && npairs > 0) || (!manifold && npairs > 0)) return fail(VR_ERR_ARGUMENT, "null argument");
     if (!w->box) return fail(VR_ERR_STATE, "not a box world");
     return guard([&] { vrbox::box_collide(w->box, pairs, npairs, manifold); });
+}
+
+int vr_box_log(vr_world *w, int32_t on, int64_t capacity) {
+    if (!w || (on && capacity < 1)) return fail(VR_ERR_ARGUMENT, "a log needs room for a sample");
+    if (!w->box) return fail(VR_ERR_STATE, "not a box world");
+    return guard([&] { vrbox::box_log(w->box, on != 0, capacity); });
+}
+
+int vr_box_samples(vr_world *w, float *x, float *t, int64_t most, int64_t *count) {
+    if (!w || !count || most < 0 || (most > 0 && (!x || !t))) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (!w->box) return fail(VR_ERR_STATE, "not a box world");
+    return guard([&] { *count = vrbox::box_samples(w->box, x, t, most); });
+}
+
+int vr_box_learned(vr_world *w, vr_mlp *m) {
+    if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    if (!w->box) return fail(VR_ERR_STATE, "not a box world");
+    return guard([&] { vrbox::box_learned(w->box, vrmlp::net_of(m)); });
 }
 
 int vr_world_profile(vr_world *w, int32_t on) {
