@@ -9,6 +9,9 @@ grid cells), generated from a seeded java.util.Random. W untimed steps run first
 step's small report.
 
 One JSON line is printed: value = spheres x substeps x K / seconds. It carries:
+  batched: the 2,048-scene box batch (about 1.5 M bodies), body-substeps per second stepped plain (the reference's
+    carried warm start) and with the learned warm start (the network trained on the device for a few epochs on
+    samples logged from the plain steps), with the network's and the carry's held-out error on those samples.
   roofline: the Gauss-Seidel round kernels (k_round_wide ... k_round_tail, launched back to back for each pass),
     timed with HIP events on the world's stream around every pass over the timed region; algorithmic bytes per
     pass 72 C + 104 N (C contacts listed that substep, N spheres), against MI355X's 8 TB/s.
@@ -61,6 +64,59 @@ def cpu_baseline(pkg, spec, state, seconds):
     }
 
 
+def batched(pkg, steps, warmup, epochs):
+    """The 2,048-scene batch on the box path: plain (the reference's carry as the warm start), then with the learned
+    warm start, trained on the device on samples logged from the plain steps. Body-substeps per second for both."""
+    scenes = __import__(pkg.__name__ + ".scenes", fromlist=["scenes"])
+    boxes = __import__(pkg.__name__ + ".boxes", fromlist=["boxes"])
+    mlp = __import__(pkg.__name__ + ".mlp", fromlist=["mlp"])
+    spec = scenes.batched_2k()
+    n = int(spec.bodies.sum())
+    sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps)
+    sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
+              spec.params)
+    for _ in range(warmup):
+        sim.stepAndWait()
+    sim.log(True, 4 << 20)
+    for _ in range(3):
+        sim.stepAndWait()
+    x, t = sim.samples(4 << 20)
+    sim.log(False)
+
+    def timed():
+        t0 = time.perf_counter()
+        reps = [sim.stepAndWait() for _ in range(steps)]
+        return time.perf_counter() - t0, reps
+
+    plain_s, reps = timed()
+    net = mlp.Mlp(seed=3)
+    held = np.random.default_rng(0).permutation(len(x))
+    cut = int(0.9 * len(x))
+    t0 = time.perf_counter()
+    net.train(x[held[:cut]], t[held[:cut]], epochs=epochs, batch=4096, rate=1e-3, seed=1)
+    train_s = time.perf_counter() - t0
+    pred = net.forward(x[held[cut:]])
+    err_net = float(np.mean(np.sum((pred - t[held[cut:]]) ** 2, axis=1)))
+    err_carry = float(np.mean(np.sum((x[held[cut:]][:, 7:10] - t[held[cut:]]) ** 2, axis=1)))
+    sim.learned(net)
+    learned_s, lreps = timed()
+    used = sim.learnedPoints
+    sim.close()
+    net.close()
+    rate = lambda secs: n * spec.substeps * steps / secs  # noqa: E731
+    return {
+        "workload": f"batched_2k: {len(spec.bodies)} scenes, {n} bodies (half boxes, half balls), {spec.substeps} "
+                    f"substeps of {spec.sweeps} sweeps, seed {spec.seed}",
+        "steps_timed": steps, "warmup": warmup,
+        "plain": {"value": rate(plain_s), "ms_per_step": plain_s * 1e3 / steps,
+                  "touching_at_end": reps[-1].touching, "rounds_at_end": reps[-1].rounds},
+        "learned": {"value": rate(learned_s), "ms_per_step": learned_s * 1e3 / steps,
+                    "points_started_from_prediction": int(used), "samples": int(len(x)), "epochs": epochs,
+                    "train_s": train_s, "held_out_sq_error": {"network": err_net, "carried_impulse": err_carry}},
+        "unit": "body-substeps/s",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +124,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="pile_64k")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--batched-steps", type=int, default=30, help="timed steps of the batched workload (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build steps one world on one MI355X and has no multi-GPU path "
@@ -148,6 +205,8 @@ def main():
     }
     if args.cpu_seconds > 0:
         line["cpu_baseline"] = cpu_baseline(pkg, spec, state, args.cpu_seconds)
+    if args.batched_steps > 0:
+        line["batched"] = batched(pkg, args.batched_steps, 5, 10)
     print(json.dumps(line))
 
 
