@@ -155,6 +155,27 @@ int vr_last_error(char *buf, int32_t len);
  * tests run as a collide-only program (ctest/.../box/BoxContactTest.java:441-505).
  */
 int vr_box_collide(vr_world *w, const int32_t *pairs, int32_t npairs, float *manifold);
+/*
+ * Diagnostics reduced on the device in f64 (SphereDiagnostics.of, core/.../sphere/SphereDiagnostics.java:34-83;
+ * BoxDiagnostics.of, core/.../box/BoxDiagnostics.java:18-44, both host-side in the reference), into out[14]:
+ * VR_DIAG_OVERLAP deepest sphere-sphere overlap over the smaller radius (spheres; through the grid, rebuilt at the
+ * current positions, so the grid buffers change), VR_DIAG_WALL deepest wall overlap over the radius (spheres),
+ * kinetic energy (with spin), rotational energy, potential energy (m (-gy) y), momentum x y z, angular momentum x y z
+ * (about the origin), top speed, 1 if any state is not finite, and (boxes) the deepest manifold point of the last
+ * collide. The box and gravity are the params' f32 words. `n` must be at least 14.
+ */
+#define VR_DIAG_OVERLAP 0
+#define VR_DIAG_WALL 1
+#define VR_DIAG_KINETIC 2
+#define VR_DIAG_ROTATIONAL 3
+#define VR_DIAG_POTENTIAL 4
+#define VR_DIAG_MOMENTUM 5
+#define VR_DIAG_ANGULAR 8
+#define VR_DIAG_SPEED 11
+#define VR_DIAG_BROKEN 12
+#define VR_DIAG_DEPTH 13
+#define VR_DIAG_WORDS 14
+int vr_world_diagnostics(vr_world *w, double *out, int32_t n);
 /* Switches pass timing on or off and zeroes its counters. */
 int vr_world_profile(vr_world *w, int32_t on);
 int vr_world_profile_read(vr_world *w, vr_profile *out);

@@ -1361,6 +1361,106 @@ int vro_world_profile_read(vro_world *, vr_profile *out) {
     return VR_OK;
 }
 
+// SphereDiagnostics.of (SphereDiagnostics.java:34-83) and BoxDiagnostics.of (BoxDiagnostics.java:18-44) in f64,
+// every pair scanned for the overlap; the layout of vr_world_diagnostics.
+int vro_world_diagnostics(vro_world *p, double *out, int32_t n) {
+    if (!p || !out || n < 14 || !p->batch.empty()) return fail(VR_ERR_ARGUMENT, "diagnostics need 14 doubles");
+    auto f = [&](const char *name) -> const float * {
+        void *ptr;
+        int64_t count;
+        int32_t dtype;
+        return find_buffer(p, name, &ptr, &count, &dtype) ? (const float *)ptr : nullptr;
+    };
+    for (int k = 0; k < 14; k++) out[k] = 0;
+    const float *P = f("params");
+    if (p->kind == VR_SPHERES) {
+        int m = p->w.n;
+        const float *x = f("x"), *y = f("y"), *z = f("z"), *u = f("u"), *v = f("v"), *w = f("w"), *ax = f("ax"),
+                    *ay = f("ay"), *az = f("az"), *r = f("r"), *im = f("im");
+        for (int s = 0; s < m; s++) {
+            double X = x[s], Y = y[s], Z = z[s], U = u[s], V = v[s], W = w[s], A = ax[s], B = ay[s], C = az[s];
+            double R = r[s], M = im[s] > 0 ? 1.0 / (double)im[s] : 0.0;
+            if (!(std::isfinite(X) && std::isfinite(Y) && std::isfinite(Z) && std::isfinite(U) && std::isfinite(V) &&
+                  std::isfinite(W) && std::isfinite(A) && std::isfinite(B) && std::isfinite(C)))
+                out[12] = 1;
+            double at[3] = {X, Y, Z}, ext[3] = {P[SX], P[SY], P[SZ]};
+            for (int a = 0; a < 3; a++)
+                out[1] = std::max(out[1], std::max((R - at[a]) / R, (at[a] + R - ext[a]) / R));
+            double speed2 = U * U + V * V + W * W, spin2 = A * A + B * B + C * C, inertia = 0.4 * M * R * R;
+            out[11] = std::max(out[11], speed2);
+            out[2] += 0.5 * M * speed2 + 0.5 * inertia * spin2;
+            out[3] += 0.5 * inertia * spin2;
+            out[4] += M * -(double)P[GY] * Y;
+            out[5] += M * U;
+            out[6] += M * V;
+            out[7] += M * W;
+            out[8] += M * (Y * W - Z * V) + inertia * A;
+            out[9] += M * (Z * U - X * W) + inertia * B;
+            out[10] += M * (X * V - Y * U) + inertia * C;
+            for (int o2 = s + 1; o2 < m; o2++) {
+                double dx = X - x[o2], dy = Y - y[o2], dz = Z - z[o2];
+                double over = (R + r[o2] - std::sqrt(dx * dx + dy * dy + dz * dz)) / std::min(R, (double)r[o2]);
+                out[0] = std::max(out[0], over);
+            }
+        }
+    } else {
+        int64_t m;
+        void *ptr;
+        int32_t dtype;
+        find_buffer(p, "x", &ptr, &m, &dtype);
+        const float *X[3] = {f("x"), f("y"), f("z")}, *V[3] = {f("u"), f("v"), f("w")},
+                    *L[3] = {f("lx"), f("ly"), f("lz")}, *Q[4] = {f("qx"), f("qy"), f("qz"), f("qw")},
+                    *Hh[3] = {f("hx"), f("hy"), f("hz")}, *im = f("im"), *shape = f("shape"), *man = f("manifold");
+        int64_t slots = 0;
+        if (man) find_buffer(p, "manifold", &ptr, &slots, &dtype), slots /= 40;
+        for (int64_t s = 0; s < m; s++) {
+            double at[3], vel[3], l[3], q[4], h[3];
+            bool ok = true;
+            for (int a = 0; a < 3; a++) {
+                at[a] = X[a][s], vel[a] = V[a][s], l[a] = L[a][s], h[a] = Hh[a][s];
+                ok = ok && std::isfinite(at[a]) && std::isfinite(vel[a]) && std::isfinite(l[a]);
+            }
+            for (int k = 0; k < 4; k++) q[k] = Q[k][s];
+            if (!ok) out[12] = 1;
+            out[11] = std::max(out[11], vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+            if (im[s] > 0) {
+                double M = 1.0 / (double)im[s], moment[3];
+                if (shape[s] > 0.5f) moment[0] = moment[1] = moment[2] = 0.4 * M * h[0] * h[0];
+                else moment[0] = M * (h[1] * h[1] + h[2] * h[2]) / 3, moment[1] = M * (h[0] * h[0] + h[2] * h[2]) / 3,
+                     moment[2] = M * (h[0] * h[0] + h[1] * h[1]) / 3;
+                auto rot = [](const double *qq, const double *pp, double *oo) {
+                    double t[3] = {2 * (qq[1] * pp[2] - qq[2] * pp[1]), 2 * (qq[2] * pp[0] - qq[0] * pp[2]),
+                                   2 * (qq[0] * pp[1] - qq[1] * pp[0])};
+                    double c[3] = {qq[1] * t[2] - qq[2] * t[1], qq[2] * t[0] - qq[0] * t[2], qq[0] * t[1] - qq[1] * t[0]};
+                    for (int a = 0; a < 3; a++) oo[a] = pp[a] + qq[3] * t[a] + c[a];
+                };
+                double cq[4] = {-q[0], -q[1], -q[2], q[3]}, own[3], sc[3], omega[3];
+                rot(cq, l, own);
+                for (int a = 0; a < 3; a++) sc[a] = moment[a] > 0 ? own[a] / moment[a] : 0;
+                rot(q, sc, omega);
+                double orbit[3] = {at[1] * vel[2] - at[2] * vel[1], at[2] * vel[0] - at[0] * vel[2],
+                                   at[0] * vel[1] - at[1] * vel[0]};
+                double spin = 0;
+                for (int a = 0; a < 3; a++) {
+                    out[2] += 0.5 * M * vel[a] * vel[a];
+                    spin += 0.5 * l[a] * omega[a];
+                    out[5 + a] += M * vel[a];
+                    out[8 + a] += M * orbit[a] + l[a];
+                }
+                out[2] += spin;
+                out[3] += spin;
+                out[4] += M * -(double)P[2] * at[1];
+            }
+            for (int k = 0; k < 32 && s * 32 + k < slots; k++) {
+                const float *mm = man + (s * 32 + k) * 40;
+                for (int pt = 0; pt < (int)mm[3] && pt < 4; pt++) out[13] = std::max(out[13], (double)mm[8 + pt * 8 + 6]);
+            }
+        }
+    }
+    out[11] = std::sqrt(out[11]);
+    return VR_OK;
+}
+
 // The learned warm start runs on the device only: the oracle's box worlds keep the reference's carry.
 int vro_box_log(vro_world *, int32_t, int64_t) { return fail(VR_ERR_STATE, "the oracle keeps no sample log"); }
 int vro_box_samples(vro_world *, float *, float *, int64_t, int64_t *) {

@@ -27,7 +27,7 @@ ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "wor
                 "world_friction", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
                 "world_profile_read", "box_collide", "batch_create", "mlp_create", "mlp_destroy", "mlp_read",
                 "mlp_write", "mlp_forward", "mlp_gradient", "mlp_train", "mlp_last_error", "box_log",
-                "box_samples", "box_learned")
+                "box_samples", "box_learned", "world_diagnostics")
 
 
 class StepDesc(ctypes.Structure):
@@ -93,6 +93,7 @@ class Abi:
             "box_samples": ([p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int64,
                              ctypes.POINTER(ctypes.c_int64)]),
             "box_learned": ([p, p]),
+            "world_diagnostics": ([p, ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
             "batch_create": ([ctypes.POINTER(StepDesc), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(p)]),
@@ -200,6 +201,12 @@ class World:
 
     def finish(self) -> None:
         self._check(self.abi.fn("world_finish")(self.handle))
+
+    def diagnostics(self) -> np.ndarray:
+        out = np.zeros(14, dtype=np.float64)
+        self._check(self.abi.fn("world_diagnostics")(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                     14))
+        return out
 
     def profile(self, on: bool) -> None:
         self._check(self.abi.fn("world_profile")(self.handle, 1 if on else 0))

@@ -231,6 +231,13 @@ class BoxSimulation:
         (None), as the reference does."""
         self._world._check(self._abi.fn("box_learned")(self._world.handle, net._h if net is not None else None))
 
+    def diagnostics(self):
+        """BoxDiagnostics.of on the device (vr_world_diagnostics), with the top speed, whether any state is not
+        finite, and the deepest manifold point of the last collide: (BoxDiagnostics, speed, broken, depth)."""
+        d = self._world.diagnostics()
+        return (BoxDiagnostics(float(d[2]), float(d[3]), d[5:8].copy(), d[8:11].copy()), float(d[11]),
+                bool(d[12] != 0), float(d[13]))
+
     def state(self) -> List[np.ndarray]:
         """``BoxSimulation.state`` (BoxSimulation.java:150-160): x, y, z, u, v, w, lx, ly, lz, qx, qy, qz, qw."""
         return [self.floats(k) for k in ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw")]

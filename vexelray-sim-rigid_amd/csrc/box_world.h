@@ -39,6 +39,7 @@ void box_wall_impulses_written(BoxWorld *w);  // after a write of lambda or rub 
 void box_log(BoxWorld *w, bool on, int64_t capacity);
 int64_t box_samples(BoxWorld *w, float *x, float *t, int64_t most);
 void box_learned(BoxWorld *w, vrmlp::Net *net);
+void box_diagnostics(BoxWorld *w, double *out);  // vr_world_diagnostics for a box world
 void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold);
 
 }  // namespace vrbox

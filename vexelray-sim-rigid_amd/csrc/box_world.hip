@@ -17,6 +17,7 @@
 
 #include "box_kernels.h"
 #include "box_math.h"
+#include "diagnostics.h"
 #include "mlp.h"
 #include "sphere_kernels.h"
 
@@ -275,6 +276,18 @@ struct BoxWorld {
         HIP_OK(hipStreamSynchronize(stream));
     }
 
+    double *diagPartials = nullptr, *diagOut = nullptr;
+
+    void diagnostics(double *out) {  // BoxDiagnostics.of, reduced on the device
+        if (!diagOut) {
+            diagPartials = alloc<double>(vrdiag::partial_words(n));
+            diagOut = alloc<double>(vrdiag::D_WORDS);
+        }
+        vrdiag::launch_boxes(D, diagPartials, diagOut, stream);
+        HIP_OK(hipMemcpyAsync(out, diagOut, sizeof(double) * vrdiag::D_WORDS, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
     int32_t read_word(const int32_t *p) {
         HIP_OK(hipMemcpyAsync(pinned, p, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
@@ -467,6 +480,7 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n) {
 void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
 void box_wall_impulses_written(BoxWorld *w) { w->wall_impulses_written(); }
 void box_log(BoxWorld *w, bool on, int64_t capacity) { w->log(on, capacity); }
+void box_diagnostics(BoxWorld *w, double *out) { w->diagnostics(out); }
 int64_t box_samples(BoxWorld *w, float *x, float *t, int64_t most) { return w->samples(x, t, most); }
 void box_learned(BoxWorld *w, vrmlp::Net *net) { w->net = net; }
 

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "box_world.h"
+#include "diagnostics.h"
 #include "mlp.h"
 #include "sphere_kernels.h"
 #include "vexelray_rigid.h"
@@ -73,6 +74,7 @@ struct vr_world {
     int32_t *devOpening = nullptr, *devEnough = nullptr;  // SphereStepper's opening batches, kept on the device
     std::vector<int> wide, wideRub;    // wide rounds to launch per pass, from what the last step used
     bool hostWaits = false;            // run the reference's launch model: host reads each report
+    double *diagPartials = nullptr, *diagOut = nullptr;  // device scratch of vr_world_diagnostics, made on first use
     bool profiling = false;
     vr_profile prof{};
     std::vector<hipEvent_t> passEvents;  // a pair a pass, recorded around its round kernels when profiling
@@ -503,6 +505,21 @@ struct vr_world {
         r->wall_ns = std::chrono::duration_cast<std::chrono::nanoseconds>(w1 - w0).count();
     }
 
+    // SphereDiagnostics.of, reduced on the device; the grid is rebuilt at the current positions for the overlaps.
+    void diagnostics(double *out) {
+        if (!diagOut) {
+            diagPartials = alloc<double>(vrdiag::partial_words(n));
+            diagOut = alloc<double>(vrdiag::D_WORDS);
+        }
+        if (grid) {
+            launch_bin(D, stream);
+            launch_sort(D, S, stream);
+        }
+        vrdiag::launch_spheres(D, grid, diagPartials, diagOut, stream);
+        HIP_OK(hipMemcpyAsync(out, diagOut, sizeof(double) * vrdiag::D_WORDS, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
     ~vr_world() {
         if (box) vrbox::box_destroy(box);
         if (stream) (void)hipStreamSynchronize(stream);
@@ -740,6 +757,12 @@ int vr_box_learned(vr_world *w, vr_mlp *m) {
     if (!w) return fail(VR_ERR_ARGUMENT, "null world");
     if (!w->box) return fail(VR_ERR_STATE, "not a box world");
     return guard([&] { vrbox::box_learned(w->box, vrmlp::net_of(m)); });
+}
+
+int vr_world_diagnostics(vr_world *w, double *out, int32_t n) {
+    if (!w || !out || n < vrdiag::D_WORDS) return fail(VR_ERR_ARGUMENT, "diagnostics need 14 doubles");
+    if (w->box) return guard([&] { vrbox::box_diagnostics(w->box, out); });
+    return guard([&] { w->diagnostics(out); });
 }
 
 int vr_world_profile(vr_world *w, int32_t on) {

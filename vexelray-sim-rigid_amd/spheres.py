@@ -289,6 +289,13 @@ class SphereSimulation:
     def readShown(self) -> np.ndarray:
         return self.ints("shown")
 
+    def diagnostics(self) -> "SphereDiagnostics":
+        """SphereDiagnostics.of on the device (vr_world_diagnostics): f64 reductions, overlaps through the grid
+        rebuilt at the current positions."""
+        d = self._world.diagnostics()
+        return SphereDiagnostics(float(d[0]), float(d[1]), float(d[2]), float(d[3]), float(d[4]), d[5:8].copy(),
+                                 d[8:11].copy(), float(d[11]), bool(d[12] != 0))
+
     def contactCount(self) -> Optional[np.ndarray]:
         return self.ints("contactCount") if self.step_.solve == Solve.GAUSS_SEIDEL_BY_CONTACT else None
 
