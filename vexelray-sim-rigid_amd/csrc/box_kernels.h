@@ -86,7 +86,7 @@ void launch_velocities(const BoxDev &D, hipStream_t s);
 void launch_scene_from_params(const BoxDev &D, SceneDev *scene, hipStream_t s);
 void launch_positions(const BoxDev &D, hipStream_t s);
 void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s);
-void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int r, hipStream_t s);
+void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int r, int pairs, hipStream_t s);
 void launch_bin(const BoxDev &D, hipStream_t s);
 void launch_list(const BoxDev &D, hipStream_t s);
 void launch_collide(const BoxDev &D, hipStream_t s);

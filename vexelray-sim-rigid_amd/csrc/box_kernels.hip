@@ -74,6 +74,9 @@ __global__ void k_positions(BoxDev D) {
     for (int k = 0; k < 4; k++) B.q[k][s] = end[k];
 }
 
+// v[a] for a run-time a, as a register select (an indexed small array would go to scratch memory).
+__device__ __forceinline__ float pick(const float *v, int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
+
 // Boxes.walls, Boxes.java:357-416: one sweep of a body's corners (a sphere's nearest point) against the six walls.
 template <int SWEEP, bool REVERSE>
 __global__ void k_walls(BoxDev D) {
@@ -101,8 +104,8 @@ __global__ void k_walls(BoxDev D) {
             for (int a = 0; a < 3; a++) r[a] = -(box.half[0] * nrm[a]);
             size_t at = (size_t)firstSlot * n + s;
             size_t tangentAt = (size_t)firstSlot * 3 * n + s;
-            float reach = box.at[axis] + r[axis];
-            float gap = high ? extent[axis] - reach : reach;
+            float reach = pick(box.at, axis) + pick(r, axis);
+            float gap = high ? pick(extent, axis) - reach : reach;
             contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft);
             roll(sweep, nullptr, box, box.half[0], B.lambda, at, B.roll, tangentAt, n, P);
         } else {
@@ -119,8 +122,8 @@ __global__ void k_walls(BoxDev D) {
                     offset[a] = box.half[a] * (2.0f * (float)bit - 1.0f);
                 }
                 rotate(box.q, offset, r);
-                float reach = box.at[axis] + r[axis];
-                float gap = high ? extent[axis] - reach : reach;
+                float reach = pick(box.at, axis) + pick(r, axis);
+                float gap = high ? pick(extent, axis) - reach : reach;
                 if (!live && !(gap < soft.margin)) continue;
                 size_t at = (size_t)(firstSlot + c) * n + s;
                 size_t tangentAt = (size_t)(firstSlot + c) * 3 * n + s;
@@ -611,8 +614,10 @@ void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s) {
     check_launch("walls");
 }
 
-void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int r, hipStream_t s) {
-    int blocks = blocks_for(D.width);
+void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int r, int pairs, hipStream_t s) {
+    if (pairs <= 0) return;
+    if (pairs > D.width) throw std::runtime_error("solve round: more pairs than the round holds");
+    int blocks = blocks_for(pairs);  // the round's pairs are its first places; the rest hold -1
     switch (sweep * 2 + (reverse ? 1 : 0)) {
         case 0: k_solve_round<WARM, false><<<blocks, THREADS, 0, s>>>(D, r); break;
         case 1: k_solve_round<WARM, true><<<blocks, THREADS, 0, s>>>(D, r); break;

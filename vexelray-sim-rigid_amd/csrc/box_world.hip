@@ -60,6 +60,7 @@ struct BoxWorld {
     int32_t *learnedUsed = nullptr;
     int32_t *pinned = nullptr;  // host: the colouring's counts read back
     int colourSteps = 1;        // steps of the colouring the last step needed
+    int roundPairs[MOST_ROUNDS] = {};  // pairs in each round this step
 
     template <class T>
     T *alloc(int64_t count) {
@@ -313,8 +314,10 @@ struct BoxWorld {
         HIP_OK(hipMemcpyAsync(pinned + 40, C.report, REPORT_WORDS * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
         *used = 0;
-        for (int r = 0; r < MOST_ROUNDS; r++)
-            if (pinned[r + 1] > pinned[r]) *used = r + 1;
+        for (int r = 0; r < MOST_ROUNDS; r++) {
+            roundPairs[r] = pinned[r + 1] - pinned[r];
+            if (roundPairs[r] > 0) *used = r + 1;
+        }
         *touching = pairs;
         *unplaced = pinned[MOST_ROUNDS + 1] - pinned[MOST_ROUNDS];
         *points = pinned[40 + REPORT_POINTS];
@@ -324,8 +327,10 @@ struct BoxWorld {
     // BoxStep.sweep, BoxStep.java:217-231.
     void sweep(int kind, bool reverse, int rounds) {
         if (reverse) launch_walls(D, kind, reverse, stream);
-        for (int order = 0; order < rounds; order++)
-            launch_solve_round(D, kind, reverse, reverse ? rounds - 1 - order : order, stream);
+        for (int order = 0; order < rounds; order++) {
+            int r = reverse ? rounds - 1 - order : order;
+            launch_solve_round(D, kind, reverse, r, roundPairs[r], stream);
+        }
         if (!reverse) launch_walls(D, kind, reverse, stream);
     }
 
