@@ -1,6 +1,6 @@
 """Benchmark: the Gauss-Seidel sphere step on the 65,536-sphere pile, body-substeps per second on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pile_64k] [--cpu-seconds S]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pile_64k|mixed_32k] [--cpu-seconds S]
 
 The workload is the pile of 65,536 spheres (radius log-uniform 0.1-0.5 m, density 1000 kg/m^3, dropped with
 N(0, 1) m/s velocities into a 100 x 100 m walled floor; mu 0.4, e 0.1; dt 1/60 s in 8 substeps of 10 passes, 1 m
@@ -117,6 +117,45 @@ def batched(pkg, steps, warmup, epochs):
     }
 
 
+_last_progress = [time.perf_counter()]
+
+
+def progress(done, total):
+    """A progress line every 100 steps, or after 30 s without one (steps run slower under a counter profiler)."""
+    now = time.perf_counter()
+    if done % 100 == 0 or now - _last_progress[0] > 30:
+        _last_progress[0] = now
+        print(f"[bench] {done}/{total} steps", file=sys.stderr, flush=True)
+
+
+def box_scene(pkg, spec, args):
+    """A box-path workload of one scene (mixed_32k): body-substeps per second, plain stepping."""
+    boxes = __import__(pkg.__name__ + ".boxes", fromlist=["boxes"])
+    sim = boxes.BoxSimulation(spec.step)
+    sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
+              spec.params)
+    for _ in range(args.warmup):
+        sim.stepAndWait()
+    t0 = time.perf_counter()
+    reps = []
+    for k in range(args.steps):
+        reps.append(sim.stepAndWait())
+        progress(k + 1, args.steps)
+    seconds = time.perf_counter() - t0
+    diag, speed, broken, depth = sim.diagnostics()
+    sim.close()
+    n, substeps = spec.step.boxes, spec.step.substeps
+    return {
+        "metric": METRIC, "value": n * substeps * args.steps / seconds, "unit": "body-substeps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": seconds * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{spec.name}: {n} bodies (16,384 boxes, 16,384 balls, {spec.towers} towers of 20 "
+                               f"cubes), box path, {substeps} substeps of {spec.step.sweeps} sweeps, seed {spec.seed}",
+                   "bodies": n, "method": "plain", "touching_at_end": reps[-1].touching,
+                   "rounds_at_end": reps[-1].rounds, "deepest_point_at_end": depth, "broken": broken},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,15 +164,31 @@ def main():
     ap.add_argument("--workload", default="pile_64k")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--batched-steps", type=int, default=30, help="timed steps of the batched workload (0: skip)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB: only copy a buffer of that size 5 times (the known-size kernel that calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build steps one world on one MI355X and has no multi-GPU path "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe > 0:
+        import torch
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": src.numel() * 4, "copy_probe_bytes_written": dst.numel() * 4}))
+        return
+
     pkg = load_package()
     scenes = __import__(pkg.__name__ + ".scenes", fromlist=["scenes"])
     spec = scenes.SCENES[args.workload]()
+    if hasattr(spec, "towers"):  # a box-path scene
+        print(json.dumps(box_scene(pkg, spec, args)))
+        return
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
@@ -152,8 +207,7 @@ def main():
     for k in range(args.steps):
         rep = sim.stepAndWait()
         reports.append(rep.solve)
-        if (k + 1) % 100 == 0:
-            print(f"[bench] {k + 1}/{args.steps} steps", file=sys.stderr, flush=True)
+        progress(k + 1, args.steps)
     sim.finish()
     sync()
     seconds = time.perf_counter() - t0
