@@ -255,6 +255,7 @@ def main():
             "bytes_per_launch": per_launch_bytes,
             "avg_launch_us": prof.round_ns / 1e3 / max(prof.round_launches, 1),
             "avg_pass_us": prof.round_ns / 1e3 / passes,
+            "host_enqueue_ms_per_step": prof.enqueue_ns / 1e6 / max(prof.steps, 1),
         },
     }
     if args.cpu_seconds > 0:

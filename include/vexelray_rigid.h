@@ -115,7 +115,9 @@ typedef struct vr_step_report {
  * Measurement (no counterpart in the reference): HIP events on the world's stream bracket every Gauss-Seidel pass,
  * whose round kernels (k_round_wide ... k_round_tail) are launched back to back, so the time in round kernels is
  * known without a profiler. listed_sum adds up, over every pass run (parting and rubbing), the contacts listed for
- * its substep: the C of the algorithmic-byte count 72 C + 104 N a pass is credited with.
+ * its substep: the C of the algorithmic-byte count 72 C + 104 N a pass is credited with. enqueue_ns adds up the
+ * host's time from a step's first launch to its last, before it waits for the step: when it is close to the step's
+ * time, the host's launch rate, not the GPU, sets the pace.
  */
 typedef struct vr_profile {
     int64_t round_ns;
@@ -123,6 +125,7 @@ typedef struct vr_profile {
     int64_t passes;
     int64_t listed_sum;
     int64_t steps;
+    int64_t enqueue_ns;
 } vr_profile;
 
 int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out);

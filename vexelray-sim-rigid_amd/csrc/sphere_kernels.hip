@@ -16,6 +16,7 @@ namespace vr {
 namespace {
 
 constexpr int THREADS = 256;
+static_assert(THREADS == SEG, "a wide round's workgroup takes one open-list segment, a place per thread");
 constexpr float COINCIDENT = 1e-12f;   // Spheres.java:156
 constexpr float LIST_MARGIN = 0.02f;   // Spheres.java:795
 constexpr float INERTIA = 0.4f;        // Spheres.java:246
@@ -268,6 +269,7 @@ __device__ __forceinline__ PairLoad load_pair(const SphereDev &D, int c, int a, 
 
 // Makes the compiler have `v` in a register here, so a load of it is issued no later than this point.
 __device__ __forceinline__ void issued(float v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void issued(int v) { asm volatile("" ::"v"(v)); }
 
 __device__ __forceinline__ void issued(const Held &h, bool rubs) {
     for (int k = 0; k < 3; k++) {
@@ -565,34 +567,66 @@ __global__ void k_round(SphereDev D, RoundConst rc) {
 }
 
 // One contact of a round run until done (the body of Spheres.contactRound, Spheres.java:1088-1131, for a contact
-// still open): it solves if it holds both its spheres' claims from the round before, else clears the spheres' third
-// claim array and claims both. Returns whether it is still open. The solve's loads are issued with the claim check,
-// not after it: in a round, a contact's spheres are written only by the one contact that holds both their claims,
-// so nothing changes what was loaded before it is used.
-template <bool RUBS>
+// still open): it solves if it holds both its spheres' claims from the round before. Returns whether it is still
+// open, and so claims (claim_wave). EAGER issues the solve's loads with the claim check, not after it (in a round, a
+// contact's spheres are written only by the one contact that holds both their claims, so nothing changes what was
+// loaded before it is used): one memory latency fewer for the contacts that solve, but every open contact loads its
+// pair, and only one in six to ten solves in a round. The wide rounds, bound by how many loads the memory system
+// takes, load after the check; the tail, a few contacts in one workgroup and bound by latency, loads with it.
+template <bool RUBS, bool EAGER>
 __device__ __forceinline__ bool round_contact(const SphereDev &D, const Params &P, const RoundConst &rc, int c, int a,
                                               int o) {
+    if (rc.checks == 0) return true;
     const long long *checked = D.claims[(rc.slot + 2) % 3];
-    long long *claimed = D.claims[rc.slot % 3];
-    long long *cleared = D.claims[(rc.slot + 1) % 3];
-    if (rc.checks > 0) {
-        long long last = priority(D, a, o, rc.checksDispatch);
-        long long ca = checked[a], co = checked[o];
+    long long last = priority(D, a, o, rc.checksDispatch);
+    long long ca = checked[a], co = checked[o];
+    if (EAGER) {
         PairLoad L = load_pair(D, c, a, o);
         issued<RUBS>(L);
         for (int k = 0; k < PARAM_COUNT; k++) issued(P[k]);
-        if (ca == last && co == last) {
-            contact<RUBS>(D, P, c, L, rc.pass);
-            D.cur.done[c] = rc.pass;
-            return false;
-        }
+        if (ca != last || co != last) return true;
+        contact<RUBS>(D, P, c, L, rc.pass);
+    } else {
+        if (ca != last || co != last) return true;
+        PairLoad L = load_pair(D, c, a, o);
+        contact<RUBS>(D, P, c, L, rc.pass);
     }
-    cleared[a] = 0;
+    D.cur.done[c] = rc.pass;
+    return false;
+}
+
+// The claims of a wave's still-open contacts (`claims`): each clears both its spheres' words in the third claim
+// array and raises both its spheres' claims to its priority (Spheres.java:1121-1130). Contacts are listed by lower
+// sphere a, so a wave's lanes hold runs of one a side by side; clearing a word twice is clearing it once, and atomicMax
+// of a run's priorities one by one leaves the word where one atomicMax of the run's largest does. So the last lane of
+// each run of claiming lanes with one a clears and claims a alone, with the run's largest priority (a segmented max
+// scan over the wave), and every word ends the round as it would with one clear and one atomic per contact: fewer
+// device atomics, the same claims. Sphere o is claimed once per contact. Called by every lane of the wave.
+__device__ __forceinline__ void claim_wave(const SphereDev &D, const RoundConst &rc, bool claims, int a, int o) {
+    long long *claimed = D.claims[rc.slot % 3];
+    long long *cleared = D.claims[(rc.slot + 1) % 3];
+    int lane = threadIdx.x & 63;
+    long long mine = claims ? priority(D, a, o, rc.dispatch) : 0;
+    int key = claims ? a : -1 - lane;  // a lane that does not claim is a run of its own
+    // Both shuffles run in every lane, before any test: a shuffle inside a short-circuit would run with lanes off,
+    // and a lane reading from a lane that is off reads nothing it can use.
+    int before = __shfl_up(key, 1, 64), after = __shfl_down(key, 1, 64);
+    bool head = lane == 0 || before != key;
+    bool tail = lane == 63 || after != key;
+    unsigned long long heads = __ballot(head) & ((2ull << lane) - 1ull);  // lane 63: every lane
+    int start = 63 - __clzll((long long)heads);
+    long long most = mine;
+    for (int d = 1; d < 64; d <<= 1) {
+        long long t = __shfl_up(most, d, 64);
+        if (lane - d >= start) most = max(most, t);
+    }
+    if (!claims) return;
     cleared[o] = 0;
-    long long mine = priority(D, a, o, rc.dispatch);
-    atomicMax(&claimed[a], mine);
     atomicMax(&claimed[o], mine);
-    return true;
+    if (tail) {
+        cleared[a] = 0;
+        atomicMax(&claimed[a], most);
+    }
 }
 
 constexpr int CYCLE = 48, BATCH = 8, SHRINK_AFTER = 120;  // SphereStep.java:48-59, SphereStepper.java:49
@@ -644,20 +678,46 @@ __global__ void k_step_begin(PassCtl *ctl, PassSlot *slots, int words) {
     }
 }
 
+// An open-list entry {c, a, o, 0} as loaded: the fourth word is always 0, so it is not taken from memory (a store
+// of a loaded word would wait for that load, and with it for every memory operation issued after it).
+__device__ __forceinline__ int4 open_entry(const int4 *list, size_t place) {
+    int4 v = list[place];
+    return make_int4(v.x, v.y, v.z, 0);
+}
+
 // Round j of a pass over the whole chip, a workgroup per segment of SEG list places. In round 0, workgroup g takes
-// the listed contacts g*SEG onwards; in each later round it takes the open contacts it kept in the round before. It
-// compacts the contacts that stay open into its own segment of the other list, counting in LDS, so no round has a
-// device-wide counter; the pass's last wide round (`last`) compacts into one list instead, for k_round_tail. Which
-// order open contacts land in does not matter: a round's outcome is the same in any order.
-template <bool RUB>
+// the listed contacts g*SEG onwards; in each later round it takes the open contacts it kept in the round before
+// (segIn[g] of them from in[g*SEG]). It compacts the contacts that stay open into its own segment of `out`, counting
+// in LDS, so no round has a device-wide counter; the pass's last wide round (`last`) compacts into one list
+// instead, for k_round_tail. Which order open contacts land in does not matter: a round's outcome is the same in any
+// order. The segment's count, the thread's entry and, in a rubbing pass, the parting pass's rounds do not depend on
+// one another, so all three are loaded before any of them is tested: one memory latency, not three, before the
+// contact's own loads.
+// FIRST: round 0 (j == 0), an instance of its own, so neither instance joins two ways of loading.
+template <bool RUB, bool FIRST>
 __global__ void __launch_bounds__(THREADS) k_round_wide(SphereDev D, const PassCtl *ctl, PassSlot *slot,
-                                                        OpenLists L, int it, int iterations, int j, int last) {
-    if (RUB && j >= ctl->k) return;
+                                                        const int4 *in, const int32_t *segIn, int4 *out,
+                                                        int32_t *segOut, int it, int iterations, int j, int last) {
     __shared__ int sCount;
-    int g = blockIdx.x;
-    int nIn = j == 0 ? min(SEG, min(D.count[LISTED], D.cap) - g * SEG) : L.seg[j & 1][g];
+    int g = blockIdx.x, i = threadIdx.x;
+    int kRub = RUB ? ctl->k : 0;
+    int nIn;
+    int4 e = make_int4(0, 0, 0, 0);
+    if (FIRST) {
+        nIn = min(SEG, min(D.count[LISTED], D.cap) - g * SEG);
+        e = listed_entry(D, min(g * SEG + i, D.cap - 1));  // launched only when cap >= 1; used only below nIn
+    } else {
+        nIn = segIn[g];
+        e = open_entry(in, (size_t)g * SEG + i);  // in the list's segments * SEG places; used only below nIn
+    }
+    issued(kRub);
+    issued(nIn);
+    issued(e.x);
+    issued(e.y);
+    issued(e.z);
+    if (RUB && j >= kRub) return;
     if (nIn <= 0) {
-        if (!last && threadIdx.x == 0) L.seg[(j + 1) & 1][g] = 0;
+        if (!last && threadIdx.x == 0) segOut[g] = 0;
         return;
     }
     if (threadIdx.x == 0) {
@@ -667,19 +727,15 @@ __global__ void __launch_bounds__(THREADS) k_round_wide(SphereDev D, const PassC
     __syncthreads();
     RoundConst rc = cycle_round(it, position(j), RUB, iterations);
     const Params P = load_params(D.params);
-    int i = threadIdx.x;
     bool open = false;
-    int4 e = make_int4(0, 0, 0, 0);
-    if (i < nIn) {
-        e = j == 0 ? listed_entry(D, g * SEG + i) : L.buf[j & 1][(size_t)g * SEG + i];
-        open = round_contact<RUB>(D, P, rc, e.x, e.y, e.z);
-    }
+    if (i < nIn) open = round_contact<RUB, false>(D, P, rc, e.x, e.y, e.z);
+    claim_wave(D, rc, open, e.y, e.z);
     if (last) {
-        append_open(open, e, L.buf[(j + 1) & 1], &slot->tailCount);
+        append_open(open, e, out, &slot->tailCount);
     } else {
-        append_open(open, e, L.buf[(j + 1) & 1] + (size_t)g * SEG, &sCount);
+        append_open(open, e, out + (size_t)g * SEG, &sCount);
         __syncthreads();
-        if (threadIdx.x == 0) L.seg[(j + 1) & 1][g] = sCount;
+        if (threadIdx.x == 0) segOut[g] = sCount;
     }
 }
 
@@ -736,9 +792,10 @@ __global__ void __launch_bounds__(TAIL) k_round_tail(SphereDev D, PassCtl *ctl, 
             bool open = false;
             int4 e = make_int4(0, 0, 0, 0);
             if (i < n) {
-                e = j == 0 ? listed_entry(D, i) : in[i];
-                open = round_contact<RUB>(D, P, rc, e.x, e.y, e.z);
+                e = j == 0 ? listed_entry(D, i) : open_entry(in, i);
+                open = round_contact<RUB, true>(D, P, rc, e.x, e.y, e.z);
             }
+            claim_wave(D, rc, open, e.y, e.z);
             append_open(open, e, out, &sCount);
         }
         __syncthreads();
@@ -1298,10 +1355,16 @@ void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLis
     if (wide < 0 || wide > MAX_WIDE) throw std::runtime_error("pass: wide rounds out of range");
     if ((int64_t)L.segments * SEG < D.cap) throw std::runtime_error("pass: open lists shorter than the contact list");
     int limit = D.cap + 2 * CYCLE;
+    if (L.segments == 0) wide = 0;  // no contact can be listed: the tail alone finds the pass empty
     for (int j = 0; j < wide; j++) {
         int last = j == wide - 1 ? 1 : 0;
-        if (rub) k_round_wide<true><<<L.segments, THREADS, 0, s>>>(D, ctl, slot, L, it, iterations, j, last);
-        else k_round_wide<false><<<L.segments, THREADS, 0, s>>>(D, ctl, slot, L, it, iterations, j, last);
+        const int4 *in = L.buf[j & 1];
+        const int32_t *segIn = L.seg[j & 1];
+        int4 *out = L.buf[(j + 1) & 1];
+        int32_t *segOut = L.seg[(j + 1) & 1];
+        auto kernel = rub ? (j == 0 ? k_round_wide<true, true> : k_round_wide<true, false>)
+                          : (j == 0 ? k_round_wide<false, true> : k_round_wide<false, false>);
+        kernel<<<L.segments, THREADS, 0, s>>>(D, ctl, slot, in, segIn, out, segOut, it, iterations, j, last);
     }
     if (rub) k_round_tail<true><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide, limit);
     else k_round_tail<false><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide, limit);

@@ -349,6 +349,7 @@ struct vr_world {
     // rounds, then the tail that finishes it), so the step runs with no host wait; the rounds run and the report
     // are the reference's.
     void step_device(vr_step_report *rep) {
+        auto enqueueStart = std::chrono::steady_clock::now();
         launch_step_begin(ctl, slots, nslots, stream);
         int q = 0;
         launchesThisStep = 0;
@@ -377,6 +378,8 @@ struct vr_world {
         velocity_list();
         launch_show(D, grid, stream);
         HIP_OK(hipMemcpyAsync(ctlHost, ctl, sizeof(PassCtl), hipMemcpyDeviceToHost, stream));
+        int64_t enqueued = std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() -
+                                                                                enqueueStart).count();
         HIP_OK(hipStreamSynchronize(stream));
         if (ctlHost->error)
             throw StateError("a pass is still open after " + std::to_string(ctlHost->errRounds) +
@@ -391,6 +394,7 @@ struct vr_world {
             prof.passes += q;
             prof.listed_sum += ctlHost->listedSum;
             prof.steps += 1;
+            prof.enqueue_ns += enqueued;
         }
         rep->waits = ctlHost->waits;
         rep->rounds = ctlHost->rounds;
