@@ -74,13 +74,48 @@ __global__ void k_positions(BoxDev D) {
     for (int k = 0; k < 4; k++) B.q[k][s] = end[k];
 }
 
+// The body this thread takes among the workgroup's THREADS consecutive ones, boxes first and then balls, each in
+// body order (a stable partition by shape within the workgroup); n or more for a place past the last body.
+__device__ __forceinline__ int shape_first(const BoxDev &D) {
+    __shared__ int sBody[THREADS];
+    __shared__ int sBoxes[THREADS / 64];
+    int mine = blockIdx.x * THREADS + threadIdx.x;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    bool box = mine < D.n && !(D.shape[mine] > 0.5f);  // Moving::sphere
+    unsigned long long boxes = __ballot(box);
+    if (lane == 0) sBoxes[wave] = __popcll(boxes);
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < THREADS / 64; w++) {
+        before += w < wave ? sBoxes[w] : 0;
+        total += sBoxes[w];
+    }
+    int boxesBefore = before + __popcll(boxes & ((1ull << lane) - 1ull));
+    sBody[box ? boxesBefore : total + (threadIdx.x - boxesBefore)] = mine;
+    __syncthreads();
+    return sBody[threadIdx.x];
+}
+
 // v[a] for a run-time a, as a register select (an indexed small array would go to scratch memory).
 __device__ __forceinline__ float pick(const float *v, int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
 
 // Boxes.walls, Boxes.java:357-416: one sweep of a body's corners (a sphere's nearest point) against the six walls.
+// A body's sweep reads and writes only that body's state and wall impulses, so which thread takes which body changes
+// nothing in the result. A workgroup takes its THREADS consecutive bodies with the boxes first and then the balls
+// (shape_first), so its waves run the box path (eight corners a wall) or the ball path (one point a wall), not one
+// after the other, while the workgroup's loads still fall in the same cache lines as in body order.
+//
+// A box's corner does something only if it holds an impulse or, in a solve or relax sweep, is within the margin of
+// the wall: any other returns at once from Boxes.contact, or is passed over before it when its wall holds nothing.
+// Lanes of a wave differ in which of their 48 corners do something, and a wave that ran every corner some lane needs
+// would run most of them. So each lane first lists its corners that may do something, in the order the reference
+// takes them (a bit a corner: the holding ones exactly, from their impulses; the ones within the margin from a bound
+// that never leaves one out), and the wave then takes each lane's list one corner at a time, each lane its own next
+// corner, with the reference's code for a corner. The corners left out are ones the reference does nothing with, and
+// each body's corners run in its order, so every body ends the sweep as the reference leaves it.
 template <int SWEEP, bool REVERSE>
-__global__ void k_walls(BoxDev D) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(THREADS) k_walls(BoxDev D) {
+    int s = shape_first(D);
     if (s >= D.n || !(D.im[s] > 0)) return;
     Bufs B = bufs(D);
     const float *P = D.params;
@@ -91,16 +126,14 @@ __global__ void k_walls(BoxDev D) {
     // The wall impulses are slot-major: word w of body s at w * n + s, so a wave's loads of one word coalesce.
     size_t n = (size_t)D.n;
     const Sweep sweep = (Sweep)SWEEP;
-    int held = D.wallHeld[s], nowHeld = 0;  // walls at which some corner holds an impulse
-    for (int order = 0; order < WALLS; order++) {
-        int wall = REVERSE ? WALLS - 1 - order : order;
-        int axis = wall / 2;
-        bool high = wall % 2 == 1;
-        int firstSlot = wall * CORNERS;
-        float nrm[3];
-        for (int a = 0; a < 3; a++) nrm[a] = a == axis ? (high ? -1.0f : 1.0f) : 0.0f;
-        if (box.sphere()) {
-            float r[3];
+    if (box.sphere()) {
+        for (int order = 0; order < WALLS; order++) {
+            int wall = REVERSE ? WALLS - 1 - order : order;
+            int axis = wall / 2;
+            bool high = wall % 2 == 1;
+            int firstSlot = wall * CORNERS;
+            float nrm[3], r[3];
+            for (int a = 0; a < 3; a++) nrm[a] = a == axis ? (high ? -1.0f : 1.0f) : 0.0f;
             for (int a = 0; a < 3; a++) r[a] = -(box.half[0] * nrm[a]);
             size_t at = (size_t)firstSlot * n + s;
             size_t tangentAt = (size_t)firstSlot * 3 * n + s;
@@ -108,33 +141,86 @@ __global__ void k_walls(BoxDev D) {
             float gap = high ? pick(extent, axis) - reach : reach;
             contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft);
             roll(sweep, nullptr, box, box.half[0], B.lambda, at, B.roll, tangentAt, n, P);
-        } else {
-            // A corner that holds no impulse and is out of the margin's reach returns at once in Boxes.contact;
-            // a wall none of whose corners holds one (its bit clear in wallHeld) needs none of its loads for that.
-            bool live = (held >> wall) & 1;
-            if (sweep == WARM && !live) continue;
-            bool holds = false;
-            for (int k = 0; k < CORNERS; k++) {
-                int c = REVERSE ? CORNERS - 1 - k : k;
-                float offset[3], r[3];
-                for (int a = 0; a < 3; a++) {
-                    int bit = (c / (1 << a)) % 2;
-                    offset[a] = box.half[a] * (2.0f * (float)bit - 1.0f);
-                }
-                rotate(box.q, offset, r);
-                float reach = pick(box.at, axis) + pick(r, axis);
-                float gap = high ? pick(extent, axis) - reach : reach;
-                if (!live && !(gap < soft.margin)) continue;
-                size_t at = (size_t)(firstSlot + c) * n + s;
-                size_t tangentAt = (size_t)(firstSlot + c) * 3 * n + s;
-                holds = contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft) ||
-                        holds;
+        }
+        store_moving(B, box);
+        return;
+    }
+    int held = D.wallHeld[s], nowHeld = 0;  // walls at which some corner holds an impulse
+    float diagonal = sqrtf((box.half[0] * box.half[0] + box.half[1] * box.half[1]) + box.half[2] * box.half[2]);
+    // The list: bit order * CORNERS + k for the k-th corner taken at the order-th wall taken.
+    unsigned long long todo = 0;
+    for (int order = 0; order < WALLS; order++) {
+        int wall = REVERSE ? WALLS - 1 - order : order;
+        int axis = wall / 2;
+        bool high = wall % 2 == 1;
+        // A wall none of whose corners holds an impulse (its bit clear in wallHeld) is passed over in a warm sweep.
+        bool live = (held >> wall) & 1;
+        if (sweep == WARM && !live) continue;
+        // rotate turns by q / |q|, so a corner's offset along the wall's axis is row . offset for the axis's row of
+        // the turn, which is the axis turned back; `slack` (1e-4 of the sizes involved) covers the rounding of both
+        // ways of working it out and of the gap's two additions, which stays below 1e-6 of them. So a corner whose
+        // gap so bounded is at least the margin has its own gap at least the margin, and no corner within the margin
+        // is left out; NaN or infinite sizes leave every corner in.
+        float centre = pick(box.at, axis), side = pick(extent, axis);
+        float slack = 1e-4f * ((diagonal + fabsf(centre)) + fabsf(side));
+        // No corner is further than the half-diagonal from the centre along any axis: a wall holding nothing whose
+        // nearest corner could not be within the margin is passed over whole.
+        float clear = high ? side - centre : centre;
+        if (!live && (clear - diagonal) - slack >= soft.margin) continue;
+        float along[3] = {0.f, 0.f, 0.f}, row[3], cq[4];
+        along[0] = axis == 0 ? 1.f : 0.f;
+        along[1] = axis == 1 ? 1.f : 0.f;
+        along[2] = axis == 2 ? 1.f : 0.f;
+        conjugate(box.q, cq);
+        rotate(cq, along, row);
+        int firstSlot = wall * CORNERS;
+        for (int k = 0; k < CORNERS; k++) {
+            int c = REVERSE ? CORNERS - 1 - k : k;
+            float offset[3];
+            for (int a = 0; a < 3; a++) {
+                int bit = (c / (1 << a)) % 2;
+                offset[a] = box.half[a] * (2.0f * (float)bit - 1.0f);
             }
-            if (holds) nowHeld |= 1 << wall;
+            float rough = dot(row, offset);
+            float bound = high ? side - (centre + rough) : centre + rough;
+            bool near = sweep != WARM && !((bound - slack) >= soft.margin);
+            bool holding = false;
+            if (live) {  // Boxes.contact's own test, on the same words
+                size_t at = (size_t)(firstSlot + c) * n + s, tangentAt = (size_t)(firstSlot + c) * 3 * n + s;
+                float tangent[3] = {B.rub[tangentAt], B.rub[tangentAt + n], B.rub[tangentAt + 2 * n]};
+                holding = B.lambda[at] > 0 || dot(tangent, tangent) > 0;
+            }
+            if (holding || near) todo |= 1ull << (order * CORNERS + k);
         }
     }
+    while (__any(todo != 0)) {
+        if (todo == 0) continue;
+        int bit = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        int order = bit / CORNERS, k = bit % CORNERS;
+        int wall = REVERSE ? WALLS - 1 - order : order;
+        int c = REVERSE ? CORNERS - 1 - k : k;
+        int axis = wall / 2;
+        bool high = wall % 2 == 1;
+        bool live = (held >> wall) & 1;
+        float nrm[3], offset[3], r[3];
+        for (int a = 0; a < 3; a++) nrm[a] = a == axis ? (high ? -1.0f : 1.0f) : 0.0f;
+        for (int a = 0; a < 3; a++) {
+            int b = (c / (1 << a)) % 2;
+            offset[a] = box.half[a] * (2.0f * (float)b - 1.0f);
+        }
+        rotate(box.q, offset, r);
+        float reach = pick(box.at, axis) + pick(r, axis);
+        float gap = high ? pick(extent, axis) - reach : reach;
+        if (!live && !(gap < soft.margin)) continue;
+        int firstSlot = wall * CORNERS;
+        size_t at = (size_t)(firstSlot + c) * n + s;
+        size_t tangentAt = (size_t)(firstSlot + c) * 3 * n + s;
+        if (contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft))
+            nowHeld |= 1 << wall;
+    }
     store_moving(B, box);
-    if (!box.sphere() && sweep != WARM) D.wallHeld[s] = nowHeld;
+    if (sweep != WARM) D.wallHeld[s] = nowHeld;
 }
 
 // BoxPairs.solve, BoxPairs.java:64-151: the pairs of round r, one thread a place, each its manifold in turn.
