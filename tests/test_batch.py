@@ -71,3 +71,35 @@ def test_batch_refuses_bad_scenes():
     spec = small_batch()
     with pytest.raises(pkg.IllegalArgumentException):
         boxes.BoxBatch([3, 0], spec.extents[:2], spec.grids[:2], 2, abi=sc.oracle_abi())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("count", [96, 100])
+def test_many_scenes_match_the_oracle(count):
+    """A batch of many small scenes, which the HIP library steps a workgroup a scene through all of a step's
+    substeps rather than pass by pass over the whole chip, against the oracle's batch, bit for bit, step by step.
+    The 100-scene batch has scenes too big for that (up to 4906 bodies; the scene step takes at most 256) and steps
+    pass by pass, as a check that both ways match."""
+    if count == 96:
+        spec = scenes.batched_2k(seed=9, scenes=count, fewest=1, most=60, substeps=3, sweeps=3)
+    else:
+        spec = scenes.batched_2k(seed=17, scenes=count, fewest=1, most=5000, substeps=3, sweeps=3)
+        assert int(spec.bodies.max()) > 256
+    out = {}
+    for backend in ("oracle", "gpu"):
+        sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps,
+                             abi=sc.abi_for(backend))
+        start(sim, spec)
+        reps, states = [], []
+        for _ in range(8):
+            r = sim.stepAndWait()
+            reps.append((r.touching, r.points, r.rounds, r.unplaced, r.unlisted))
+            states.append({name: sim.floats(name).copy() for name in STATE})
+        out[backend] = (reps, states)
+        sim.close()
+    (ra, sa), (rb, sb) = out["oracle"], out["gpu"]
+    assert ra == rb
+    assert max(r[0] for r in ra) > 0, "the scenes should touch"
+    for t in range(len(sa)):
+        for name in STATE:
+            assert np.array_equal(sa[t][name].view(np.int32), sb[t][name].view(np.int32)), (t, name)

@@ -86,6 +86,10 @@ void launch_velocities(const BoxDev &D, hipStream_t s);
 void launch_scene_from_params(const BoxDev &D, SceneDev *scene, hipStream_t s);
 void launch_positions(const BoxDev &D, hipStream_t s);
 void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s);
+// Every substep of a step (velocities, the warm, solve and relax sweeps, positions), a workgroup a scene, for a batch
+// of scenes (k_scene_step). starts: the colouring's round starts (device); used: the step's rounds.
+void launch_scene_step(const BoxDev &D, const int32_t *sceneFirst, const int32_t *starts, int used, int substeps,
+                       int sweeps, bool symmetric, hipStream_t s);
 void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int r, int pairs, hipStream_t s);
 void launch_bin(const BoxDev &D, hipStream_t s);
 void launch_list(const BoxDev &D, hipStream_t s);

@@ -40,20 +40,23 @@ __device__ __forceinline__ Bufs bufs(const BoxDev &D) {
     return B;
 }
 
-// Boxes.velocities, Boxes.java:276-289.
-__global__ void k_velocities(BoxDev D) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= D.n || !(D.im[s] > 0)) return;
+// Boxes.velocities, Boxes.java:276-289, for body s.
+__device__ __forceinline__ void velocity_of(const BoxDev &D, int s) {
+    if (!(D.im[s] > 0)) return;
     const float *P = D.params;
     float h = P[H];
     float g[3] = {P[GX], P[GY], P[GZ]};
     for (int a = 0; a < 3; a++) D.vel[a][s] = D.vel[a][s] + h * g[a];
 }
 
-// Boxes.positions, Boxes.java:299-342: the midpoint turn, with the angular momentum as the state.
-__global__ void k_positions(BoxDev D) {
+__global__ void k_velocities(BoxDev D) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= D.n || !(D.im[s] > 0)) return;
+    if (s < D.n) velocity_of(D, s);
+}
+
+// Boxes.positions, Boxes.java:299-342, for body s: the midpoint turn, with the angular momentum as the state.
+__device__ __forceinline__ void position_of(const BoxDev &D, int s) {
+    if (!(D.im[s] > 0)) return;
     Bufs B = bufs(D);
     float h = D.params[H];
     for (int a = 0; a < 3; a++) B.x[a][s] = B.x[a][s] + h * B.vel[a][s];
@@ -72,6 +75,11 @@ __global__ void k_positions(BoxDev D) {
     rotate(mid, own, omegaMid);
     turned(m.q, omegaMid, h, end);
     for (int k = 0; k < 4; k++) B.q[k][s] = end[k];
+}
+
+__global__ void k_positions(BoxDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < D.n) position_of(D, s);
 }
 
 // The body this thread takes among the workgroup's THREADS consecutive ones, boxes first and then balls, each in
@@ -114,9 +122,8 @@ __device__ __forceinline__ float pick(const float *v, int a) { return a == 0 ? v
 // corner, with the reference's code for a corner. The corners left out are ones the reference does nothing with, and
 // each body's corners run in its order, so every body ends the sweep as the reference leaves it.
 template <int SWEEP, bool REVERSE>
-__global__ void __launch_bounds__(THREADS) k_walls(BoxDev D) {
-    int s = shape_first(D);
-    if (s >= D.n || !(D.im[s] > 0)) return;
+__device__ void walls_of(const BoxDev &D, int s) {
+    if (!(D.im[s] > 0)) return;
     Bufs B = bufs(D);
     const float *P = D.params;
     const SceneDev sc = D.sceneInfo[D.scene[s]];
@@ -223,12 +230,15 @@ __global__ void __launch_bounds__(THREADS) k_walls(BoxDev D) {
     if (sweep != WARM) D.wallHeld[s] = nowHeld;
 }
 
-// BoxPairs.solve, BoxPairs.java:64-151: the pairs of round r, one thread a place, each its manifold in turn.
 template <int SWEEP, bool REVERSE>
-__global__ void k_solve_round(BoxDev D, int r) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= D.width) return;
-    int pair = D.rounds[(size_t)r * D.width + k];
+__global__ void __launch_bounds__(THREADS) k_walls(BoxDev D) {
+    int s = shape_first(D);
+    if (s < D.n) walls_of<SWEEP, REVERSE>(D, s);
+}
+
+// BoxPairs.solve, BoxPairs.java:64-151: one pair of a round (a slot, or -1 for none), its manifold's points in turn.
+template <int SWEEP, bool REVERSE>
+__device__ void solve_pair(const BoxDev &D, int pair) {
     if (pair < 0) return;
     size_t out = (size_t)pair * MANIFOLD_WORDS;
     int count = (int)D.manifold[out + COUNT];
@@ -269,6 +279,87 @@ __global__ void k_solve_round(BoxDev D, int r) {
     }
     store_moving(B, first);
     store_moving(B, second);
+}
+
+// The pairs of round r, one thread a place.
+template <int SWEEP, bool REVERSE>
+__global__ void k_solve_round(BoxDev D, int r) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < D.width) solve_pair<SWEEP, REVERSE>(D, D.rounds[(size_t)r * D.width + k]);
+}
+
+// ---- a whole step's substeps, a workgroup a scene ----
+//
+// The scenes of a batch share no body and no pair, so one scene's sweeps never wait for another's. One workgroup
+// takes one scene through every substep of the step: velocities, the warm sweep, the solve sweeps, positions, the
+// relax sweeps (BoxStep.java:199-231), each a pass over the scene's bodies or one of its colour rounds, with a
+// workgroup barrier between passes where the launches over the whole batch had a kernel boundary. A scene's bodies
+// and pairs are met in the same order, by the same code, as in those launches, so each scene ends the step as they
+// leave it; what changes is that a round or a wall sweep of a few hundred bodies no longer costs a launch over the
+// whole chip, and that the batch's scenes run their rounds side by side.
+constexpr int SCENE_THREADS = 128;
+
+// lower bound in round r's list (places 0 .. count) of the first pair whose scene is at least `scene`; a round lists
+// its pairs in slot order, so by body and so by scene.
+__device__ int first_pair_of(const BoxDev &D, const int32_t *list, int count, int scene) {
+    int lo = 0, hi = count;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (D.scene[list[mid] / SLOTS] < scene) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int SWEEP, bool REVERSE>
+__device__ void scene_sweep(const BoxDev &D, int b0, int b1, const int *from, const int *to, int used) {
+    if (REVERSE) {
+        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) walls_of<SWEEP, REVERSE>(D, s);
+        __syncthreads();
+    }
+    for (int order = 0; order < used; order++) {  // BoxStep.sweep, BoxStep.java:217-231
+        int r = REVERSE ? used - 1 - order : order;
+        if (from[r] == to[r]) continue;  // no pair of this scene in round r: nothing to wait for
+        const int32_t *list = D.rounds + (size_t)r * D.width;
+        for (int k = from[r] + (int)threadIdx.x; k < to[r]; k += SCENE_THREADS) solve_pair<SWEEP, REVERSE>(D, list[k]);
+        __syncthreads();
+    }
+    if (!REVERSE) {
+        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) walls_of<SWEEP, REVERSE>(D, s);
+        __syncthreads();
+    }
+}
+
+// sceneFirst: scenes + 1 words, scene j's bodies sceneFirst[j] .. sceneFirst[j + 1]. starts: the colouring's round
+// starts (round r has starts[r + 1] - starts[r] pairs). used: the rounds the step has.
+__global__ void __launch_bounds__(SCENE_THREADS) k_scene_step(BoxDev D, const int32_t *sceneFirst,
+                                                              const int32_t *starts, int used, int substeps,
+                                                              int sweeps, int symmetric) {
+    __shared__ int from[MOST_ROUNDS], to[MOST_ROUNDS];
+    int j = blockIdx.x;
+    int b0 = sceneFirst[j], b1 = sceneFirst[j + 1];
+    for (int r = threadIdx.x; r < used; r += SCENE_THREADS) {
+        const int32_t *list = D.rounds + (size_t)r * D.width;
+        int count = starts[r + 1] - starts[r];
+        from[r] = first_pair_of(D, list, count, j);
+        to[r] = first_pair_of(D, list, count, j + 1);
+    }
+    __syncthreads();
+    for (int sub = 0; sub < substeps; sub++) {
+        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) velocity_of(D, s);
+        __syncthreads();
+        scene_sweep<WARM, false>(D, b0, b1, from, to, used);
+        for (int k = 0; k < sweeps; k++) {
+            if (symmetric && k % 2 == 1) scene_sweep<SOLVE, true>(D, b0, b1, from, to, used);
+            else scene_sweep<SOLVE, false>(D, b0, b1, from, to, used);
+        }
+        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) position_of(D, s);
+        __syncthreads();
+        for (int k = 0; k < sweeps; k++) {
+            if (symmetric && k % 2 == 1) scene_sweep<RELAX, true>(D, b0, b1, from, to, used);
+            else scene_sweep<RELAX, false>(D, b0, b1, from, to, used);
+        }
+    }
 }
 
 // BoxBroad.bin, BoxBroad.java:74-89, in the body's scene's grid.
@@ -698,6 +789,14 @@ void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s) {
         default: walls_then<RELAX, true>(D, s); break;
     }
     check_launch("walls");
+}
+
+void launch_scene_step(const BoxDev &D, const int32_t *sceneFirst, const int32_t *starts, int used, int substeps,
+                       int sweeps, bool symmetric, hipStream_t s) {
+    if (D.scenes <= 0) return;
+    if (used < 0 || used > MOST_ROUNDS) throw std::runtime_error("scene step: rounds out of range");
+    k_scene_step<<<D.scenes, SCENE_THREADS, 0, s>>>(D, sceneFirst, starts, used, substeps, sweeps, symmetric ? 1 : 0);
+    check_launch("scene step");
 }
 
 void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int r, int pairs, hipStream_t s) {

@@ -48,6 +48,13 @@ struct BoxWorld {
     std::vector<SceneDev> scenes;      // walls and grid a scene (cellBase worked out in build)
     bool wallsFromParams = true;       // one scene, walled by the params' box (the reference's box world)
     SceneDev *sceneDev = nullptr;
+    int32_t *sceneFirst = nullptr;  // scenes + 1: each scene's first body
+    bool sceneStep = false;         // substeps a workgroup a scene (k_scene_step)
+    static constexpr size_t SCENE_STEP_LEAST = 64;  // scenes, for the scene step to fill the chip
+    // Bodies in the largest scene, for one workgroup to keep up: a scene's sweep is a chain of dependent steps in
+    // one workgroup, which beats launches over the whole chip for batches of small scenes (2,048 scenes of 16-64
+    // bodies: 19 against 32 ms a step) but not once a scene holds thousands (16-4,096 bodies: 207 against 97).
+    static constexpr int SCENE_STEP_MOST = 256;
     Colouring C{};
     // The learned warm start (no counterpart in the reference): per-point data, a log of (features, targets), and
     // the network whose prediction starts each step's impulses, when one is set.
@@ -106,6 +113,17 @@ struct BoxWorld {
         std::vector<int32_t> sceneOf(n);
         for (int k = 0, at = 0; k < (int)sceneBodies.size(); k++)
             for (int b = 0; b < sceneBodies[k]; b++) sceneOf[at++] = k;
+        // A batch of many small scenes steps a workgroup a scene (k_scene_step); one world, or a few big scenes,
+        // step pass by pass over the whole chip.
+        std::vector<int32_t> first(sceneBodies.size() + 1, 0);
+        int most = 0;
+        for (size_t k = 0; k < sceneBodies.size(); k++) {
+            first[k + 1] = first[k] + sceneBodies[k];
+            most = std::max(most, sceneBodies[k]);
+        }
+        sceneStep = !wallsFromParams && sceneBodies.size() >= SCENE_STEP_LEAST && most <= SCENE_STEP_MOST;
+        sceneFirst = alloc<int32_t>((int64_t)first.size());
+        HIP_OK(hipMemcpy(sceneFirst, first.data(), sizeof(int32_t) * first.size(), hipMemcpyHostToDevice));
         int32_t *sceneIdx = alloc<int32_t>(n);
         sceneDev = alloc<SceneDev>((int64_t)scenes.size());
         HIP_OK(hipMemcpy(sceneIdx, sceneOf.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
@@ -360,12 +378,16 @@ struct BoxWorld {
                 }
             }
         }
-        for (int sub = 0; sub < d.substeps; sub++) {
-            launch_velocities(D, stream);
-            sweep(WARM, false, used);
-            for (int k = 0; k < d.sweeps; k++) sweep(SOLVE, d.symmetric && k % 2 == 1, used);
-            launch_positions(D, stream);
-            for (int k = 0; k < d.sweeps; k++) sweep(RELAX, d.symmetric && k % 2 == 1, used);
+        if (sceneStep) {
+            launch_scene_step(D, sceneFirst, C.starts, used, d.substeps, d.sweeps, d.symmetric != 0, stream);
+        } else {
+            for (int sub = 0; sub < d.substeps; sub++) {
+                launch_velocities(D, stream);
+                sweep(WARM, false, used);
+                for (int k = 0; k < d.sweeps; k++) sweep(SOLVE, d.symmetric && k % 2 == 1, used);
+                launch_positions(D, stream);
+                for (int k = 0; k < d.sweeps; k++) sweep(RELAX, d.symmetric && k % 2 == 1, used);
+            }
         }
         if (logging && grid && touching > 0) {
             launch_point_targets(D, C, L, touching, stream);  // what the next step's carry would start from
