@@ -34,6 +34,11 @@ from __graft_entry__ import load_package  # noqa: E402
 
 METRIC = "body-substeps/s at equal residual: plain vs batched+learned warm-start"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes a round-kernel launch, measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE passes, each scaled by what it
+# reads for a copy of known size) over the first 100 steps of pile_64k: profiles/traffic_round_kernels_pile_64k.md.
+ROUND_TRAFFIC_BYTES = 5904187.0
+ROUND_TRAFFIC_SAMPLE = ("rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 (copy-calibrated) over the first 100 steps "
+                        "of pile_64k, all round-kernel launches; 892,506 algorithmic B a launch over the same steps")
 
 
 def cpu_baseline(pkg, spec, state, seconds):
@@ -251,7 +256,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": ROUND_TRAFFIC_BYTES,
+            "traffic_sample": ROUND_TRAFFIC_SAMPLE,
             "bytes_per_launch": per_launch_bytes,
             "avg_launch_us": prof.round_ns / 1e3 / max(prof.round_launches, 1),
             "avg_pass_us": prof.round_ns / 1e3 / passes,
