@@ -237,29 +237,53 @@ __global__ void __launch_bounds__(THREADS) k_walls(BoxDev D) {
 }
 
 // BoxPairs.solve, BoxPairs.java:64-151: one pair of a round (a slot, or -1 for none), its manifold's points in turn.
+// Only this pair's thread touches the pair's manifold and impulse words in a round, so every load the pair needs
+// (its manifold's count, normal and four points' anchors, the points' impulse words, its bodies) is issued first and
+// the points then run from registers; the words a point writes are stored at the end. The arithmetic and its order
+// are those of the reference, point by point.
+// One manifold point of a pair as solve_pair holds it: its anchors and its impulse words (normal, tangent 3,
+// rolling 3), and whether the sweep wrote them.
+struct HeldPoint {
+    float anchorA[3], anchorB[3], imp[POINT_IMPULSE];
+    bool wroteContact, wroteRoll;
+};
+
 template <int SWEEP, bool REVERSE>
 __device__ void solve_pair(const BoxDev &D, int pair) {
     if (pair < 0) return;
     size_t out = (size_t)pair * MANIFOLD_WORDS;
+    size_t held = (size_t)pair * IMPULSE_WORDS;
     int count = (int)D.manifold[out + COUNT];
+    float nrm[3] = {D.manifold[out], D.manifold[out + 1], D.manifold[out + 2]};
+    // Four points, each a variable of its own (not an array indexed by the point), so they stay in registers.
+    HeldPoint p0, p1, p2, p3;
+    auto load = [&](HeldPoint &p, int slot) {
+        size_t where = out + POINTS + slot * POINT_WORDS;
+        for (int a = 0; a < 3; a++) {
+            p.anchorA[a] = D.manifold[where + ANCHOR_FIRST + a];
+            p.anchorB[a] = D.manifold[where + ANCHOR_SECOND + a];
+        }
+        for (int w = 0; w < POINT_IMPULSE; w++) p.imp[w] = D.impulses[held + POINT_IMPULSE * slot + w];
+        p.wroteContact = p.wroteRoll = false;
+    };
+    load(p0, 0);
+    load(p1, 1);
+    load(p2, 2);
+    load(p3, 3);
+    int ia = D.pairs[2 * (size_t)pair], ib = D.pairs[2 * (size_t)pair + 1];
     if (!(count > 0)) return;
     Bufs B = bufs(D);
     const float *P = D.params;
     const Sweep sweep = (Sweep)SWEEP;
-    int ia = D.pairs[2 * (size_t)pair], ib = D.pairs[2 * (size_t)pair + 1];
     float imA = B.im[ia], imB = B.im[ib];
     Moving first = load_moving(B, ia), second = load_moving(B, ib);
     Soft soft(P, false);
     if (imA == 0 || imB == 0) soft.stiffen(P);
-    float nrm[3] = {D.manifold[out], D.manifold[out + 1], D.manifold[out + 2]};
-    size_t held = (size_t)pair * IMPULSE_WORDS;
-    for (int order = 0; order < MOST_POINTS; order++) {
-        int slot = REVERSE ? MOST_POINTS - 1 - order : order;
-        if (!(slot < count)) continue;
-        size_t where = out + POINTS + slot * POINT_WORDS;
+    auto point = [&](HeldPoint &p, int slot) {
+        if (!(slot < count)) return;
         float rA[3], rB[3];
-        rotate(first.q, D.manifold + where + ANCHOR_FIRST, rA);
-        rotate(second.q, D.manifold + where + ANCHOR_SECOND, rB);
+        rotate(first.q, p.anchorA, rA);
+        rotate(second.q, p.anchorB, rB);
         if (first.sphere())
             for (int a = 0; a < 3; a++) rA[a] = first.half[0] * nrm[a];
         if (second.sphere())
@@ -267,23 +291,44 @@ __device__ void solve_pair(const BoxDev &D, int pair) {
         float gap = ((second.at[0] + rB[0]) - (first.at[0] + rA[0])) * nrm[0] +
                     ((second.at[1] + rB[1]) - (first.at[1] + rA[1])) * nrm[1];
         gap = gap + ((second.at[2] + rB[2]) - (first.at[2] + rA[2])) * nrm[2];
-        size_t normalAt = held + POINT_IMPULSE * slot;
-        contact(sweep, gap, nrm, &first, rA, second, rB, D.impulses, normalAt, D.impulses, normalAt + 1, 1, soft);
+        contact_held(sweep, gap, nrm, &first, rA, second, rB, p.imp[0], &p.imp[1], p.wroteContact, soft);
         if (first.sphere() || second.sphere()) {
             float radius = second.half[0];
             if (first.sphere()) radius = first.half[0];
             if (first.sphere() && second.sphere())
                 radius = (first.half[0] * second.half[0]) / (first.half[0] + second.half[0]);
-            roll(sweep, &first, second, radius, D.impulses, normalAt, D.impulses, normalAt + 4, 1, P);
+            roll_held(sweep, &first, second, radius, p.imp[0], &p.imp[4], p.wroteRoll, P);
         }
+    };
+    if (REVERSE) {
+        point(p3, 3);
+        point(p2, 2);
+        point(p1, 1);
+        point(p0, 0);
+    } else {
+        point(p0, 0);
+        point(p1, 1);
+        point(p2, 2);
+        point(p3, 3);
     }
+    auto store = [&](const HeldPoint &p, int slot) {
+        size_t at = held + POINT_IMPULSE * slot;
+        if (p.wroteContact)
+            for (int w = 0; w < 4; w++) D.impulses[at + w] = p.imp[w];
+        if (p.wroteRoll)
+            for (int w = 4; w < POINT_IMPULSE; w++) D.impulses[at + w] = p.imp[w];
+    };
+    store(p0, 0);
+    store(p1, 1);
+    store(p2, 2);
+    store(p3, 3);
     store_moving(B, first);
     store_moving(B, second);
 }
 
 // The pairs of round r, one thread a place.
 template <int SWEEP, bool REVERSE>
-__global__ void k_solve_round(BoxDev D, int r) {
+__global__ void __launch_bounds__(THREADS) k_solve_round(BoxDev D, int r) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < D.width) solve_pair<SWEEP, REVERSE>(D, D.rounds[(size_t)r * D.width + k]);
 }

@@ -224,13 +224,14 @@ __device__ float mass_along(const Moving *first, const float *rA, const Moving &
     return first ? k + first->inverseMassAlong(rA, n) : k;
 }
 
-// Boxes.contact, Boxes.java:462-542: one point, one sweep.
-// The tangent impulse's three words are `stride` apart (1 for a pair's point; the body count for a wall's, whose
-// impulses are laid out word by word over the bodies). Returns whether the point holds an impulse afterwards.
-__device__ bool contact(Sweep sweep, float gap, const float *n, Moving *first, const float *rA, Moving &second, const float *rB,
-             float *normals, size_t normalAt, float *tangents, size_t tangentAt, size_t stride, const Soft &soft) {
-    float normal = normals[normalAt];
-    float tangent[3] = {tangents[tangentAt], tangents[tangentAt + stride], tangents[tangentAt + 2 * stride]};
+// Boxes.contact, Boxes.java:462-542: one point, one sweep, on the point's impulse held in registers: `normal` and
+// `tangent` come in as the point holds them and leave as Boxes.contact leaves them; `wrote` says whether it writes
+// them (not in a warm sweep, nor for a point it passes over). Returns whether the point holds an impulse afterwards.
+__device__ bool contact_held(Sweep sweep, float gap, const float *n, Moving *first, const float *rA, Moving &second,
+                             const float *rB, float &normalHeld, float *tangentHeld, bool &wrote, const Soft &soft) {
+    wrote = false;
+    float normal = normalHeld;
+    float tangent[3] = {tangentHeld[0], tangentHeld[1], tangentHeld[2]};
     bool holding = normal > 0 || dot(tangent, tangent) > 0;
     if (sweep == WARM) {
         if (holding) {
@@ -254,7 +255,6 @@ __device__ bool contact(Sweep sweep, float gap, const float *n, Moving *first, c
     }
     float change = -((massScale / k) * (closing + bias)) - impulseScale * normal;
     float total = mx(0.f, normal + change);
-    normals[normalAt] = total;
     float p[3];
     for (int a = 0; a < 3; a++) p[a] = (total - normal) * n[a];
     apply_pair(first, rA, second, rB, p);
@@ -278,12 +278,27 @@ __device__ bool contact(Sweep sweep, float gap, const float *n, Moving *first, c
         for (int a = 0; a < 3; a++) whole[a] = whole[a] * cut;
     }
     float rubbed[3];
-    for (int a = 0; a < 3; a++) {
-        tangents[tangentAt + a * stride] = whole[a];
-        rubbed[a] = whole[a] - tangent[a];
-    }
+    for (int a = 0; a < 3; a++) rubbed[a] = whole[a] - tangent[a];
     apply_pair(first, rA, second, rB, rubbed);
+    normalHeld = total;
+    for (int a = 0; a < 3; a++) tangentHeld[a] = whole[a];
+    wrote = true;
     return total > 0 || dot(whole, whole) > 0;
+}
+
+// Boxes.contact on the point's words in memory. The tangent impulse's three words are `stride` apart (1 for a pair's
+// point; the body count for a wall's, whose impulses are laid out word by word over the bodies).
+__device__ bool contact(Sweep sweep, float gap, const float *n, Moving *first, const float *rA, Moving &second, const float *rB,
+             float *normals, size_t normalAt, float *tangents, size_t tangentAt, size_t stride, const Soft &soft) {
+    float normal = normals[normalAt];
+    float tangent[3] = {tangents[tangentAt], tangents[tangentAt + stride], tangents[tangentAt + 2 * stride]};
+    bool wrote;
+    bool holds = contact_held(sweep, gap, n, first, rA, second, rB, normal, tangent, wrote, soft);
+    if (wrote) {
+        normals[normalAt] = normal;
+        for (int a = 0; a < 3; a++) tangents[tangentAt + a * stride] = tangent[a];
+    }
+    return holds;
 }
 
 __device__ void apply_angular(Moving *first, Moving &second, const float *j) {  // Boxes.java:603-612
@@ -294,10 +309,12 @@ __device__ void apply_angular(Moving *first, Moving &second, const float *j) {  
     }
 }
 
-// Boxes.roll, Boxes.java:553-600.
-__device__ void roll(Sweep sweep, Moving *first, Moving &second, float radius, const float *normals, size_t normalAt,
-          float *rolls, size_t rollAt, size_t stride, const float *P) {
-    float old[3] = {rolls[rollAt], rolls[rollAt + stride], rolls[rollAt + 2 * stride]};
+// Boxes.roll, Boxes.java:553-600, on the point's rolling impulse held in registers (`held`, in and out) and its
+// normal impulse as the point's contact left it; `wrote` says whether Boxes.roll writes the rolling impulse.
+__device__ void roll_held(Sweep sweep, Moving *first, Moving &second, float radius, float normal, float *held,
+                          bool &wrote, const float *P) {
+    wrote = false;
+    float old[3] = {held[0], held[1], held[2]};
     if (sweep == WARM) {
         apply_angular(first, second, old);
         return;
@@ -323,7 +340,7 @@ __device__ void roll(Sweep sweep, Moving *first, Moving &second, float radius, c
         float stop = speed / k;
         for (int a = 0; a < 3; a++) whole[a] = whole[a] - stop * t[a];
     }
-    float most = (muRolling * radius) * mx(0.f, normals[normalAt]);
+    float most = (muRolling * radius) * mx(0.f, normal);
     float size = sqrtf(dot(whole, whole));
     if (size > most) {
         float scale = most / size;
@@ -331,10 +348,21 @@ __device__ void roll(Sweep sweep, Moving *first, Moving &second, float radius, c
     }
     float change[3];
     for (int a = 0; a < 3; a++) {
-        rolls[rollAt + a * stride] = whole[a];
+        held[a] = whole[a];
         change[a] = whole[a] - old[a];
     }
+    wrote = true;
     apply_angular(first, second, change);
+}
+
+// Boxes.roll on the point's words in memory (the rolling impulse's three words `stride` apart).
+__device__ void roll(Sweep sweep, Moving *first, Moving &second, float radius, const float *normals, size_t normalAt,
+          float *rolls, size_t rollAt, size_t stride, const float *P) {
+    float held[3] = {rolls[rollAt], rolls[rollAt + stride], rolls[rollAt + 2 * stride]};
+    bool wrote;
+    roll_held(sweep, first, second, radius, normals[normalAt], held, wrote, P);
+    if (wrote)
+        for (int a = 0; a < 3; a++) rolls[rollAt + a * stride] = held[a];
 }
 
 // ---- the narrow phase (BoxContact.java) ----
