@@ -71,12 +71,28 @@ def cpu_baseline(pkg, spec, state, seconds):
 
 def batched(pkg, steps, warmup, epochs):
     """The 2,048-scene batch on the box path: plain (the reference's carry as the warm start), then with the learned
-    warm start, trained on the device on samples logged from the plain steps. Body-substeps per second for both."""
+    warm start, trained on the device on samples logged from the plain steps. Body-substeps per second for both,
+    timed over the same steps from the same state: two batches step alike (the stepping is deterministic) through the
+    warm-up and the logged steps; one then runs plain, the other with the network."""
     scenes = __import__(pkg.__name__ + ".scenes", fromlist=["scenes"])
     boxes = __import__(pkg.__name__ + ".boxes", fromlist=["boxes"])
     mlp = __import__(pkg.__name__ + ".mlp", fromlist=["mlp"])
     spec = scenes.batched_2k()
     n = int(spec.bodies.sum())
+
+    def started():
+        sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps)
+        sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape,
+                  spec.im, spec.params)
+        for _ in range(warmup + 3):
+            sim.stepAndWait()
+        return sim
+
+    def timed(sim):
+        t0 = time.perf_counter()
+        reps = [sim.stepAndWait() for _ in range(steps)]
+        return time.perf_counter() - t0, reps
+
     sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps)
     sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
               spec.params)
@@ -87,13 +103,9 @@ def batched(pkg, steps, warmup, epochs):
         sim.stepAndWait()
     x, t = sim.samples(4 << 20)
     sim.log(False)
-
-    def timed():
-        t0 = time.perf_counter()
-        reps = [sim.stepAndWait() for _ in range(steps)]
-        return time.perf_counter() - t0, reps
-
-    plain_s, reps = timed()
+    plain_s, reps = timed(sim)
+    plain_end = [sim.floats(k).copy() for k in ("x", "y", "z")]
+    sim.close()
     net = mlp.Mlp(seed=3)
     held = np.random.default_rng(0).permutation(len(x))
     cut = int(0.9 * len(x))
@@ -103,9 +115,12 @@ def batched(pkg, steps, warmup, epochs):
     pred = net.forward(x[held[cut:]])
     err_net = float(np.mean(np.sum((pred - t[held[cut:]]) ** 2, axis=1)))
     err_carry = float(np.mean(np.sum((x[held[cut:]][:, 7:10] - t[held[cut:]]) ** 2, axis=1)))
+    sim = started()
     sim.learned(net)
-    learned_s, lreps = timed()
+    learned_s, lreps = timed(sim)
     used = sim.learnedPoints
+    # How far the learned start moved the batch from the plain one over the same steps.
+    apart = max(float(np.max(np.abs(sim.floats(k) - plain_end[a]))) for a, k in enumerate(("x", "y", "z")))
     sim.close()
     net.close()
     rate = lambda secs: n * spec.substeps * steps / secs  # noqa: E731
@@ -113,11 +128,14 @@ def batched(pkg, steps, warmup, epochs):
         "workload": f"batched_2k: {len(spec.bodies)} scenes, {n} bodies (half boxes, half balls), {spec.substeps} "
                     f"substeps of {spec.sweeps} sweeps, seed {spec.seed}",
         "steps_timed": steps, "warmup": warmup,
+        "timed_from": f"step {warmup + 3}, the same state for both",
         "plain": {"value": rate(plain_s), "ms_per_step": plain_s * 1e3 / steps,
                   "touching_at_end": reps[-1].touching, "rounds_at_end": reps[-1].rounds},
         "learned": {"value": rate(learned_s), "ms_per_step": learned_s * 1e3 / steps,
+                    "touching_at_end": lreps[-1].touching, "rounds_at_end": lreps[-1].rounds,
                     "points_started_from_prediction": int(used), "samples": int(len(x)), "epochs": epochs,
-                    "train_s": train_s, "held_out_sq_error": {"network": err_net, "carried_impulse": err_carry}},
+                    "train_s": train_s, "held_out_sq_error": {"network": err_net, "carried_impulse": err_carry},
+                    "largest_position_difference_from_plain_m": apart},
         "unit": "body-substeps/s",
     }
 
