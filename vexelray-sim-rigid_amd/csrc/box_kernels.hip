@@ -326,11 +326,261 @@ __device__ void solve_pair(const BoxDev &D, int pair) {
     store_moving(B, second);
 }
 
-// The pairs of round r, one thread a place.
+// ---- a pair solved by two lanes: the even lane holds the pair's first body, its neighbour the second ----
+//
+// About half of a point's work is per body (the anchor turned, inertia products, the point's velocity, an impulse
+// applied), and the reference does it for one body and then the other. Two neighbouring lanes do it at once, each for
+// its own body, and pass each other the few values the other needs (a mass term, a velocity, a position, a spin).
+// Every value both bodies feed is formed in the reference's order from the two, so each lane's arithmetic, and every
+// bit of the result, is the reference's; the two lanes take the same branches, since the values that decide them
+// are the same in both.
+struct Side {
+    Moving me;
+    bool second;  // this lane holds the pair's second body
+};
+
+// The first body's and the second's value of a per-body quantity, from this lane's and its neighbour's.
+__device__ __forceinline__ void both(const Side &S, float mine, float &ofFirst, float &ofSecond) {
+    float theirs = __shfl_xor(mine, 1, 64);
+    ofFirst = S.second ? theirs : mine;
+    ofSecond = S.second ? mine : theirs;
+}
+
+// apply_pair: p on the second at rB, -p on the first at rA, each lane on its own body.
+__device__ __forceinline__ void apply_side(Side &S, const float *r, const float *p) {
+    if (S.second) {
+        S.me.apply(r, p);
+    } else {
+        float back[3] = {-p[0], -p[1], -p[2]};
+        S.me.apply(r, back);
+    }
+}
+
+// mass_along: the second's inverse mass along n, then the first's added.
+__device__ __forceinline__ float mass_along_side(const Side &S, const float *r, const float *n) {
+    float f, s;
+    both(S, S.me.inverseMassAlong(r, n), f, s);
+    return s + f;
+}
+
+// relative: the second's point velocity less the first's.
+__device__ __forceinline__ void relative_side(const Side &S, const float *r, float *out) {
+    float v[3];
+    S.me.pointVelocity(r, v);
+    for (int a = 0; a < 3; a++) {
+        float f, s;
+        both(S, v[a], f, s);
+        out[a] = s - f;
+    }
+}
+
+// apply_angular: j turns the second, -j the first.
+__device__ __forceinline__ void apply_angular_side(Side &S, const float *j) {
+    if (S.second) {
+        S.me.turn(j);
+    } else {
+        float back[3] = {-j[0], -j[1], -j[2]};
+        S.me.turn(back);
+    }
+}
+
+// contact_held (Boxes.contact, Boxes.java:462-542) for the pair's point, with r this lane's body's arm.
+__device__ bool contact_side(Sweep sweep, float gap, const float *n, Side &S, const float *r, float &normalHeld,
+                             float *tangentHeld, bool &wrote, const Soft &soft) {
+    wrote = false;
+    float normal = normalHeld;
+    float tangent[3] = {tangentHeld[0], tangentHeld[1], tangentHeld[2]};
+    bool holding = normal > 0 || dot(tangent, tangent) > 0;
+    if (sweep == WARM) {
+        if (holding) {
+            float p[3];
+            for (int a = 0; a < 3; a++) p[a] = normal * n[a] + tangent[a];
+            apply_side(S, r, p);
+        }
+        return holding;
+    }
+    if (!(gap < soft.margin || holding)) return false;
+    float k = mass_along_side(S, r, n);
+    float rel[3];
+    relative_side(S, r, rel);
+    float closing = dot(rel, n);
+    float bias = 0, massScale = 1, impulseScale = 0;
+    if (gap > 0) bias = gap / soft.h;
+    if (sweep == SOLVE && !(gap > 0)) {
+        bias = mx(soft.bias * gap, -soft.push);
+        massScale = soft.massScale;
+        impulseScale = soft.impulseScale;
+    }
+    float change = -((massScale / k) * (closing + bias)) - impulseScale * normal;
+    float total = mx(0.f, normal + change);
+    float p[3];
+    for (int a = 0; a < 3; a++) p[a] = (total - normal) * n[a];
+    apply_side(S, r, p);
+
+    float sliding[3];
+    relative_side(S, r, sliding);
+    float along = dot(sliding, n);
+    for (int a = 0; a < 3; a++) sliding[a] = sliding[a] - along * n[a];
+    float whole[3] = {tangent[0], tangent[1], tangent[2]};
+    float speed = sqrtf(dot(sliding, sliding));
+    if (speed > 1e-12f) {
+        float t[3];
+        for (int a = 0; a < 3; a++) t[a] = sliding[a] / speed;
+        float kt = mass_along_side(S, r, t);
+        float stop = speed / kt;
+        for (int a = 0; a < 3; a++) whole[a] = whole[a] - stop * t[a];
+    }
+    float size = sqrtf(dot(whole, whole));
+    if (size > soft.muStatic * total) {
+        float cut = (soft.muKinetic * total) / size;
+        for (int a = 0; a < 3; a++) whole[a] = whole[a] * cut;
+    }
+    float rubbed[3];
+    for (int a = 0; a < 3; a++) rubbed[a] = whole[a] - tangent[a];
+    apply_side(S, r, rubbed);
+    normalHeld = total;
+    for (int a = 0; a < 3; a++) tangentHeld[a] = whole[a];
+    wrote = true;
+    return total > 0 || dot(whole, whole) > 0;
+}
+
+// roll_held (Boxes.roll, Boxes.java:553-600) for the pair's point.
+__device__ void roll_side(Sweep sweep, Side &S, float radius, float normal, float *held, bool &wrote,
+                          const float *P) {
+    wrote = false;
+    float old[3] = {held[0], held[1], held[2]};
+    if (sweep == WARM) {
+        apply_angular_side(S, old);
+        return;
+    }
+    float muRolling = P[MU_ROLLING];
+    float whole[3], turning[3];
+    for (int a = 0; a < 3; a++) {
+        whole[a] = old[a];
+        float f, s;
+        both(S, S.me.omega[a], f, s);
+        turning[a] = s - f;
+    }
+    float speed = sqrtf(dot(turning, turning));
+    if (speed > 1e-12f && muRolling > 0) {
+        float t[3], it[3];
+        for (int a = 0; a < 3; a++) t[a] = turning[a] / speed;
+        S.me.inverseInertiaTimes(t, it);
+        float f, s;
+        both(S, dot(t, it), f, s);
+        float k = s;
+        k = k + f;
+        float stop = speed / k;
+        for (int a = 0; a < 3; a++) whole[a] = whole[a] - stop * t[a];
+    }
+    float most = (muRolling * radius) * mx(0.f, normal);
+    float size = sqrtf(dot(whole, whole));
+    if (size > most) {
+        float scale = most / size;
+        for (int a = 0; a < 3; a++) whole[a] = whole[a] * scale;
+    }
+    float change[3];
+    for (int a = 0; a < 3; a++) {
+        held[a] = whole[a];
+        change[a] = whole[a] - old[a];
+    }
+    wrote = true;
+    apply_angular_side(S, change);
+}
+
+// solve_pair on two lanes: `second` tells which of the pair's bodies this lane holds. Both lanes of a pair call it
+// together with the same pair.
+template <int SWEEP, bool REVERSE>
+__device__ void solve_pair_side(const BoxDev &D, int pair, bool second) {
+    if (pair < 0) return;
+    size_t out = (size_t)pair * MANIFOLD_WORDS;
+    size_t held = (size_t)pair * IMPULSE_WORDS;
+    int count = (int)D.manifold[out + COUNT];
+    float nrm[3] = {D.manifold[out], D.manifold[out + 1], D.manifold[out + 2]};
+    HeldPoint p0, p1, p2, p3;
+    auto load = [&](HeldPoint &p, int slot) {
+        size_t where = out + POINTS + slot * POINT_WORDS;
+        for (int a = 0; a < 3; a++) {
+            p.anchorA[a] = D.manifold[where + ANCHOR_FIRST + a];
+            p.anchorB[a] = D.manifold[where + ANCHOR_SECOND + a];
+        }
+        for (int w = 0; w < POINT_IMPULSE; w++) p.imp[w] = D.impulses[held + POINT_IMPULSE * slot + w];
+        p.wroteContact = p.wroteRoll = false;
+    };
+    load(p0, 0);
+    load(p1, 1);
+    load(p2, 2);
+    load(p3, 3);
+    int ia = D.pairs[2 * (size_t)pair], ib = D.pairs[2 * (size_t)pair + 1];
+    if (!(count > 0)) return;
+    Bufs B = bufs(D);
+    const float *P = D.params;
+    const Sweep sweep = (Sweep)SWEEP;
+    float imA = B.im[ia], imB = B.im[ib];
+    Side S;
+    S.second = second;
+    S.me = load_moving(B, second ? ib : ia);
+    Soft soft(P, false);
+    if (imA == 0 || imB == 0) soft.stiffen(P);
+    float halfFirst, halfSecond, sphereFirst, sphereSecond;
+    both(S, S.me.half[0], halfFirst, halfSecond);
+    both(S, S.me.sphere() ? 1.f : 0.f, sphereFirst, sphereSecond);
+    bool firstSphere = sphereFirst > 0.5f, secondSphere = sphereSecond > 0.5f;
+    auto point = [&](HeldPoint &p, int slot) {
+        if (!(slot < count)) return;
+        float r[3];
+        rotate(S.me.q, second ? p.anchorB : p.anchorA, r);
+        if (S.me.sphere())
+            for (int a = 0; a < 3; a++) r[a] = second ? -(S.me.half[0] * nrm[a]) : S.me.half[0] * nrm[a];
+        float at[3];
+        for (int a = 0; a < 3; a++) at[a] = S.me.at[a] + r[a];  // first.at + rA, or second.at + rB
+        float f0, s0, f1, s1, f2, s2;
+        both(S, at[0], f0, s0);
+        both(S, at[1], f1, s1);
+        both(S, at[2], f2, s2);
+        float gap = (s0 - f0) * nrm[0] + (s1 - f1) * nrm[1];
+        gap = gap + (s2 - f2) * nrm[2];
+        contact_side(sweep, gap, nrm, S, r, p.imp[0], &p.imp[1], p.wroteContact, soft);
+        if (firstSphere || secondSphere) {
+            float radius = halfSecond;
+            if (firstSphere) radius = halfFirst;
+            if (firstSphere && secondSphere) radius = (halfFirst * halfSecond) / (halfFirst + halfSecond);
+            roll_side(sweep, S, radius, p.imp[0], &p.imp[4], p.wroteRoll, P);
+        }
+    };
+    if (REVERSE) {
+        point(p3, 3);
+        point(p2, 2);
+        point(p1, 1);
+        point(p0, 0);
+    } else {
+        point(p0, 0);
+        point(p1, 1);
+        point(p2, 2);
+        point(p3, 3);
+    }
+    if (!second) {  // both lanes hold the same impulse words; the first lane stores them
+        auto store = [&](const HeldPoint &p, int slot) {
+            size_t at = held + POINT_IMPULSE * slot;
+            if (p.wroteContact)
+                for (int w = 0; w < 4; w++) D.impulses[at + w] = p.imp[w];
+            if (p.wroteRoll)
+                for (int w = 4; w < POINT_IMPULSE; w++) D.impulses[at + w] = p.imp[w];
+        };
+        store(p0, 0);
+        store(p1, 1);
+        store(p2, 2);
+        store(p3, 3);
+    }
+    store_moving(B, S.me);
+}
+
+// The pairs of round r, two neighbouring lanes a place (solve_pair_side).
 template <int SWEEP, bool REVERSE>
 __global__ void __launch_bounds__(THREADS) k_solve_round(BoxDev D, int r) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < D.width) solve_pair<SWEEP, REVERSE>(D, D.rounds[(size_t)r * D.width + k]);
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = t >> 1;
+    if (k < D.width) solve_pair_side<SWEEP, REVERSE>(D, D.rounds[(size_t)r * D.width + k], (t & 1) != 0);
 }
 
 // ---- a whole step's substeps, a workgroup a scene ----
@@ -847,7 +1097,7 @@ void launch_scene_step(const BoxDev &D, const int32_t *sceneFirst, const int32_t
 void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int r, int pairs, hipStream_t s) {
     if (pairs <= 0) return;
     if (pairs > D.width) throw std::runtime_error("solve round: more pairs than the round holds");
-    int blocks = blocks_for(pairs);  // the round's pairs are its first places; the rest hold -1
+    int blocks = blocks_for(2 * (int64_t)pairs);  // two lanes a pair; the round's pairs are its first places
     switch (sweep * 2 + (reverse ? 1 : 0)) {
         case 0: k_solve_round<WARM, false><<<blocks, THREADS, 0, s>>>(D, r); break;
         case 1: k_solve_round<WARM, true><<<blocks, THREADS, 0, s>>>(D, r); break;
