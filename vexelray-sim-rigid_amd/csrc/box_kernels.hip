@@ -129,6 +129,7 @@ __device__ void walls_of(const BoxDev &D, int s) {
     const SceneDev sc = D.sceneInfo[D.scene[s]];
     float extent[3] = {sc.sx, sc.sy, sc.sz};
     Moving box = load_moving(B, s);
+    const Moving loaded = box;
     Soft soft(P, true);
     // The wall impulses are slot-major: word w of body s at w * n + s, so a wave's loads of one word coalesce.
     size_t n = (size_t)D.n;
@@ -149,7 +150,7 @@ __device__ void walls_of(const BoxDev &D, int s) {
             contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft);
             roll(sweep, nullptr, box, box.half[0], B.lambda, at, B.roll, tangentAt, n, P);
         }
-        store_moving(B, box);
+        store_moving_changed(B, box, loaded);
         return;
     }
     int held = D.wallHeld[s], nowHeld = 0;  // walls at which some corner holds an impulse
@@ -226,7 +227,7 @@ __device__ void walls_of(const BoxDev &D, int s) {
         if (contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft))
             nowHeld |= 1 << wall;
     }
-    store_moving(B, box);
+    store_moving_changed(B, box, loaded);
     if (sweep != WARM) D.wallHeld[s] = nowHeld;
 }
 

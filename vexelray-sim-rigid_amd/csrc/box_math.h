@@ -39,6 +39,7 @@ constexpr int SHOWN_STRIDE = 20;                // Boxes.java:792
 enum Sweep { WARM, SOLVE, RELAX };
 
 __device__ __forceinline__ float mx(float a, float b) { return a < b ? b : a; }  // Math.max / std::max
+__device__ __forceinline__ unsigned bits(float v) { return __float_as_uint(v); }
 __device__ __forceinline__ float mn(float a, float b) { return b < a ? b : a; }  // Math.min / std::min
 
 // Boxes.dot / BoxContact.dot: (x0 y0 + x1 y1) + x2 y2.
@@ -178,6 +179,14 @@ __device__ void store_moving(const Bufs &B, const Moving &m) {
     }
 }
 
+// store_moving for a body loaded as `was`: a word whose bits did not change is left as it is.
+__device__ void store_moving_changed(const Bufs &B, const Moving &m, const Moving &was) {
+    for (int a = 0; a < 3; a++) {
+        if (bits(m.vel[a]) != bits(was.vel[a])) B.vel[a][m.s] = m.vel[a];
+        if (bits(m.mom[a]) != bits(was.mom[a])) B.l[a][m.s] = m.mom[a];
+    }
+}
+
 // Boxes.Soft, Boxes.java:421-450.
 struct Soft {
     float h, margin, bias, massScale, impulseScale, push, muStatic, muKinetic;
@@ -292,11 +301,13 @@ __device__ bool contact(Sweep sweep, float gap, const float *n, Moving *first, c
              float *normals, size_t normalAt, float *tangents, size_t tangentAt, size_t stride, const Soft &soft) {
     float normal = normals[normalAt];
     float tangent[3] = {tangents[tangentAt], tangents[tangentAt + stride], tangents[tangentAt + 2 * stride]};
+    float was[4] = {normal, tangent[0], tangent[1], tangent[2]};
     bool wrote;
     bool holds = contact_held(sweep, gap, n, first, rA, second, rB, normal, tangent, wrote, soft);
-    if (wrote) {
-        normals[normalAt] = normal;
-        for (int a = 0; a < 3; a++) tangents[tangentAt + a * stride] = tangent[a];
+    if (wrote) {  // a word written with the bits it holds is left as it is
+        if (bits(normal) != bits(was[0])) normals[normalAt] = normal;
+        for (int a = 0; a < 3; a++)
+            if (bits(tangent[a]) != bits(was[1 + a])) tangents[tangentAt + a * stride] = tangent[a];
     }
     return holds;
 }
@@ -359,10 +370,12 @@ __device__ void roll_held(Sweep sweep, Moving *first, Moving &second, float radi
 __device__ void roll(Sweep sweep, Moving *first, Moving &second, float radius, const float *normals, size_t normalAt,
           float *rolls, size_t rollAt, size_t stride, const float *P) {
     float held[3] = {rolls[rollAt], rolls[rollAt + stride], rolls[rollAt + 2 * stride]};
+    float was[3] = {held[0], held[1], held[2]};
     bool wrote;
     roll_held(sweep, first, second, radius, normals[normalAt], held, wrote, P);
-    if (wrote)
-        for (int a = 0; a < 3; a++) rolls[rollAt + a * stride] = held[a];
+    if (wrote)  // a word written with the bits it holds is left as it is
+        for (int a = 0; a < 3; a++)
+            if (bits(held[a]) != bits(was[a])) rolls[rollAt + a * stride] = held[a];
 }
 
 // ---- the narrow phase (BoxContact.java) ----
