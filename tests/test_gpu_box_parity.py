@@ -198,3 +198,29 @@ def test_corners_at_the_wall_margin_match_oracle():
     assert np.all(speed < 3.9), "every body should have reached its wall"
     for k in WALL_IMPULSES + STATE:
         assert np.array_equal(sa[k].view(np.int32), sb[k].view(np.int32)), k
+
+
+def test_shapes_rewritten_between_steps_match_oracle():
+    """The HIP wall sweep keeps each body's wall impulses where its thread takes the body (a workgroup's boxes before
+    its balls), so writing the shape buffer moves them. A pile stepped with wall impulses held, then with every
+    seventh body's shape flipped, steps as the oracle's, and its wall impulses read back the same."""
+    out = {}
+    for backend in ("oracle", "gpu"):
+        s = mixed_pile(200, seed=13)
+        s.start(backend)
+        for _ in range(30):  # settled enough that about a hundred corners and balls hold wall impulses
+            s.sim.stepAndWait()
+        shape = s.floats("shape").copy()
+        shape[::7] = np.float32(1.0) - shape[::7]
+        s.sim.write("shape", shape)
+        after = {k: s.floats(k).copy() for k in WALL_IMPULSES}
+        for _ in range(6):
+            s.sim.stepAndWait()
+        out[backend] = (after, {k: s.floats(k).copy() for k in STATE + WALL_IMPULSES})
+        s.close()
+    (wa, sa), (wb, sb) = out["oracle"], out["gpu"]
+    assert any(np.any(wa[k] != 0) for k in WALL_IMPULSES), "impulses should be held when the shapes change"
+    for k in WALL_IMPULSES:
+        assert np.array_equal(wa[k].view(np.int32), wb[k].view(np.int32)), ("after the write", k)
+    for k in STATE + WALL_IMPULSES:
+        assert np.array_equal(sa[k].view(np.int32), sb[k].view(np.int32)), k

@@ -29,6 +29,9 @@ struct BoxDev {
     const float *params;
     float *lambda, *rub, *roll, *shown;
     int32_t *wallHeld;  // n: bit w set when a corner of the body holds an impulse at wall w (boxes; all set = unknown)
+    // n: where body s's wall impulses sit: word w of body s is at w * n + wallPlace[s], wallPlace[s] the place of the
+    // thread that sweeps the body (shape_first), so a wave's loads of one word take neighbouring words
+    int32_t *wallPlace;
     int32_t *keys, *order, *starts;
     int32_t *pairs, *pairsWas, *contacts, *rounds;
     float *manifold, *manifoldWas, *impulses, *impulsesWas;
@@ -86,6 +89,10 @@ void launch_velocities(const BoxDev &D, hipStream_t s);
 void launch_scene_from_params(const BoxDev &D, SceneDev *scene, hipStream_t s);
 void launch_positions(const BoxDev &D, hipStream_t s);
 void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s);
+// The wall sweep's places for the bodies' shapes into `place` (n words), as the sweep takes the bodies.
+void launch_wall_place(const BoxDev &D, int32_t *place, hipStream_t s);
+// Each body's wall impulse words moved from place `from[s]` to `to[s]` (both within the body's workgroup of bodies).
+void launch_wall_move(const BoxDev &D, const int32_t *from, const int32_t *to, hipStream_t s);
 // Every substep of a step (velocities, the warm, solve and relax sweeps, positions), a workgroup a scene, for a batch
 // of scenes (k_scene_step). starts: the colouring's round starts (device); used: the step's rounds.
 void launch_scene_step(const BoxDev &D, const int32_t *sceneFirst, const int32_t *starts, int used, int substeps,

@@ -121,8 +121,9 @@ __device__ __forceinline__ float pick(const float *v, int a) { return a == 0 ? v
 // that never leaves one out), and the wave then takes each lane's list one corner at a time, each lane its own next
 // corner, with the reference's code for a corner. The corners left out are ones the reference does nothing with, and
 // each body's corners run in its order, so every body ends the sweep as the reference leaves it.
+// Body s, its wall impulse words at place `at` (word w at w * n + at).
 template <int SWEEP, bool REVERSE>
-__device__ void walls_of(const BoxDev &D, int s) {
+__device__ void walls_of(const BoxDev &D, int s, int place) {
     if (!(D.im[s] > 0)) return;
     Bufs B = bufs(D);
     const float *P = D.params;
@@ -131,7 +132,7 @@ __device__ void walls_of(const BoxDev &D, int s) {
     Moving box = load_moving(B, s);
     const Moving loaded = box;
     Soft soft(P, true);
-    // The wall impulses are slot-major: word w of body s at w * n + s, so a wave's loads of one word coalesce.
+    // The wall impulses are slot-major: word w of body s at w * n + place, so a wave's loads of one word coalesce.
     size_t n = (size_t)D.n;
     const Sweep sweep = (Sweep)SWEEP;
     if (box.sphere()) {
@@ -143,8 +144,8 @@ __device__ void walls_of(const BoxDev &D, int s) {
             float nrm[3], r[3];
             for (int a = 0; a < 3; a++) nrm[a] = a == axis ? (high ? -1.0f : 1.0f) : 0.0f;
             for (int a = 0; a < 3; a++) r[a] = -(box.half[0] * nrm[a]);
-            size_t at = (size_t)firstSlot * n + s;
-            size_t tangentAt = (size_t)firstSlot * 3 * n + s;
+            size_t at = (size_t)firstSlot * n + place;
+            size_t tangentAt = (size_t)firstSlot * 3 * n + place;
             float reach = pick(box.at, axis) + pick(r, axis);
             float gap = high ? pick(extent, axis) - reach : reach;
             contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft);
@@ -194,7 +195,7 @@ __device__ void walls_of(const BoxDev &D, int s) {
             bool near = sweep != WARM && !((bound - slack) >= soft.margin);
             bool holding = false;
             if (live) {  // Boxes.contact's own test, on the same words
-                size_t at = (size_t)(firstSlot + c) * n + s, tangentAt = (size_t)(firstSlot + c) * 3 * n + s;
+                size_t at = (size_t)(firstSlot + c) * n + place, tangentAt = (size_t)(firstSlot + c) * 3 * n + place;
                 float tangent[3] = {B.rub[tangentAt], B.rub[tangentAt + n], B.rub[tangentAt + 2 * n]};
                 holding = B.lambda[at] > 0 || dot(tangent, tangent) > 0;
             }
@@ -222,8 +223,8 @@ __device__ void walls_of(const BoxDev &D, int s) {
         float gap = high ? pick(extent, axis) - reach : reach;
         if (!live && !(gap < soft.margin)) continue;
         int firstSlot = wall * CORNERS;
-        size_t at = (size_t)(firstSlot + c) * n + s;
-        size_t tangentAt = (size_t)(firstSlot + c) * 3 * n + s;
+        size_t at = (size_t)(firstSlot + c) * n + place;
+        size_t tangentAt = (size_t)(firstSlot + c) * 3 * n + place;
         if (contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft))
             nowHeld |= 1 << wall;
     }
@@ -234,7 +235,26 @@ __device__ void walls_of(const BoxDev &D, int s) {
 template <int SWEEP, bool REVERSE>
 __global__ void __launch_bounds__(THREADS) k_walls(BoxDev D) {
     int s = shape_first(D);
-    if (s < D.n) walls_of<SWEEP, REVERSE>(D, s);
+    if (s < D.n) walls_of<SWEEP, REVERSE>(D, s, blockIdx.x * THREADS + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(THREADS) k_wall_place(BoxDev D, int32_t *place) {
+    int s = shape_first(D);
+    if (s < D.n) place[s] = blockIdx.x * THREADS + threadIdx.x;
+}
+
+// One word row (w) of the three wall impulse buffers for a workgroup's bodies, moved through LDS.
+__global__ void __launch_bounds__(THREADS) k_wall_move(BoxDev D, const int32_t *from, const int32_t *to) {
+    __shared__ float row[THREADS];
+    int s = blockIdx.x * THREADS + threadIdx.x;
+    int w = blockIdx.y;  // 0 .. LAMBDA_WORDS + 2 * RUB_WORDS - 1
+    float *buf = w < LAMBDA_WORDS ? D.lambda : (w < LAMBDA_WORDS + RUB_WORDS ? D.rub : D.roll);
+    int k = w < LAMBDA_WORDS ? w : (w < LAMBDA_WORDS + RUB_WORDS ? w - LAMBDA_WORDS : w - LAMBDA_WORDS - RUB_WORDS);
+    size_t rowAt = (size_t)k * D.n;
+    bool body = s < D.n;
+    if (body) row[threadIdx.x] = buf[rowAt + from[s]];
+    __syncthreads();
+    if (body) buf[rowAt + to[s]] = row[threadIdx.x];
 }
 
 // BoxPairs.solve, BoxPairs.java:64-151: one pair of a round (a slot, or -1 for none), its manifold's points in turn.
@@ -610,7 +630,7 @@ __device__ int first_pair_of(const BoxDev &D, const int32_t *list, int count, in
 template <int SWEEP, bool REVERSE>
 __device__ void scene_sweep(const BoxDev &D, int b0, int b1, const int *from, const int *to, int used) {
     if (REVERSE) {
-        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) walls_of<SWEEP, REVERSE>(D, s);
+        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) walls_of<SWEEP, REVERSE>(D, s, s);
         __syncthreads();
     }
     for (int order = 0; order < used; order++) {  // BoxStep.sweep, BoxStep.java:217-231
@@ -621,7 +641,7 @@ __device__ void scene_sweep(const BoxDev &D, int b0, int b1, const int *from, co
         __syncthreads();
     }
     if (!REVERSE) {
-        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) walls_of<SWEEP, REVERSE>(D, s);
+        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) walls_of<SWEEP, REVERSE>(D, s, s);
         __syncthreads();
     }
 }
@@ -1073,6 +1093,17 @@ void launch_positions(const BoxDev &D, hipStream_t s) {
 template <int SWEEP, bool REVERSE>
 static void walls_then(const BoxDev &D, hipStream_t s) {
     k_walls<SWEEP, REVERSE><<<blocks_for(D.n), THREADS, 0, s>>>(D);
+}
+
+void launch_wall_place(const BoxDev &D, int32_t *place, hipStream_t s) {
+    k_wall_place<<<blocks_for(D.n), THREADS, 0, s>>>(D, place);
+    check_launch("wall place");
+}
+
+void launch_wall_move(const BoxDev &D, const int32_t *from, const int32_t *to, hipStream_t s) {
+    dim3 grid((unsigned)blocks_for(D.n), (unsigned)(LAMBDA_WORDS + 2 * RUB_WORDS));
+    k_wall_move<<<grid, THREADS, 0, s>>>(D, from, to);
+    check_launch("wall move");
 }
 
 void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s) {

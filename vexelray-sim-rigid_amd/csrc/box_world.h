@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "vexelray_rigid.h"
 
@@ -35,6 +36,9 @@ hipStream_t box_stream(BoxWorld *w);
 void box_params(BoxWorld *w, const int32_t *words, int32_t n);
 void box_step(BoxWorld *w, vr_step_report *rep);
 void box_wall_impulses_written(BoxWorld *w);  // after a write of lambda or rub through the ABI
+void box_shapes_written(BoxWorld *w);         // after a write of shape: the wall impulses follow their bodies
+// Where each body's wall impulse words sit in the slot-major buffers (word w of body s at w * n + out[s]).
+void box_wall_places(BoxWorld *w, std::vector<int32_t> &out);
 // The learned warm start: a log of per-point (features, targets), and the network that starts each step.
 void box_log(BoxWorld *w, bool on, int64_t capacity);
 int64_t box_samples(BoxWorld *w, float *x, float *t, int64_t most);

@@ -49,6 +49,7 @@ struct BoxWorld {
     bool wallsFromParams = true;       // one scene, walled by the params' box (the reference's box world)
     SceneDev *sceneDev = nullptr;
     int32_t *sceneFirst = nullptr;  // scenes + 1: each scene's first body
+    int32_t *wallPlaceNext = nullptr;  // n: the wall places being worked out for newly written shapes
     bool sceneStep = false;         // substeps a workgroup a scene (k_scene_step)
     static constexpr size_t SCENE_STEP_LEAST = 64;  // scenes, for the scene step to fill the chip
     // Bodies in the largest scene, for one workgroup to keep up: a scene's sweep is a chain of dependent steps in
@@ -150,6 +151,8 @@ struct BoxWorld {
         D.roll = alloc<float>((int64_t)RUB_WORDS * n);
         D.shown = alloc<float>((int64_t)SHOWN_STRIDE * n);
         D.wallHeld = alloc<int32_t>(n);
+        D.wallPlace = alloc<int32_t>(n);
+        wallPlaceNext = alloc<int32_t>(n);
         reg("lambda", D.lambda, (int64_t)LAMBDA_WORDS * n, VR_F32, LAMBDA_WORDS);
         reg("rub", D.rub, (int64_t)RUB_WORDS * n, VR_F32, RUB_WORDS);
         reg("roll", D.roll, (int64_t)RUB_WORDS * n, VR_F32, RUB_WORDS);
@@ -247,6 +250,33 @@ struct BoxWorld {
         }
         HIP_OK(hipMemsetAsync(D.wallHeld, 0, (size_t)n * 4, stream));  // no impulse anywhere
         vr::launch_fill(D.q[3], 1.0f, n, stream);
+        place_walls(false);  // every shape 0 (a box), every impulse 0
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    // The wall sweep's places for the shapes as they are; with `move`, the wall impulse words follow their bodies.
+    // A batch stepped a workgroup a scene keeps body order (the scene step sweeps bodies in order).
+    void place_walls(bool move) {
+        if (sceneStep) {
+            launch_wall_identity();
+            return;
+        }
+        launch_wall_place(D, wallPlaceNext, stream);
+        if (move) launch_wall_move(D, D.wallPlace, wallPlaceNext, stream);
+        std::swap(D.wallPlace, wallPlaceNext);
+    }
+
+    void launch_wall_identity() {
+        std::vector<int32_t> id((size_t)n);
+        for (int s = 0; s < n; s++) id[s] = s;
+        HIP_OK(hipMemcpyAsync(D.wallPlace, id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    // Where each body's wall impulse words sit (host copy of D.wallPlace).
+    void wall_places(std::vector<int32_t> &out) {
+        out.resize((size_t)n);
+        HIP_OK(hipMemcpyAsync(out.data(), D.wallPlace, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
     }
 
@@ -506,6 +536,12 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n) {
 
 void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
 void box_wall_impulses_written(BoxWorld *w) { w->wall_impulses_written(); }
+void box_shapes_written(BoxWorld *w) {
+    w->place_walls(true);
+    // wallHeld is kept for boxes only: a body that was a ball has none, so after new shapes every wall is looked at.
+    w->wall_impulses_written();
+}
+void box_wall_places(BoxWorld *w, std::vector<int32_t> &out) { w->wall_places(out); }
 void box_log(BoxWorld *w, bool on, int64_t capacity) { w->log(on, capacity); }
 void box_diagnostics(BoxWorld *w, double *out) { w->diagnostics(out); }
 int64_t box_samples(BoxWorld *w, float *x, float *t, int64_t most) { return w->samples(x, t, most); }

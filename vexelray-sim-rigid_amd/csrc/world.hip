@@ -626,12 +626,14 @@ Buf find_any(vr_world *w, const std::string &name, int *bodyWords = nullptr) {
 
 hipStream_t stream_of(vr_world *w) { return w->box ? vrbox::box_stream(w->box) : w->stream; }
 
-// A slot-major device buffer to the reference's body-by-body order (toBodies) or back, the whole buffer.
-void reorder(const int32_t *from, int32_t *to, int64_t count, int words, bool toBodies) {
+// A slot-major device buffer to the reference's body-by-body order (toBodies) or back, the whole buffer: word k of
+// body b sits at k * bodies + place[b] on the device.
+void reorder(const int32_t *from, int32_t *to, int64_t count, int words, bool toBodies,
+             const std::vector<int32_t> &place) {
     int64_t bodies = count / words;
     for (int64_t b = 0; b < bodies; b++)
         for (int k = 0; k < words; k++) {
-            int64_t bodyOrder = b * words + k, slotOrder = (int64_t)k * bodies + b;
+            int64_t bodyOrder = b * words + k, slotOrder = (int64_t)k * bodies + place[(size_t)b];
             if (toBodies) to[bodyOrder] = from[slotOrder];
             else to[slotOrder] = from[bodyOrder];
         }
@@ -643,17 +645,20 @@ void copy_in(vr_world *w, const std::string &name, const void *src, int64_t coun
     if (count < 0 || count > b.count) throw NameError("buffer " + name + " is shorter than the write");
     hipStream_t s = stream_of(w);
     if (words > 0) {  // a part written: the rest kept, so the whole buffer goes through the host
-        std::vector<int32_t> dev((size_t)b.count), host((size_t)b.count);
+        std::vector<int32_t> dev((size_t)b.count), host((size_t)b.count), place;
+        vrbox::box_wall_places(w->box, place);
         HIP_OK(hipMemcpyAsync(dev.data(), b.p, (size_t)b.count * 4, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-        reorder(dev.data(), host.data(), b.count, words, true);
+        reorder(dev.data(), host.data(), b.count, words, true, place);
         std::memcpy(host.data(), src, (size_t)count * 4);
-        reorder(host.data(), dev.data(), b.count, words, false);
+        reorder(host.data(), dev.data(), b.count, words, false, place);
         HIP_OK(hipMemcpyAsync(b.p, dev.data(), (size_t)b.count * 4, hipMemcpyHostToDevice, s));
         HIP_OK(hipStreamSynchronize(s));
         vrbox::box_wall_impulses_written(w->box);
     } else {
         HIP_OK(hipMemcpyAsync(b.p, src, (size_t)count * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (w->box && name == "shape") vrbox::box_shapes_written(w->box);
     }
     HIP_OK(hipStreamSynchronize(s));
 }
@@ -664,10 +669,11 @@ void copy_out(vr_world *w, const std::string &name, void *dst, int64_t count) {
     if (count < 0 || count > b.count) throw NameError("buffer " + name + " is shorter than the read");
     hipStream_t s = stream_of(w);
     if (words > 0) {
-        std::vector<int32_t> dev((size_t)b.count), host((size_t)b.count);
+        std::vector<int32_t> dev((size_t)b.count), host((size_t)b.count), place;
+        vrbox::box_wall_places(w->box, place);
         HIP_OK(hipMemcpyAsync(dev.data(), b.p, (size_t)b.count * 4, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-        reorder(dev.data(), host.data(), b.count, words, true);
+        reorder(dev.data(), host.data(), b.count, words, true, place);
         std::memcpy(dst, host.data(), (size_t)count * 4);
     } else {
         HIP_OK(hipMemcpyAsync(dst, b.p, (size_t)count * 4, hipMemcpyDeviceToHost, s));
