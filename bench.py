@@ -192,7 +192,7 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build steps one world on one MI355X and has no multi-GPU path "
+        print(f"bench.py measures one GPU: the library steps one world on one MI355X and has no multi-GPU path "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
 

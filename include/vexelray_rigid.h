@@ -95,7 +95,7 @@ typedef struct vr_step_desc {
 /*
  * SphereStepper.Report (SphereStepper.java:42) and SphereSimulation.StepReport's timings (SphereSimulation.java:48).
  * `waits` counts the host waits the reference's stepper makes for the same step (one per report it reads), so the
- * two reports compare; this build decides "done" on the device and waits for none of them.
+ * two reports compare; this library decides "done" on the device and waits for none of them.
  */
 typedef struct vr_step_report {
     int32_t waits;

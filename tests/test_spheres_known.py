@@ -2,7 +2,7 @@
 
 Each test names the reference test it restates (ctest/sphere/... = vexelray-sim-rigid-core/src/test/java/dev/vexelray/
 sim/rigid/sphere/...). The "oracle" leg runs the CPU restatement and pins it to closed forms; the "gpu" leg runs the
-HIP library through the C ABI and is judged by the same numbers, as the reference judges each backend on its own.
+HIP library through the C ABI and is held to the same numbers, as the reference holds each backend on its own.
 """
 
 import math
