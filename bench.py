@@ -140,6 +140,90 @@ def batched(pkg, steps, warmup, epochs):
     }
 
 
+def residual_sweeps(pkg, scenes_count=256, settle=200, epochs=10):
+    """The learned warm start by the residual it leaves: a batch of `scenes_count` scenes settles for `settle` steps,
+    logs three more, and the network trains on the samples; then, from that same state, one step runs plain and one
+    with the network at each sweep count, and the residual after it is measured: the penetration at every touching
+    point (the step's listed pairs put through the library's narrow phase on the final state) and of every body past
+    its scene's walls, over the body's size (a ball's radius, a box's smallest half-extent)."""
+    scenes = __import__(pkg.__name__ + ".scenes", fromlist=["scenes"])
+    boxes = __import__(pkg.__name__ + ".boxes", fromlist=["boxes"])
+    mlp = __import__(pkg.__name__ + ".mlp", fromlist=["mlp"])
+    spec = scenes.batched_2k(scenes=scenes_count)
+    scene_of = np.repeat(np.arange(scenes_count), spec.bodies)
+    ext = np.asarray(spec.extents, dtype=np.float64)[scene_of]
+    half = spec.half.astype(np.float64)
+    ball = spec.shape > 0.5
+    size = np.where(ball, half[0], half.min(axis=0))
+    kept = ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw", "lambda", "rub", "roll",
+            "pairs", "pairsWas", "manifold", "manifoldWas", "impulses", "impulsesWas")
+
+    def make(sweeps):
+        b = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, sweeps)
+        b.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
+                spec.params)
+        return b
+
+    def residual(b):
+        pos = np.stack([b.floats(k).astype(np.float64) for k in ("x", "y", "z")])
+        qx, qy, qz, qw = (b.floats(k).astype(np.float64) for k in ("qx", "qy", "qz", "qw"))
+        pairs = b.ints("pairs").reshape(-1, 2)
+        listed = pairs[pairs[:, 1] >= 0]
+        parts = []
+        if len(listed):
+            m = b._world.collide(listed.reshape(-1)).reshape(len(listed), 40)  # the library's narrow phase
+            count = m[:, 3].astype(np.int64)
+            depth = np.stack([m[:, 8 + 8 * k + 6] for k in range(4)], axis=1)
+            valid = np.arange(4)[None, :] < count[:, None]
+            scale = np.minimum(size[listed[:, 0]], size[listed[:, 1]])
+            parts.append((depth / scale[:, None])[valid])
+        rot = np.array([[1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)],
+                        [2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)],
+                        [2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)]])
+        reach = np.where(ball[None, :], half[0][None, :], (np.abs(rot) * half[None, :, :]).sum(axis=1))
+        parts.append(np.maximum(reach - pos, pos + reach - ext.T).max(axis=0) / size)
+        pen = np.concatenate(parts)
+        pen = pen[pen > 0]
+        return {"max": float(pen.max()), "p99": float(np.percentile(pen, 99)), "mean": float(pen.mean())} \
+            if len(pen) else {"max": 0.0, "p99": 0.0, "mean": 0.0}
+
+    sim = make(spec.sweeps)
+    for _ in range(settle):
+        sim.stepAndWait()
+    sim.log(True, 4 << 20)
+    for _ in range(3):
+        sim.stepAndWait()
+    x, t = sim.samples(4 << 20)
+    sim.log(False)
+    snap = {k: (sim.ints(k).copy() if k in ("pairs", "pairsWas") else sim.floats(k).copy()) for k in kept}
+    settled = residual(sim)
+    sim.close()
+    net = mlp.Mlp(seed=3)
+    net.train(x, t, epochs=epochs, batch=4096, rate=1e-3, seed=1)
+
+    def one_step(sweeps, learned):
+        b = make(sweeps)
+        for k in kept:
+            b.write(k, snap[k])
+        if learned:
+            b.learned(net)
+        b.stepAndWait()
+        out = residual(b)
+        b.close()
+        return out
+
+    rows = []
+    for sweeps in (1, 2, 3, 5, 8, 10, 15, 20, 30):
+        rows.append({"sweeps": sweeps, "plain": one_step(sweeps, False), "learned": one_step(sweeps, True)})
+        progress(len(rows), 9)
+    net.close()
+    return {"workload": f"batched_2k with {scenes_count} scenes, {int(spec.bodies.sum())} bodies, settled "
+                        f"{settle} steps, one step from the same state at each sweep count",
+            "residual": "penetration over the body's size at touching points and walls (max, 99th percentile, "
+                        "mean over penetrating points)",
+            "settled": settled, "samples": int(len(x)), "epochs": epochs, "by_sweeps": rows}
+
+
 _last_progress = [time.perf_counter()]
 
 
@@ -187,6 +271,9 @@ def main():
     ap.add_argument("--workload", default="pile_64k")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--batched-steps", type=int, default=30, help="timed steps of the batched workload (0: skip)")
+    ap.add_argument("--residual", action="store_true",
+                    help="only measure the residual a step leaves, plain and with the learned warm start, at several "
+                         "sweep counts from the same state (residual_sweeps), then exit")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB: only copy a buffer of that size 5 times (the known-size kernel that calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -207,6 +294,9 @@ def main():
         return
 
     pkg = load_package()
+    if args.residual:
+        print(json.dumps(residual_sweeps(pkg)))
+        return
     scenes = __import__(pkg.__name__ + ".scenes", fromlist=["scenes"])
     spec = scenes.SCENES[args.workload]()
     if hasattr(spec, "towers"):  # a box-path scene
