@@ -707,12 +707,15 @@ __device__ __forceinline__ void reach_of(const BoxDev &D, int s, float *out) {  
 }
 
 // BoxBroad.list, BoxBroad.java:103-164: a body's partners of higher index, in ascending order, into its 32 slots.
-__global__ void k_list(BoxDev D) {
+__global__ void __launch_bounds__(THREADS) k_list(BoxDev D) {
+    // A thread's partners so far, sorted, in LDS: word k of thread i at k * THREADS + i, so lanes at different places
+    // in their lists still take different banks (a per-lane index into a register array costs a select a word).
+    __shared__ int parts[SLOTS * THREADS];
+    int *part = parts + threadIdx.x;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= D.n) return;
     int s = D.order[t];  // threads take bodies in cell order, so a wave walks neighbouring cells together
     size_t base = (size_t)s * SLOTS;
-    int part[SLOTS];
     float margin = D.params[MARGIN];
     float centre[3] = {D.x[0][s], D.x[1][s], D.x[2][s]};
     float rs[3];
@@ -737,18 +740,18 @@ __global__ void k_list(BoxDev D) {
             if (!close) continue;
             if (found < SLOTS) {
                 int place = found;
-                while (place > 0 && part[place - 1] > o) {
-                    part[place] = part[place - 1];
+                while (place > 0 && part[(place - 1) * THREADS] > o) {
+                    part[place * THREADS] = part[(place - 1) * THREADS];
                     place--;
                 }
-                part[place] = o;
+                part[place * THREADS] = o;
             }
             found++;
         }
     }
     for (int m = 0; m < SLOTS; m++) {
         D.pairs[(base + m) * 2] = s;
-        D.pairs[(base + m) * 2 + 1] = m < found ? part[m] : -1;
+        D.pairs[(base + m) * 2 + 1] = m < found ? part[m * THREADS] : -1;
     }
     D.contacts[2 * (size_t)D.slots + s] = found;
 }
