@@ -224,3 +224,38 @@ def test_shapes_rewritten_between_steps_match_oracle():
         assert np.array_equal(wa[k].view(np.int32), wb[k].view(np.int32)), ("after the write", k)
     for k in STATE + WALL_IMPULSES:
         assert np.array_equal(sa[k].view(np.int32), sb[k].view(np.int32)), k
+
+
+def test_pair_buffers_written_from_outside_match_oracle():
+    """The HIP step leaves a slot's manifold count and impulses alone where it knows they already read as an empty
+    slot's; a buffer written through the ABI may hold anything, so every slot is written again. The kept manifold
+    and impulses, filled with words other than zero everywhere, become the step's own buffers after the swap: the
+    step must leave them, empty slots included, as the oracle's does."""
+    out = {}
+    for backend in ("oracle", "gpu"):
+        s = mixed_pile(120, seed=17)
+        s.start(backend)
+        for _ in range(8):
+            s.sim.stepAndWait()
+        for name in ("manifoldWas", "impulsesWas"):
+            junk = np.full(s.floats(name).shape, np.float32(0.25))
+            s.sim.write(name, junk)
+        r = s.sim.stepAndWait()
+        out[backend] = ((r.touching, r.points, r.rounds),
+                        {k: s.floats(k).copy() for k in ("manifold", "impulses") + STATE})
+        s.close()
+    (ra, a), (rb, b) = out["oracle"], out["gpu"]
+    assert ra == rb
+    for k in ("impulses",) + STATE:
+        assert np.array_equal(a[k].view(np.int32), b[k].view(np.int32)), k
+    # The manifold's words a step writes: every slot's count, and a touching pair's normal, axis and the 7 words of
+    # each counted point (of its 8). The rest hold what the buffer held before; the HIP library swaps the kept and
+    # current buffers where the oracle copies, so those words differ, and nothing reads them.
+    ma, mb = a["manifold"].reshape(-1, 40).view(np.int32), b["manifold"].reshape(-1, 40).view(np.int32)
+    assert np.array_equal(ma[:, 3], mb[:, 3]), "counts"
+    count = a["manifold"].reshape(-1, 40)[:, 3].astype(np.int64)
+    assert count.max() > 0, "the pile should touch"
+    for p in np.nonzero(count > 0)[0]:
+        assert np.array_equal(ma[p, :5], mb[p, :5]), p
+        for k in range(int(count[p])):
+            assert np.array_equal(ma[p, 8 + 8 * k:15 + 8 * k], mb[p, 8 + 8 * k:15 + 8 * k]), (p, k)

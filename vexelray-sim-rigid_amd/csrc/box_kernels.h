@@ -35,6 +35,10 @@ struct BoxDev {
     int32_t *keys, *order, *starts;
     int32_t *pairs, *pairsWas, *contacts, *rounds;
     float *manifold, *manifoldWas, *impulses, *impulsesWas;
+    // n words each, one a body, bit k for the body's slot k, set where the buffer beside it may hold a word other
+    // than zero at that slot (its manifold's count; its impulses): a slot whose bit is clear already reads as an
+    // empty slot is written, so the per-step passes over every slot leave it alone. They swap with their buffers.
+    uint32_t *manifoldHeld, *manifoldHeldWas, *impulseHeld, *impulseHeldWas;
 };
 
 // The device's colouring of the touching pairs into rounds (box_colour.hip).

@@ -828,13 +828,27 @@ __device__ void collide_pair(const BoxDev &D, int ia, int ib, float *m) {
     if (shapeA > 0.5f || shapeB > 0.5f) sphere_contact(first, second, shapeA, shapeB, margin, m);
 }
 
-__global__ void k_collide(BoxDev D) {
+// The held bits of a workgroup's slots (a body's 32 slots are 32 neighbouring lanes, half a wave) as one word a body:
+// the lane of each body's slot 0 stores the word the ballot of `held` gives its half of the wave.
+__device__ __forceinline__ void store_held(uint32_t *words, int p, bool held) {
+    unsigned long long mask = __ballot(held);
+    int lane = threadIdx.x & 63;
+    if (p % SLOTS == 0) words[p / SLOTS] = (uint32_t)(mask >> (lane & 32));
+}
+
+__global__ void __launch_bounds__(THREADS) k_collide(BoxDev D) {
+    static_assert(SLOTS == 32 && THREADS % 64 == 0, "a body's slots are half a wave");
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= D.slots) return;
+    if (p >= D.slots) return;  // slots is bodies * 32: a body's slots are all in or all out
     float *m = D.manifold + (size_t)p * MANIFOLD_WORDS;
     int partner = D.pairs[2 * (size_t)p + 1];
+    bool held = (D.manifoldHeld[p / SLOTS] >> (p % SLOTS)) & 1u;
+    store_held(D.manifoldHeld, p, partner >= 0);
+    if (partner < 0) {
+        if (held) m[COUNT] = 0;  // a count of 0 already there is left as it is
+        return;
+    }
     m[COUNT] = 0;
-    if (partner < 0) return;
     collide_pair(D, D.pairs[2 * (size_t)p], partner, m);
 }
 
@@ -848,18 +862,24 @@ __global__ void k_collide_pairs(BoxDev D, const int32_t *pairs, int npairs, floa
 }
 
 // BoxPairs.carry, BoxPairs.java:191-267, and BoxBroad.counts, BoxBroad.java:174-188, one thread a slot.
-__global__ void k_carry_counts(BoxDev D) {
+__global__ void __launch_bounds__(THREADS) k_carry_counts(BoxDev D) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D.slots) return;
     size_t out = (size_t)p * MANIFOLD_WORDS, held = (size_t)p * IMPULSE_WORDS;
-    int count = (int)D.manifold[out + COUNT];
     int partner = D.pairs[2 * (size_t)p + 1];
+    // A slot with no partner has count 0 in its manifold (k_collide wrote it, or its held bit was clear).
+    int count = partner < 0 ? 0 : (int)D.manifold[out + COUNT];
+    bool wasHeld = (D.impulseHeld[p / SLOTS] >> (p % SLOTS)) & 1u;
+    store_held(D.impulseHeld, p, count > 0);
     if (!(count > 0)) {
         // A slot with no point carries nothing: its impulses are zero whatever it held before (the search below
         // finds the slot's old place only to copy from it for points it has), and it lists no contact. Most slots
-        // are such; they skip the search, and their 28 words go out as seven 16-byte stores.
-        float4 *zero = reinterpret_cast<float4 *>(D.impulses + held);  // 112 bytes a slot: 16-byte aligned
-        for (int k = 0; k < IMPULSE_WORDS / 4; k++) zero[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        // are such; they skip the search, and their 28 words go out as seven 16-byte stores, or not at all where
+        // the slot's held bit says they are zero already.
+        if (wasHeld) {
+            float4 *zero = reinterpret_cast<float4 *>(D.impulses + held);  // 112 bytes a slot: 16-byte aligned
+            for (int k = 0; k < IMPULSE_WORDS / 4; k++) zero[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         D.contacts[p] = -1;
         D.contacts[(size_t)D.slots + p] = count;
         return;

@@ -37,6 +37,7 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n);
 void box_step(BoxWorld *w, vr_step_report *rep);
 void box_wall_impulses_written(BoxWorld *w);  // after a write of lambda or rub through the ABI
 void box_shapes_written(BoxWorld *w);         // after a write of shape: the wall impulses follow their bodies
+void box_pair_buffer_written(BoxWorld *w, const std::string &name);  // after a write of a manifold or impulse buffer
 // Where each body's wall impulse words sit in the slot-major buffers (word w of body s at w * n + out[s]).
 void box_wall_places(BoxWorld *w, std::vector<int32_t> &out);
 // The learned warm start: a log of per-point (features, targets), and the network that starts each step.

@@ -172,6 +172,10 @@ struct BoxWorld {
             D.manifoldWas = alloc<float>((int64_t)MANIFOLD_WORDS * slots);
             D.impulses = alloc<float>((int64_t)IMPULSE_WORDS * slots);
             D.impulsesWas = alloc<float>((int64_t)IMPULSE_WORDS * slots);
+            D.manifoldHeld = alloc<uint32_t>(n);
+            D.manifoldHeldWas = alloc<uint32_t>(n);
+            D.impulseHeld = alloc<uint32_t>(n);
+            D.impulseHeldWas = alloc<uint32_t>(n);
             D.contacts = alloc<int32_t>(2 * slots + n);
             D.rounds = alloc<int32_t>((int64_t)MOST_ROUNDS * D.width);
             reg("contacts", D.contacts, 2 * slots + n, VR_I32);
@@ -247,6 +251,9 @@ struct BoxWorld {
                 HIP_OK(hipMemsetAsync(p, 0, slots * MANIFOLD_WORDS * 4, stream));
             for (void *p : {(void *)D.impulses, (void *)D.impulsesWas})
                 HIP_OK(hipMemsetAsync(p, 0, slots * IMPULSE_WORDS * 4, stream));
+            for (void *p : {(void *)D.manifoldHeld, (void *)D.manifoldHeldWas, (void *)D.impulseHeld,
+                            (void *)D.impulseHeldWas})
+                HIP_OK(hipMemsetAsync(p, 0, (size_t)n * 4, stream));  // every slot zero
         }
         HIP_OK(hipMemsetAsync(D.wallHeld, 0, (size_t)n * 4, stream));  // no impulse anywhere
         vr::launch_fill(D.q[3], 1.0f, n, stream);
@@ -270,6 +277,18 @@ struct BoxWorld {
         std::vector<int32_t> id((size_t)n);
         for (int s = 0; s < n; s++) id[s] = s;
         HIP_OK(hipMemcpyAsync(D.wallPlace, id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    // A manifold or impulse buffer written from outside: any slot may hold words other than zero.
+    void pair_buffer_written(const std::string &name) {
+        uint32_t *held = nullptr;
+        if (name == "manifold") held = D.manifoldHeld;
+        if (name == "manifoldWas") held = D.manifoldHeldWas;
+        if (name == "impulses") held = D.impulseHeld;
+        if (name == "impulsesWas") held = D.impulseHeldWas;
+        if (!held) return;
+        vr::launch_fill_i32(reinterpret_cast<int32_t *>(held), -1, n, stream);
         HIP_OK(hipStreamSynchronize(stream));
     }
 
@@ -392,6 +411,8 @@ struct BoxWorld {
             std::swap(D.pairs, D.pairsWas);
             std::swap(D.manifold, D.manifoldWas);
             std::swap(D.impulses, D.impulsesWas);
+            std::swap(D.manifoldHeld, D.manifoldHeldWas);
+            std::swap(D.impulseHeld, D.impulseHeldWas);
             launch_bin(D, stream);
             vr::launch_sort(sortView, S, stream);
             launch_list(D, stream);
@@ -542,6 +563,7 @@ void box_shapes_written(BoxWorld *w) {
     w->wall_impulses_written();
 }
 void box_wall_places(BoxWorld *w, std::vector<int32_t> &out) { w->wall_places(out); }
+void box_pair_buffer_written(BoxWorld *w, const std::string &name) { w->pair_buffer_written(name); }
 void box_log(BoxWorld *w, bool on, int64_t capacity) { w->log(on, capacity); }
 void box_diagnostics(BoxWorld *w, double *out) { w->diagnostics(out); }
 int64_t box_samples(BoxWorld *w, float *x, float *t, int64_t most) { return w->samples(x, t, most); }

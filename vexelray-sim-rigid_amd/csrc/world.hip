@@ -659,6 +659,7 @@ void copy_in(vr_world *w, const std::string &name, const void *src, int64_t coun
         HIP_OK(hipMemcpyAsync(b.p, src, (size_t)count * 4, hipMemcpyHostToDevice, s));
         HIP_OK(hipStreamSynchronize(s));
         if (w->box && name == "shape") vrbox::box_shapes_written(w->box);
+        if (w->box) vrbox::box_pair_buffer_written(w->box, name);
     }
     HIP_OK(hipStreamSynchronize(s));
 }
