@@ -49,13 +49,13 @@ def test_random_pair_manifolds_with_margin_and_spheres_match():
             assert np.array_equal(a[p * 40 + 8:p * 40 + used].view(np.int32), b[p * 40 + 8:p * 40 + used].view(np.int32))
 
 
-def mixed_pile(n=120, seed=31, spheres=True):
-    """Boxes and spheres dropped into a 1 m box with friction and rolling resistance."""
+def mixed_pile(n=120, seed=31, spheres=True, rolling=0.05):
+    """Boxes and spheres dropped into a 1 m box with friction and (unless `rolling` is 0) rolling resistance."""
     rnd = pkg.JavaRandom(seed)
     s = BoxScene(n)
     s.sy = 2
     s.muStatic = s.muKinetic = 0.5
-    s.muRolling = 0.05
+    s.muRolling = rolling
     for k in range(n):
         if spheres and k % 2 == 1:
             s.sphere(k, 0.03 + 0.03 * rnd.nextDouble())
@@ -88,15 +88,15 @@ def run_both(make, steps):
     return out["oracle"], out["gpu"]
 
 
-@pytest.mark.parametrize("spheres", [False, True])
-def test_mixed_pile_matches_oracle(spheres):
-    (sa, ra, ca, qa), (sb, rb, cb, qb) = run_both(lambda: mixed_pile(spheres=spheres), 40)
+@pytest.mark.parametrize("spheres,rolling", [(False, 0.05), (True, 0.05), (True, 0.0)])
+def test_mixed_pile_matches_oracle(spheres, rolling):
+    (sa, ra, ca, qa), (sb, rb, cb, qb) = run_both(lambda: mixed_pile(spheres=spheres, rolling=rolling), 40)
     assert ra == rb
     assert any(r[0] > 0 for r in ra), "the pile should touch"
     assert np.array_equal(ca, cb)
     assert np.array_equal(qa, qb), "the rounds (colouring) differ"
     for k in WALL_IMPULSES:
-        assert np.any(sa[k] != 0) or (k == "roll" and not spheres), k  # rolling resistance is for balls
+        assert np.any(sa[k] != 0) or (k == "roll" and not (spheres and rolling)), k  # rolling resistance is for balls
         assert np.array_equal(sa[k].view(np.int32), sb[k].view(np.int32)), k
     for k in STATE:
         scale = np.maximum(np.abs(sa[k]), 1e-3)

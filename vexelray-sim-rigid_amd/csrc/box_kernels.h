@@ -28,7 +28,10 @@ struct BoxDev {
     float *x[3], *vel[3], *l[3], *q[4], *im, *half[3], *shape;
     const float *params;
     float *lambda, *rub, *roll, *shown;
-    int32_t *wallHeld;  // n: bit w set when a corner of the body holds an impulse at wall w (boxes; all set = unknown)
+    // n: for a box, bit wall * 8 + c set for every corner c holding an impulse at the wall (and maybe others); for a
+    // ball, bit wall clear only when the wall's impulse words are quiet (nothing held, rolling words +0). All set when
+    // not known.
+    uint64_t *cornerHeld;
     // n: where body s's wall impulses sit: word w of body s is at w * n + wallPlace[s], wallPlace[s] the place of the
     // thread that sweeps the body (shape_first), so a wave's loads of one word take neighbouring words
     int32_t *wallPlace;

@@ -111,19 +111,162 @@ __device__ __forceinline__ float pick(const float *v, int a) { return a == 0 ? v
 
 // Boxes.walls, Boxes.java:357-416: one sweep of a body's corners (a sphere's nearest point) against the six walls.
 // A body's sweep reads and writes only that body's state and wall impulses, so which thread takes which body changes
-// nothing in the result. A workgroup takes its THREADS consecutive bodies with the boxes first and then the balls
-// (shape_first), so its waves run the box path (eight corners a wall) or the ball path (one point a wall), not one
-// after the other, while the workgroup's loads still fall in the same cache lines as in body order.
+// nothing in the result. The wall impulses are slot-major: word w of the body at place `place` is at w * n + place,
+// so a wave's loads of one word coalesce.
 //
 // A box's corner does something only if it holds an impulse or, in a solve or relax sweep, is within the margin of
 // the wall: any other returns at once from Boxes.contact, or is passed over before it when its wall holds nothing.
 // Lanes of a wave differ in which of their 48 corners do something, and a wave that ran every corner some lane needs
-// would run most of them. So each lane first lists its corners that may do something, in the order the reference
+// would run most of them. So each box first lists its corners that may do something, in the order the reference
 // takes them (a bit a corner: the holding ones exactly, from their impulses; the ones within the margin from a bound
 // that never leaves one out), and the wave then takes each lane's list one corner at a time, each lane its own next
 // corner, with the reference's code for a corner. The corners left out are ones the reference does nothing with, and
 // each body's corners run in its order, so every body ends the sweep as the reference leaves it.
-// Body s, its wall impulse words at place `at` (word w at w * n + at).
+
+// A ball's sweep: its point nearest each wall, Boxes.contact and then Boxes.roll. `held` is the ball's cornerHeld
+// word: bit `wall` clear only when the wall's words are quiet, its normal impulse not above zero, its tangent impulse
+// of zero length and its rolling impulse three words of +0. At a quiet wall Boxes.contact returns at once in a warm
+// sweep, and in a solve or relax sweep when the point is outside the margin; Boxes.roll then turns the ball by a
+// zero change (in a warm sweep by its rolling impulse; otherwise by the new impulse less the old, which with no
+// rolling friction is the old impulse less itself) and writes back what the words held. So such a wall is the turn by
+// +0 alone, which does not need its words. Returns the word as the sweep leaves it.
+template <int SWEEP, bool REVERSE>
+__device__ unsigned long long ball_walls(const Bufs &B, Moving &box, const float *extent, unsigned long long held,
+                                         size_t n, int place, const Soft &soft, const float *P) {
+    const Sweep sweep = (Sweep)SWEEP;
+    bool still = !(P[MU_ROLLING] > 0);  // Boxes.roll's turning stop is taken only with rolling friction
+    for (int order = 0; order < WALLS; order++) {
+        int wall = REVERSE ? WALLS - 1 - order : order;
+        int axis = wall / 2;
+        bool high = wall % 2 == 1;
+        int firstSlot = wall * CORNERS;
+        float nrm[3], r[3];
+        for (int a = 0; a < 3; a++) nrm[a] = a == axis ? (high ? -1.0f : 1.0f) : 0.0f;
+        for (int a = 0; a < 3; a++) r[a] = -(box.half[0] * nrm[a]);
+        float reach = pick(box.at, axis) + pick(r, axis);
+        float gap = high ? pick(extent, axis) - reach : reach;
+        bool quiet = !((held >> wall) & 1ull);
+        if (quiet && (sweep == WARM || (still && !(gap < soft.margin)))) {
+            const float none[3] = {0.f, 0.f, 0.f};
+            box.turn(none);
+            continue;
+        }
+        size_t at = (size_t)firstSlot * n + place;
+        size_t tangentAt = (size_t)firstSlot * 3 * n + place;
+        float normal = B.lambda[at];
+        float tangent[3] = {B.rub[tangentAt], B.rub[tangentAt + n], B.rub[tangentAt + 2 * n]};
+        float was[4] = {normal, tangent[0], tangent[1], tangent[2]};
+        bool wrote;
+        contact_held(sweep, gap, nrm, nullptr, nullptr, box, r, normal, tangent, wrote, soft);
+        if (wrote) {  // a word written with the bits it holds is left as it is
+            if (bits(normal) != bits(was[0])) B.lambda[at] = normal;
+            for (int a = 0; a < 3; a++)
+                if (bits(tangent[a]) != bits(was[1 + a])) B.rub[tangentAt + a * n] = tangent[a];
+        }
+        float rolled[3] = {B.roll[tangentAt], B.roll[tangentAt + n], B.roll[tangentAt + 2 * n]};
+        float rollWas[3] = {rolled[0], rolled[1], rolled[2]};
+        roll_held(sweep, nullptr, box, box.half[0], normal, rolled, wrote, P);
+        if (wrote)
+            for (int a = 0; a < 3; a++)
+                if (bits(rolled[a]) != bits(rollWas[a])) B.roll[tangentAt + a * n] = rolled[a];
+        if (sweep != WARM) {
+            bool nowQuiet = !(normal > 0) && !(dot(tangent, tangent) > 0) &&
+                            (bits(rolled[0]) | bits(rolled[1]) | bits(rolled[2])) == 0u;
+            held = nowQuiet ? held & ~(1ull << wall) : held | (1ull << wall);
+        }
+    }
+    return held;
+}
+
+// The corners of a box (centre `at`, turn q, half sizes `half`) that may do something in this sweep: bit
+// order * CORNERS + k for the k-th corner taken at the order-th wall taken. `held` is the box's cornerHeld word: bit
+// wall * CORNERS + c is set for every corner c that holds an impulse at wall `wall` (and maybe for others).
+template <int SWEEP, bool REVERSE>
+__device__ unsigned long long corner_list(const float *at, const float *q, const float *half, const float *extent,
+                                          unsigned long long held, const Soft &soft) {
+    const Sweep sweep = (Sweep)SWEEP;
+    float diagonal = sqrtf((half[0] * half[0] + half[1] * half[1]) + half[2] * half[2]);
+    unsigned long long todo = 0;
+    for (int order = 0; order < WALLS; order++) {
+        int wall = REVERSE ? WALLS - 1 - order : order;
+        int axis = wall / 2;
+        bool high = wall % 2 == 1;
+        unsigned holding = (unsigned)(held >> (wall * CORNERS)) & 0xffu;
+        if (sweep == WARM) {  // a warm sweep does something only at the corners holding an impulse
+            for (int k = 0; k < CORNERS; k++)
+                if ((holding >> (REVERSE ? CORNERS - 1 - k : k)) & 1u) todo |= 1ull << (order * CORNERS + k);
+            continue;
+        }
+        // rotate turns by q / |q|, so a corner's offset along the wall's axis is row . offset for the axis's row of
+        // the turn, which is the axis turned back; `slack` (1e-4 of the sizes involved) covers the rounding of both
+        // ways of working it out and of the gap's two additions, which stays below 1e-6 of them. So a corner whose
+        // gap so bounded is at least the margin has its own gap at least the margin, and no corner within the margin
+        // is left out; NaN or infinite sizes leave every corner in.
+        float centre = pick(at, axis), side = pick(extent, axis);
+        float slack = 1e-4f * ((diagonal + fabsf(centre)) + fabsf(side));
+        // No corner is further than the half-diagonal from the centre along any axis: a wall none of whose corners
+        // holds anything and whose nearest corner could not be within the margin is passed over whole.
+        float clear = high ? side - centre : centre;
+        if (holding == 0 && (clear - diagonal) - slack >= soft.margin) continue;
+        float along[3] = {0.f, 0.f, 0.f}, row[3], cq[4];
+        along[0] = axis == 0 ? 1.f : 0.f;
+        along[1] = axis == 1 ? 1.f : 0.f;
+        along[2] = axis == 2 ? 1.f : 0.f;
+        conjugate(q, cq);
+        rotate(cq, along, row);
+        for (int k = 0; k < CORNERS; k++) {
+            int c = REVERSE ? CORNERS - 1 - k : k;
+            float offset[3];
+            for (int a = 0; a < 3; a++) {
+                int bit = (c / (1 << a)) % 2;
+                offset[a] = half[a] * (2.0f * (float)bit - 1.0f);
+            }
+            float rough = dot(row, offset);
+            float bound = high ? side - (centre + rough) : centre + rough;
+            bool near = !((bound - slack) >= soft.margin);
+            if (((holding >> c) & 1u) || near) todo |= 1ull << (order * CORNERS + k);
+        }
+    }
+    return todo;
+}
+
+// The listed corners of a box, in list order, each through Boxes.contact; returns the corners that hold an impulse
+// after it, as cornerHeld has them.
+template <int SWEEP, bool REVERSE>
+__device__ unsigned long long corner_sweep(const Bufs &B, Moving &box, const float *extent, unsigned long long held,
+                                           unsigned long long todo, size_t n, int place, const Soft &soft) {
+    const Sweep sweep = (Sweep)SWEEP;
+    unsigned long long nowHeld = 0;
+    while (__any(todo != 0)) {
+        if (todo == 0) continue;
+        int bit = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        int order = bit / CORNERS, k = bit % CORNERS;
+        int wall = REVERSE ? WALLS - 1 - order : order;
+        int c = REVERSE ? CORNERS - 1 - k : k;
+        int axis = wall / 2;
+        bool high = wall % 2 == 1;
+        bool holding = (held >> (wall * CORNERS + c)) & 1;
+        float nrm[3], offset[3], r[3];
+        for (int a = 0; a < 3; a++) nrm[a] = a == axis ? (high ? -1.0f : 1.0f) : 0.0f;
+        for (int a = 0; a < 3; a++) {
+            int b = (c / (1 << a)) % 2;
+            offset[a] = box.half[a] * (2.0f * (float)b - 1.0f);
+        }
+        rotate(box.q, offset, r);
+        float reach = pick(box.at, axis) + pick(r, axis);
+        float gap = high ? pick(extent, axis) - reach : reach;
+        // A corner holding nothing and outside the margin returns at once from Boxes.contact.
+        if (!holding && !(gap < soft.margin)) continue;
+        size_t at = (size_t)(wall * CORNERS + c) * n + place, tangentAt = (size_t)(wall * CORNERS + c) * 3 * n + place;
+        if (contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft))
+            nowHeld |= 1ull << (wall * CORNERS + c);
+    }
+    return nowHeld;
+}
+
+// Body s, its wall impulse words at place `place`. Its cornerHeld word is rewritten by a solve or relax sweep (a
+// warm sweep changes no impulse).
 template <int SWEEP, bool REVERSE>
 __device__ void walls_of(const BoxDev &D, int s, int place) {
     if (!(D.im[s] > 0)) return;
@@ -134,106 +277,21 @@ __device__ void walls_of(const BoxDev &D, int s, int place) {
     Moving box = load_moving(B, s);
     const Moving loaded = box;
     Soft soft(P, true);
-    // The wall impulses are slot-major: word w of body s at w * n + place, so a wave's loads of one word coalesce.
     size_t n = (size_t)D.n;
-    const Sweep sweep = (Sweep)SWEEP;
+    unsigned long long held = D.cornerHeld[s], nowHeld;
     if (box.sphere()) {
-        for (int order = 0; order < WALLS; order++) {
-            int wall = REVERSE ? WALLS - 1 - order : order;
-            int axis = wall / 2;
-            bool high = wall % 2 == 1;
-            int firstSlot = wall * CORNERS;
-            float nrm[3], r[3];
-            for (int a = 0; a < 3; a++) nrm[a] = a == axis ? (high ? -1.0f : 1.0f) : 0.0f;
-            for (int a = 0; a < 3; a++) r[a] = -(box.half[0] * nrm[a]);
-            size_t at = (size_t)firstSlot * n + place;
-            size_t tangentAt = (size_t)firstSlot * 3 * n + place;
-            float reach = pick(box.at, axis) + pick(r, axis);
-            float gap = high ? pick(extent, axis) - reach : reach;
-            contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft);
-            roll(sweep, nullptr, box, box.half[0], B.lambda, at, B.roll, tangentAt, n, P);
-        }
-        store_moving_changed(B, box, loaded);
-        return;
+        nowHeld = ball_walls<SWEEP, REVERSE>(B, box, extent, held, n, place, soft, P);
+    } else {
+        unsigned long long todo = corner_list<SWEEP, REVERSE>(box.at, box.q, box.half, extent, held, soft);
+        nowHeld = corner_sweep<SWEEP, REVERSE>(B, box, extent, held, todo, n, place, soft);
     }
-    int held = D.wallHeld[s], nowHeld = 0;  // walls at which some corner holds an impulse
-    float diagonal = sqrtf((box.half[0] * box.half[0] + box.half[1] * box.half[1]) + box.half[2] * box.half[2]);
-    // The list: bit order * CORNERS + k for the k-th corner taken at the order-th wall taken.
-    unsigned long long todo = 0;
-    for (int order = 0; order < WALLS; order++) {
-        int wall = REVERSE ? WALLS - 1 - order : order;
-        int axis = wall / 2;
-        bool high = wall % 2 == 1;
-        // A wall none of whose corners holds an impulse (its bit clear in wallHeld) is passed over in a warm sweep.
-        bool live = (held >> wall) & 1;
-        if (sweep == WARM && !live) continue;
-        // rotate turns by q / |q|, so a corner's offset along the wall's axis is row . offset for the axis's row of
-        // the turn, which is the axis turned back; `slack` (1e-4 of the sizes involved) covers the rounding of both
-        // ways of working it out and of the gap's two additions, which stays below 1e-6 of them. So a corner whose
-        // gap so bounded is at least the margin has its own gap at least the margin, and no corner within the margin
-        // is left out; NaN or infinite sizes leave every corner in.
-        float centre = pick(box.at, axis), side = pick(extent, axis);
-        float slack = 1e-4f * ((diagonal + fabsf(centre)) + fabsf(side));
-        // No corner is further than the half-diagonal from the centre along any axis: a wall holding nothing whose
-        // nearest corner could not be within the margin is passed over whole.
-        float clear = high ? side - centre : centre;
-        if (!live && (clear - diagonal) - slack >= soft.margin) continue;
-        float along[3] = {0.f, 0.f, 0.f}, row[3], cq[4];
-        along[0] = axis == 0 ? 1.f : 0.f;
-        along[1] = axis == 1 ? 1.f : 0.f;
-        along[2] = axis == 2 ? 1.f : 0.f;
-        conjugate(box.q, cq);
-        rotate(cq, along, row);
-        int firstSlot = wall * CORNERS;
-        for (int k = 0; k < CORNERS; k++) {
-            int c = REVERSE ? CORNERS - 1 - k : k;
-            float offset[3];
-            for (int a = 0; a < 3; a++) {
-                int bit = (c / (1 << a)) % 2;
-                offset[a] = box.half[a] * (2.0f * (float)bit - 1.0f);
-            }
-            float rough = dot(row, offset);
-            float bound = high ? side - (centre + rough) : centre + rough;
-            bool near = sweep != WARM && !((bound - slack) >= soft.margin);
-            bool holding = false;
-            if (live) {  // Boxes.contact's own test, on the same words
-                size_t at = (size_t)(firstSlot + c) * n + place, tangentAt = (size_t)(firstSlot + c) * 3 * n + place;
-                float tangent[3] = {B.rub[tangentAt], B.rub[tangentAt + n], B.rub[tangentAt + 2 * n]};
-                holding = B.lambda[at] > 0 || dot(tangent, tangent) > 0;
-            }
-            if (holding || near) todo |= 1ull << (order * CORNERS + k);
-        }
-    }
-    while (__any(todo != 0)) {
-        if (todo == 0) continue;
-        int bit = __ffsll((long long)todo) - 1;
-        todo &= todo - 1;
-        int order = bit / CORNERS, k = bit % CORNERS;
-        int wall = REVERSE ? WALLS - 1 - order : order;
-        int c = REVERSE ? CORNERS - 1 - k : k;
-        int axis = wall / 2;
-        bool high = wall % 2 == 1;
-        bool live = (held >> wall) & 1;
-        float nrm[3], offset[3], r[3];
-        for (int a = 0; a < 3; a++) nrm[a] = a == axis ? (high ? -1.0f : 1.0f) : 0.0f;
-        for (int a = 0; a < 3; a++) {
-            int b = (c / (1 << a)) % 2;
-            offset[a] = box.half[a] * (2.0f * (float)b - 1.0f);
-        }
-        rotate(box.q, offset, r);
-        float reach = pick(box.at, axis) + pick(r, axis);
-        float gap = high ? pick(extent, axis) - reach : reach;
-        if (!live && !(gap < soft.margin)) continue;
-        int firstSlot = wall * CORNERS;
-        size_t at = (size_t)(firstSlot + c) * n + place;
-        size_t tangentAt = (size_t)(firstSlot + c) * 3 * n + place;
-        if (contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft))
-            nowHeld |= 1 << wall;
-    }
+    if (SWEEP != WARM && nowHeld != held) D.cornerHeld[s] = nowHeld;
     store_moving_changed(B, box, loaded);
-    if (sweep != WARM) D.wallHeld[s] = nowHeld;
 }
 
+// The sweep over all bodies. A workgroup takes its THREADS consecutive bodies with the boxes first and then the balls
+// (shape_first), so its waves run the box path or the ball path, not one after the other, while the workgroup's
+// loads still fall in the same cache lines as in body order.
 template <int SWEEP, bool REVERSE>
 __global__ void __launch_bounds__(THREADS) k_walls(BoxDev D) {
     int s = shape_first(D);

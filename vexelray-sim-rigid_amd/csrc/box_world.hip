@@ -150,7 +150,7 @@ struct BoxWorld {
         D.rub = alloc<float>((int64_t)RUB_WORDS * n);
         D.roll = alloc<float>((int64_t)RUB_WORDS * n);
         D.shown = alloc<float>((int64_t)SHOWN_STRIDE * n);
-        D.wallHeld = alloc<int32_t>(n);
+        D.cornerHeld = alloc<uint64_t>(n);
         D.wallPlace = alloc<int32_t>(n);
         wallPlaceNext = alloc<int32_t>(n);
         reg("lambda", D.lambda, (int64_t)LAMBDA_WORDS * n, VR_F32, LAMBDA_WORDS);
@@ -255,7 +255,7 @@ struct BoxWorld {
                             (void *)D.impulseHeldWas})
                 HIP_OK(hipMemsetAsync(p, 0, (size_t)n * 4, stream));  // every slot zero
         }
-        HIP_OK(hipMemsetAsync(D.wallHeld, 0, (size_t)n * 4, stream));  // no impulse anywhere
+        HIP_OK(hipMemsetAsync(D.cornerHeld, 0, (size_t)n * 8, stream));  // no impulse anywhere
         vr::launch_fill(D.q[3], 1.0f, n, stream);
         place_walls(false);  // every shape 0 (a box), every impulse 0
         HIP_OK(hipStreamSynchronize(stream));
@@ -338,9 +338,9 @@ struct BoxWorld {
         return take;
     }
 
-    // Wall impulses written from outside: which walls hold one is not known, so every wall is looked at.
+    // Wall impulses written from outside: which corners hold one is not known, so every corner is looked at.
     void wall_impulses_written() {
-        vr::launch_fill_i32(D.wallHeld, (1 << WALLS) - 1, n, stream);
+        vr::launch_fill_i32(reinterpret_cast<int32_t *>(D.cornerHeld), -1, 2 * n, stream);
         HIP_OK(hipStreamSynchronize(stream));
     }
 
@@ -559,7 +559,7 @@ void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
 void box_wall_impulses_written(BoxWorld *w) { w->wall_impulses_written(); }
 void box_shapes_written(BoxWorld *w) {
     w->place_walls(true);
-    // wallHeld is kept for boxes only: a body that was a ball has none, so after new shapes every wall is looked at.
+    // cornerHeld is kept for boxes only: a body that was a ball has none, so after new shapes every corner is looked at.
     w->wall_impulses_written();
 }
 void box_wall_places(BoxWorld *w, std::vector<int32_t> &out) { w->wall_places(out); }
