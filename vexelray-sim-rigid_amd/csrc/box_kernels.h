@@ -42,6 +42,19 @@ struct BoxDev {
     // than zero at that slot (its manifold's count; its impulses): a slot whose bit is clear already reads as an
     // empty slot is written, so the per-step passes over every slot leave it alone. They swap with their buffers.
     uint32_t *manifoldHeld, *manifoldHeldWas, *impulseHeld, *impulseHeldWas;
+    // The touching pairs packed in round order for a step's sweeps (k_pack, then the rounds, then k_unpack), field by
+    // field with stride packCap, so that a round's neighbouring lanes read neighbouring words. Entry j is the pair
+    // at place j of the colouring's order: round r's pairs are entries starts[r] .. starts[r + 1] - 1.
+    // n records each: a body's frame and motion over a run of sweeps of the pass-by-pass step (box_math.h), made from
+    // the state by k_frame_in and the motion returned to it by k_motion_out.
+    float4 *frame, *motion;
+    int64_t packCap;
+    int32_t *packPair;     // the pair's slot
+    int32_t *packCount;    // its manifold's point count
+    int32_t *packBody;     // 2 fields: its first body, then its second
+    float *packNormal;     // 3 fields
+    float *packAnchor;     // 24 fields: field (side * MOST_POINTS + point) * 3 + axis, side 0 the first body's
+    float *packImpulse;    // 28 fields: field point * POINT_IMPULSE + word
 };
 
 // The device's colouring of the touching pairs into rounds (box_colour.hip).
@@ -104,7 +117,15 @@ void launch_wall_move(const BoxDev &D, const int32_t *from, const int32_t *to, h
 // of scenes (k_scene_step). starts: the colouring's round starts (device); used: the step's rounds.
 void launch_scene_step(const BoxDev &D, const int32_t *sceneFirst, const int32_t *starts, int used, int substeps,
                        int sweeps, bool symmetric, hipStream_t s);
-void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int r, int pairs, hipStream_t s);
+// Round r's pairs, packed entries base .. base + pairs - 1.
+void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int base, int pairs, hipStream_t s);
+// The touching pairs (entries 0 .. touching - 1 of the colouring's order) into the packed fields, and the packed
+// impulses back to the pairs' slots.
+void launch_pack(const BoxDev &D, const Colouring &C, int touching, hipStream_t s);
+// The frame records (and with `motion` the motion records) from the state; the motion records back to the state.
+void launch_frame_in(const BoxDev &D, bool motion, hipStream_t s);
+void launch_motion_out(const BoxDev &D, hipStream_t s);
+void launch_unpack(const BoxDev &D, int touching, hipStream_t s);
 void launch_bin(const BoxDev &D, hipStream_t s);
 void launch_list(const BoxDev &D, hipStream_t s);
 void launch_collide(const BoxDev &D, hipStream_t s);

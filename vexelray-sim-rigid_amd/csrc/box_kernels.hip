@@ -267,14 +267,14 @@ __device__ unsigned long long corner_sweep(const Bufs &B, Moving &box, const flo
 
 // Body s, its wall impulse words at place `place`. Its cornerHeld word is rewritten by a solve or relax sweep (a
 // warm sweep changes no impulse).
-template <int SWEEP, bool REVERSE>
+template <int SWEEP, bool REVERSE, bool MOTION = false>
 __device__ void walls_of(const BoxDev &D, int s, int place) {
     if (!(D.im[s] > 0)) return;
     Bufs B = bufs(D);
     const float *P = D.params;
     const SceneDev sc = D.sceneInfo[D.scene[s]];
     float extent[3] = {sc.sx, sc.sy, sc.sz};
-    Moving box = load_moving(B, s);
+    Moving box = MOTION ? load_moved(B, D.motion, s) : load_moving(B, s);
     const Moving loaded = box;
     Soft soft(P, true);
     size_t n = (size_t)D.n;
@@ -286,7 +286,10 @@ __device__ void walls_of(const BoxDev &D, int s, int place) {
         nowHeld = corner_sweep<SWEEP, REVERSE>(B, box, extent, held, todo, n, place, soft);
     }
     if (SWEEP != WARM && nowHeld != held) D.cornerHeld[s] = nowHeld;
-    store_moving_changed(B, box, loaded);
+    if (MOTION)
+        store_motion_changed(D.motion, box, loaded);
+    else
+        store_moving_changed(B, box, loaded);
 }
 
 // The sweep over all bodies. A workgroup takes its THREADS consecutive bodies with the boxes first and then the balls
@@ -295,7 +298,7 @@ __device__ void walls_of(const BoxDev &D, int s, int place) {
 template <int SWEEP, bool REVERSE>
 __global__ void __launch_bounds__(THREADS) k_walls(BoxDev D) {
     int s = shape_first(D);
-    if (s < D.n) walls_of<SWEEP, REVERSE>(D, s, blockIdx.x * THREADS + threadIdx.x);
+    if (s < D.n) walls_of<SWEEP, REVERSE, true>(D, s, blockIdx.x * THREADS + threadIdx.x);
 }
 
 __global__ void __launch_bounds__(THREADS) k_wall_place(BoxDev D, int32_t *place) {
@@ -569,38 +572,40 @@ __device__ void roll_side(Sweep sweep, Side &S, float radius, float normal, floa
     apply_angular_side(S, change);
 }
 
-// solve_pair on two lanes: `second` tells which of the pair's bodies this lane holds. Both lanes of a pair call it
-// together with the same pair.
+// solve_pair on two lanes, for packed entry j: `second` tells which of the pair's bodies this lane holds. Both lanes
+// of a pair call it together with the same entry. Each lane loads its own body's anchors and the impulses of the
+// points the pair has.
 template <int SWEEP, bool REVERSE>
-__device__ void solve_pair_side(const BoxDev &D, int pair, bool second) {
-    if (pair < 0) return;
-    size_t out = (size_t)pair * MANIFOLD_WORDS;
-    size_t held = (size_t)pair * IMPULSE_WORDS;
-    int count = (int)D.manifold[out + COUNT];
-    float nrm[3] = {D.manifold[out], D.manifold[out + 1], D.manifold[out + 2]};
+__device__ void solve_pair_side(const BoxDev &D, int j, bool second) {
+    size_t T = (size_t)D.packCap;
+    int side = second ? 1 : 0;
+    int count = D.packCount[j];
+    float nrm[3] = {D.packNormal[j], D.packNormal[T + j], D.packNormal[2 * T + j]};
+    Side S;
+    S.second = second;
+    S.me = load_framed(D.frame, D.motion, D.packBody[side * T + j]);
+    if (!(count > 0)) return;
     HeldPoint p0, p1, p2, p3;
     auto load = [&](HeldPoint &p, int slot) {
-        size_t where = out + POINTS + slot * POINT_WORDS;
-        for (int a = 0; a < 3; a++) {
-            p.anchorA[a] = D.manifold[where + ANCHOR_FIRST + a];
-            p.anchorB[a] = D.manifold[where + ANCHOR_SECOND + a];
-        }
-        for (int w = 0; w < POINT_IMPULSE; w++) p.imp[w] = D.impulses[held + POINT_IMPULSE * slot + w];
         p.wroteContact = p.wroteRoll = false;
+        if (!(slot < count)) {
+            for (int a = 0; a < 3; a++) p.anchorA[a] = p.anchorB[a] = 0.f;
+            for (int w = 0; w < POINT_IMPULSE; w++) p.imp[w] = 0.f;
+            return;
+        }
+        const float *anchor = D.packAnchor + (size_t)((side * MOST_POINTS + slot) * 3) * T + j;
+        for (int a = 0; a < 3; a++) p.anchorA[a] = p.anchorB[a] = anchor[a * T];
+        const float *imp = D.packImpulse + (size_t)(slot * POINT_IMPULSE) * T + j;
+        for (int w = 0; w < POINT_IMPULSE; w++) p.imp[w] = imp[w * T];
     };
     load(p0, 0);
     load(p1, 1);
     load(p2, 2);
     load(p3, 3);
-    int ia = D.pairs[2 * (size_t)pair], ib = D.pairs[2 * (size_t)pair + 1];
-    if (!(count > 0)) return;
-    Bufs B = bufs(D);
     const float *P = D.params;
     const Sweep sweep = (Sweep)SWEEP;
-    float imA = B.im[ia], imB = B.im[ib];
-    Side S;
-    S.second = second;
-    S.me = load_moving(B, second ? ib : ia);
+    float imA, imB;
+    both(S, S.me.im, imA, imB);
     Soft soft(P, false);
     if (imA == 0 || imB == 0) soft.stiffen(P);
     float halfFirst, halfSecond, sphereFirst, sphereSecond;
@@ -642,26 +647,95 @@ __device__ void solve_pair_side(const BoxDev &D, int pair, bool second) {
     }
     if (!second) {  // both lanes hold the same impulse words; the first lane stores them
         auto store = [&](const HeldPoint &p, int slot) {
-            size_t at = held + POINT_IMPULSE * slot;
+            float *imp = D.packImpulse + (size_t)(slot * POINT_IMPULSE) * T + j;
             if (p.wroteContact)
-                for (int w = 0; w < 4; w++) D.impulses[at + w] = p.imp[w];
+                for (int w = 0; w < 4; w++) imp[w * T] = p.imp[w];
             if (p.wroteRoll)
-                for (int w = 4; w < POINT_IMPULSE; w++) D.impulses[at + w] = p.imp[w];
+                for (int w = 4; w < POINT_IMPULSE; w++) imp[w * T] = p.imp[w];
         };
         store(p0, 0);
         store(p1, 1);
         store(p2, 2);
         store(p3, 3);
     }
-    store_moving(B, S.me);
+    store_motion(D.motion, S.me);
 }
 
-// The pairs of round r, two neighbouring lanes a place (solve_pair_side).
+// A round's pairs, packed entries base .. base + pairs - 1, two neighbouring lanes an entry (solve_pair_side).
 template <int SWEEP, bool REVERSE>
-__global__ void __launch_bounds__(THREADS) k_solve_round(BoxDev D, int r) {
+__global__ void __launch_bounds__(THREADS) k_solve_round(BoxDev D, int base, int pairs) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int k = t >> 1;
-    if (k < D.width) solve_pair_side<SWEEP, REVERSE>(D, D.rounds[(size_t)r * D.width + k], (t & 1) != 0);
+    if (k < pairs) solve_pair_side<SWEEP, REVERSE>(D, base + k, (t & 1) != 0);
+}
+
+// Body s's frame record from the state, and with `motion` its motion record.
+__global__ void k_frame_in(BoxDev D, int motion) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    float im = D.im[s], shape = D.shape[s], half[3], inverse[3];
+    for (int a = 0; a < 3; a++) half[a] = D.half[a][s];
+    inverse_inertia(half, im, shape, inverse);
+    float4 *f = D.frame + (size_t)s * FRAME_VECTORS;
+    f[0] = make_float4(D.x[0][s], D.x[1][s], D.x[2][s], im);
+    f[1] = make_float4(D.q[0][s], D.q[1][s], D.q[2][s], D.q[3][s]);
+    f[2] = make_float4(half[0], half[1], half[2], shape);
+    f[3] = make_float4(inverse[0], inverse[1], inverse[2], 0.f);
+    if (motion) {
+        float4 *v = D.motion + (size_t)s * MOTION_VECTORS;
+        v[0] = make_float4(D.vel[0][s], D.vel[1][s], D.vel[2][s], 0.f);
+        v[1] = make_float4(D.l[0][s], D.l[1][s], D.l[2][s], 0.f);
+    }
+}
+
+// Body s's velocity and momentum from its motion record back to the state.
+__global__ void k_motion_out(BoxDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    const float4 *v = D.motion + (size_t)s * MOTION_VECTORS;
+    float4 e = v[0], g = v[1];
+    D.vel[0][s] = e.x, D.vel[1][s] = e.y, D.vel[2][s] = e.z;
+    D.l[0][s] = g.x, D.l[1][s] = g.y, D.l[2][s] = g.z;
+}
+
+// BoxPairs' words of the touching pairs gathered into the packed fields, entry j the j-th of the colouring's order.
+__global__ void k_pack(BoxDev D, Colouring C, int touching) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= touching) return;
+    size_t T = (size_t)D.packCap;
+    int pair = C.touch[C.order[j]];
+    const float *m = D.manifold + (size_t)pair * MANIFOLD_WORDS;
+    const float *imp = D.impulses + (size_t)pair * IMPULSE_WORDS;
+    int count = (int)m[COUNT];
+    D.packPair[j] = pair;
+    D.packCount[j] = count;
+    D.packBody[j] = D.pairs[2 * (size_t)pair];
+    D.packBody[T + j] = D.pairs[2 * (size_t)pair + 1];
+    for (int a = 0; a < 3; a++) D.packNormal[a * T + j] = m[a];
+    for (int k = 0; k < MOST_POINTS; k++) {
+        if (!(k < count)) break;
+        const float *point = m + POINTS + k * POINT_WORDS;
+        for (int a = 0; a < 3; a++) {
+            D.packAnchor[(size_t)(k * 3 + a) * T + j] = point[ANCHOR_FIRST + a];
+            D.packAnchor[(size_t)((MOST_POINTS + k) * 3 + a) * T + j] = point[ANCHOR_SECOND + a];
+        }
+        for (int w = 0; w < POINT_IMPULSE; w++)
+            D.packImpulse[(size_t)(k * POINT_IMPULSE + w) * T + j] = imp[k * POINT_IMPULSE + w];
+    }
+}
+
+// The packed impulses back to their pairs' slots: the points a pair has (the rounds touch no others).
+__global__ void k_unpack(BoxDev D, int touching) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= touching) return;
+    size_t T = (size_t)D.packCap;
+    int pair = D.packPair[j], count = D.packCount[j];
+    float *imp = D.impulses + (size_t)pair * IMPULSE_WORDS;
+    for (int k = 0; k < MOST_POINTS; k++) {
+        if (!(k < count)) break;
+        for (int w = 0; w < POINT_IMPULSE; w++)
+            imp[k * POINT_IMPULSE + w] = D.packImpulse[(size_t)(k * POINT_IMPULSE + w) * T + j];
+    }
 }
 
 // ---- a whole step's substeps, a workgroup a scene ----
@@ -1219,19 +1293,43 @@ void launch_scene_step(const BoxDev &D, const int32_t *sceneFirst, const int32_t
     check_launch("scene step");
 }
 
-void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int r, int pairs, hipStream_t s) {
+void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int base, int pairs, hipStream_t s) {
     if (pairs <= 0) return;
-    if (pairs > D.width) throw std::runtime_error("solve round: more pairs than the round holds");
-    int blocks = blocks_for(2 * (int64_t)pairs);  // two lanes a pair; the round's pairs are its first places
+    if (base < 0 || (int64_t)base + pairs > D.packCap)
+        throw std::runtime_error("solve round: entries past the packed pairs");
+    int blocks = blocks_for(2 * (int64_t)pairs);  // two lanes a pair
     switch (sweep * 2 + (reverse ? 1 : 0)) {
-        case 0: k_solve_round<WARM, false><<<blocks, THREADS, 0, s>>>(D, r); break;
-        case 1: k_solve_round<WARM, true><<<blocks, THREADS, 0, s>>>(D, r); break;
-        case 2: k_solve_round<SOLVE, false><<<blocks, THREADS, 0, s>>>(D, r); break;
-        case 3: k_solve_round<SOLVE, true><<<blocks, THREADS, 0, s>>>(D, r); break;
-        case 4: k_solve_round<RELAX, false><<<blocks, THREADS, 0, s>>>(D, r); break;
-        default: k_solve_round<RELAX, true><<<blocks, THREADS, 0, s>>>(D, r); break;
+        case 0: k_solve_round<WARM, false><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
+        case 1: k_solve_round<WARM, true><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
+        case 2: k_solve_round<SOLVE, false><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
+        case 3: k_solve_round<SOLVE, true><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
+        case 4: k_solve_round<RELAX, false><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
+        default: k_solve_round<RELAX, true><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
     }
     check_launch("solve round");
+}
+
+void launch_pack(const BoxDev &D, const Colouring &C, int touching, hipStream_t s) {
+    if (touching <= 0) return;
+    if (touching > D.packCap) throw std::runtime_error("pack: more touching pairs than the packed fields hold");
+    k_pack<<<blocks_for(touching), THREADS, 0, s>>>(D, C, touching);
+    check_launch("pack");
+}
+
+void launch_frame_in(const BoxDev &D, bool motion, hipStream_t s) {
+    k_frame_in<<<blocks_for(D.n), THREADS, 0, s>>>(D, motion ? 1 : 0);
+    check_launch("frame in");
+}
+
+void launch_motion_out(const BoxDev &D, hipStream_t s) {
+    k_motion_out<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    check_launch("motion out");
+}
+
+void launch_unpack(const BoxDev &D, int touching, hipStream_t s) {
+    if (touching <= 0) return;
+    k_unpack<<<blocks_for(touching), THREADS, 0, s>>>(D, touching);
+    check_launch("unpack");
 }
 
 void launch_bin(const BoxDev &D, hipStream_t s) {

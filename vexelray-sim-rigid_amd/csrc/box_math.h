@@ -147,6 +147,15 @@ struct Bufs {
     float *lambda, *rub, *roll;
 };
 
+// Boxes.inverseInertia, Boxes.java:854-871: the body-frame inverse inertia of a box, or of a ball (shape > 0.5).
+__device__ __forceinline__ void inverse_inertia(const float *half, float im, float shape, float *inverse) {
+    float h2[3];
+    for (int a = 0; a < 3; a++) h2[a] = half[a] * half[a];
+    for (int a = 0; a < 3; a++) inverse[a] = (3.0f * im) / (h2[(a + 1) % 3] + h2[(a + 2) % 3]);
+    if (shape > 0.5f)
+        for (int a = 0; a < 3; a++) inverse[a] = im / (SPHERE_INERTIA * h2[0]);
+}
+
 __device__ Moving load_moving(const Bufs &B, int s) {
     Moving m;
     m.s = s;
@@ -159,15 +168,7 @@ __device__ Moving load_moving(const Bufs &B, int s) {
         m.mom[a] = B.l[a][s];
     }
     for (int k = 0; k < 4; k++) m.q[k] = B.q[k][s];
-    // Boxes.inverseInertia, Boxes.java:854-871.
-    float h2[3];
-    for (int a = 0; a < 3; a++) {
-        float e = B.half[a][s];
-        h2[a] = e * e;
-    }
-    for (int a = 0; a < 3; a++) m.inverse[a] = (3.0f * m.im) / (h2[(a + 1) % 3] + h2[(a + 2) % 3]);
-    if (B.shape[s] > 0.5f)
-        for (int a = 0; a < 3; a++) m.inverse[a] = m.im / (SPHERE_INERTIA * h2[0]);
+    inverse_inertia(m.half, m.im, m.shape, m.inverse);
     m.inverseInertiaTimes(m.mom, m.omega);
     return m;
 }
@@ -185,6 +186,67 @@ __device__ void store_moving_changed(const Bufs &B, const Moving &m, const Movin
         if (bits(m.vel[a]) != bits(was.vel[a])) B.vel[a][m.s] = m.vel[a];
         if (bits(m.mom[a]) != bits(was.mom[a])) B.l[a][m.s] = m.mom[a];
     }
+}
+
+// ---- a body as the sweeps of the pass-by-pass step hold it: two records a body ----
+//
+// frame, 4 float4 a body: (at, im), q, (half, shape), (inverse, 0); fixed over a run of sweeps (positions rewrite
+// it). motion, 2 float4 a body: (vel, 0), (mom, 0); what the sweeps change. A round's bodies lie anywhere in the
+// world, so it gathers each from these records, two cache lines, not from the state's 18 fields.
+constexpr int FRAME_VECTORS = 4, MOTION_VECTORS = 2;
+
+__device__ __forceinline__ Moving load_framed(const float4 *frame, const float4 *motion, int s) {
+    const float4 *f = frame + (size_t)s * FRAME_VECTORS;
+    const float4 *v = motion + (size_t)s * MOTION_VECTORS;
+    float4 a = f[0], b = f[1], c = f[2], d = f[3], e = v[0], g = v[1];
+    Moving m;
+    m.s = s;
+    m.at[0] = a.x, m.at[1] = a.y, m.at[2] = a.z, m.im = a.w;
+    m.q[0] = b.x, m.q[1] = b.y, m.q[2] = b.z, m.q[3] = b.w;
+    m.half[0] = c.x, m.half[1] = c.y, m.half[2] = c.z, m.shape = c.w;
+    m.inverse[0] = d.x, m.inverse[1] = d.y, m.inverse[2] = d.z;
+    m.vel[0] = e.x, m.vel[1] = e.y, m.vel[2] = e.z;
+    m.mom[0] = g.x, m.mom[1] = g.y, m.mom[2] = g.z;
+    m.inverseInertiaTimes(m.mom, m.omega);
+    return m;
+}
+
+// The state's fields with the motion record's velocity and momentum (the walls sweep, whose neighbouring lanes hold
+// neighbouring bodies, so the fields' loads coalesce).
+__device__ __forceinline__ Moving load_moved(const Bufs &B, const float4 *motion, int s) {
+    const float4 *v = motion + (size_t)s * MOTION_VECTORS;
+    float4 e = v[0], g = v[1];
+    Moving m;
+    m.s = s;
+    m.im = B.im[s];
+    m.shape = B.shape[s];
+    for (int a = 0; a < 3; a++) {
+        m.at[a] = B.x[a][s];
+        m.half[a] = B.half[a][s];
+    }
+    for (int k = 0; k < 4; k++) m.q[k] = B.q[k][s];
+    m.vel[0] = e.x, m.vel[1] = e.y, m.vel[2] = e.z;
+    m.mom[0] = g.x, m.mom[1] = g.y, m.mom[2] = g.z;
+    inverse_inertia(m.half, m.im, m.shape, m.inverse);
+    m.inverseInertiaTimes(m.mom, m.omega);
+    return m;
+}
+
+__device__ __forceinline__ void store_motion(float4 *motion, const Moving &m) {
+    float4 *v = motion + (size_t)m.s * MOTION_VECTORS;
+    v[0] = make_float4(m.vel[0], m.vel[1], m.vel[2], 0.f);
+    v[1] = make_float4(m.mom[0], m.mom[1], m.mom[2], 0.f);
+}
+
+// store_motion for a body loaded as `was`: a vector none of whose words changed is left as it is.
+__device__ __forceinline__ void store_motion_changed(float4 *motion, const Moving &m, const Moving &was) {
+    float4 *v = motion + (size_t)m.s * MOTION_VECTORS;
+    if ((bits(m.vel[0]) != bits(was.vel[0])) | (bits(m.vel[1]) != bits(was.vel[1])) |
+        (bits(m.vel[2]) != bits(was.vel[2])))
+        v[0] = make_float4(m.vel[0], m.vel[1], m.vel[2], 0.f);
+    if ((bits(m.mom[0]) != bits(was.mom[0])) | (bits(m.mom[1]) != bits(was.mom[1])) |
+        (bits(m.mom[2]) != bits(was.mom[2])))
+        v[1] = make_float4(m.mom[0], m.mom[1], m.mom[2], 0.f);
 }
 
 // Boxes.Soft, Boxes.java:421-450.

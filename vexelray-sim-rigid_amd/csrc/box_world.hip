@@ -69,6 +69,7 @@ struct BoxWorld {
     int32_t *pinned = nullptr;  // host: the colouring's counts read back
     int colourSteps = 1;        // steps of the colouring the last step needed
     int roundPairs[MOST_ROUNDS] = {};  // pairs in each round this step
+    int roundBase[MOST_ROUNDS] = {};   // its first entry in the colouring's order (the packed entries)
 
     template <class T>
     T *alloc(int64_t count) {
@@ -151,6 +152,8 @@ struct BoxWorld {
         D.roll = alloc<float>((int64_t)RUB_WORDS * n);
         D.shown = alloc<float>((int64_t)SHOWN_STRIDE * n);
         D.cornerHeld = alloc<uint64_t>(n);
+        D.frame = alloc<float4>((int64_t)FRAME_VECTORS * n);
+        D.motion = alloc<float4>((int64_t)MOTION_VECTORS * n);
         D.wallPlace = alloc<int32_t>(n);
         wallPlaceNext = alloc<int32_t>(n);
         reg("lambda", D.lambda, (int64_t)LAMBDA_WORDS * n, VR_F32, LAMBDA_WORDS);
@@ -300,6 +303,24 @@ struct BoxWorld {
     }
 
     // Room for this step's points (at most 4 a touching pair), grown as needed.
+    // Packed fields for `touching` pairs, grown by half again when short (the stream is idle: the colouring has just
+    // read its counts back).
+    void pack_room(int touching) {
+        if (touching <= D.packCap) return;
+        int64_t cap = std::max<int64_t>((int64_t)touching + touching / 2, 1024);
+        HIP_OK(hipStreamSynchronize(stream));
+        for (void *p : {(void *)D.packPair, (void *)D.packCount, (void *)D.packBody, (void *)D.packNormal,
+                        (void *)D.packAnchor, (void *)D.packImpulse})
+            if (p) HIP_OK(hipFree(p));
+        HIP_OK(hipMalloc((void **)&D.packPair, sizeof(int32_t) * cap));
+        HIP_OK(hipMalloc((void **)&D.packCount, sizeof(int32_t) * cap));
+        HIP_OK(hipMalloc((void **)&D.packBody, sizeof(int32_t) * 2 * cap));
+        HIP_OK(hipMalloc((void **)&D.packNormal, sizeof(float) * 3 * cap));
+        HIP_OK(hipMalloc((void **)&D.packAnchor, sizeof(float) * 2 * MOST_POINTS * 3 * cap));
+        HIP_OK(hipMalloc((void **)&D.packImpulse, sizeof(float) * MOST_POINTS * POINT_IMPULSE * cap));
+        D.packCap = cap;
+    }
+
     void points_room(int64_t points) {
         if (points <= pointCap) return;
         int64_t cap = std::max<int64_t>(points + points / 4, 1024);
@@ -383,6 +404,7 @@ struct BoxWorld {
         *used = 0;
         for (int r = 0; r < MOST_ROUNDS; r++) {
             roundPairs[r] = pinned[r + 1] - pinned[r];
+            roundBase[r] = pinned[r];
             if (roundPairs[r] > 0) *used = r + 1;
         }
         *touching = pairs;
@@ -396,7 +418,7 @@ struct BoxWorld {
         if (reverse) launch_walls(D, kind, reverse, stream);
         for (int order = 0; order < rounds; order++) {
             int r = reverse ? rounds - 1 - order : order;
-            launch_solve_round(D, kind, reverse, r, roundPairs[r], stream);
+            launch_solve_round(D, kind, reverse, roundBase[r], roundPairs[r], stream);
         }
         if (!reverse) launch_walls(D, kind, reverse, stream);
     }
@@ -432,13 +454,22 @@ struct BoxWorld {
         if (sceneStep) {
             launch_scene_step(D, sceneFirst, C.starts, used, d.substeps, d.sweeps, d.symmetric != 0, stream);
         } else {
-            for (int sub = 0; sub < d.substeps; sub++) {
+            if (used > 0) {  // the sweeps' rounds read and write the touching pairs packed in round order
+                pack_room(touching);
+                launch_pack(D, C, touching, stream);
+            }
+            for (int sub = 0; sub < d.substeps; sub++) {  // the sweeps work on the frame and motion records
                 launch_velocities(D, stream);
+                launch_frame_in(D, true, stream);
                 sweep(WARM, false, used);
                 for (int k = 0; k < d.sweeps; k++) sweep(SOLVE, d.symmetric && k % 2 == 1, used);
+                launch_motion_out(D, stream);
                 launch_positions(D, stream);
+                launch_frame_in(D, false, stream);
                 for (int k = 0; k < d.sweeps; k++) sweep(RELAX, d.symmetric && k % 2 == 1, used);
+                launch_motion_out(D, stream);
             }
+            if (used > 0) launch_unpack(D, touching, stream);
         }
         if (logging && grid && touching > 0) {
             launch_point_targets(D, C, L, touching, stream);  // what the next step's carry would start from
@@ -491,7 +522,8 @@ struct BoxWorld {
         if (stream) (void)hipStreamSynchronize(stream);
         if (pinned) (void)hipHostFree(pinned);
         for (void *p : {(void *)L.pointBase, (void *)L.x, (void *)L.y, (void *)L.t, (void *)L.used, (void *)logX,
-                        (void *)logT})
+                        (void *)logT, (void *)D.packPair, (void *)D.packCount, (void *)D.packBody,
+                        (void *)D.packNormal, (void *)D.packAnchor, (void *)D.packImpulse})
             if (p) (void)hipFree(p);
         for (void *p : allocs) (void)hipFree(p);
         if (t0) (void)hipEventDestroy(t0);
