@@ -699,7 +699,7 @@ __global__ void k_motion_out(BoxDev D) {
 }
 
 // BoxPairs' words of the touching pairs gathered into the packed fields, entry j the j-th of the colouring's order.
-__global__ void k_pack(BoxDev D, Colouring C, int touching) {
+__global__ void k_pack_pairs(BoxDev D, Colouring C, int touching) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= touching) return;
     size_t T = (size_t)D.packCap;
@@ -725,7 +725,7 @@ __global__ void k_pack(BoxDev D, Colouring C, int touching) {
 }
 
 // The packed impulses back to their pairs' slots: the points a pair has (the rounds touch no others).
-__global__ void k_unpack(BoxDev D, int touching) {
+__global__ void k_unpack_pairs(BoxDev D, int touching) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= touching) return;
     size_t T = (size_t)D.packCap;
@@ -1312,7 +1312,7 @@ void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int base, int 
 void launch_pack(const BoxDev &D, const Colouring &C, int touching, hipStream_t s) {
     if (touching <= 0) return;
     if (touching > D.packCap) throw std::runtime_error("pack: more touching pairs than the packed fields hold");
-    k_pack<<<blocks_for(touching), THREADS, 0, s>>>(D, C, touching);
+    k_pack_pairs<<<blocks_for(touching), THREADS, 0, s>>>(D, C, touching);
     check_launch("pack");
 }
 
@@ -1328,7 +1328,7 @@ void launch_motion_out(const BoxDev &D, hipStream_t s) {
 
 void launch_unpack(const BoxDev &D, int touching, hipStream_t s) {
     if (touching <= 0) return;
-    k_unpack<<<blocks_for(touching), THREADS, 0, s>>>(D, touching);
+    k_unpack_pairs<<<blocks_for(touching), THREADS, 0, s>>>(D, touching);
     check_launch("unpack");
 }
 
