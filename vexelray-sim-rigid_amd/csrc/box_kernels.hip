@@ -423,9 +423,15 @@ struct Side {
     bool second;  // this lane holds the pair's second body
 };
 
+// The neighbour's value: lanes 2k and 2k + 1 swap, a DPP quad permutation [1, 0, 3, 2] (one VALU move, where a
+// general shuffle goes through the LDS unit). Both lanes of a pair are always active together.
+__device__ __forceinline__ float neighbour(float mine) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(mine), 0xB1, 0xF, 0xF, false));
+}
+
 // The first body's and the second's value of a per-body quantity, from this lane's and its neighbour's.
 __device__ __forceinline__ void both(const Side &S, float mine, float &ofFirst, float &ofSecond) {
-    float theirs = __shfl_xor(mine, 1, 64);
+    float theirs = neighbour(mine);
     ofFirst = S.second ? theirs : mine;
     ofSecond = S.second ? mine : theirs;
 }
