@@ -595,6 +595,22 @@ __device__ __forceinline__ bool round_contact(const SphereDev &D, const Params &
     return false;
 }
 
+// DPP lane permutations (GFX9 encodings): shifts within each row of 16 lanes, the broadcast of a row's last lane to
+// the rows after it, and shifts of the whole wave by one lane.
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
+constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143, DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+
+// One step of a max scan over 64-bit values: the value CTRL brings to this lane (the lane's own where CTRL brings
+// none or the lane's row is not in ROWS), taken when `take`.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ long long scan_step(long long most, bool take) {
+    int lo = (int)(unsigned)(unsigned long long)most, hi = (int)(unsigned)((unsigned long long)most >> 32);
+    int l = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xF, false);
+    int h = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xF, false);
+    long long t = (long long)(((unsigned long long)(unsigned)h << 32) | (unsigned long long)(unsigned)l);
+    return take ? max(most, t) : most;
+}
+
 // The claims of a wave's still-open contacts (`claims`): each clears both its spheres' words in the third claim
 // array and raises both its spheres' claims to its priority (Spheres.java:1121-1130). Contacts are listed by lower
 // sphere a, so a wave's lanes hold runs of one a side by side; clearing a word twice is clearing it once, and atomicMax
@@ -608,18 +624,25 @@ __device__ __forceinline__ void claim_wave(const SphereDev &D, const RoundConst 
     int lane = threadIdx.x & 63;
     long long mine = claims ? priority(D, a, o, rc.dispatch) : 0;
     int key = claims ? a : -1 - lane;  // a lane that does not claim is a run of its own
-    // Both shuffles run in every lane, before any test: a shuffle inside a short-circuit would run with lanes off,
-    // and a lane reading from a lane that is off reads nothing it can use.
-    int before = __shfl_up(key, 1, 64), after = __shfl_down(key, 1, 64);
+    // The neighbours' keys, by whole-wave DPP shifts (lane 0 and lane 63 keep their own and are decided by the lane
+    // test). Both run in every lane, before any test.
+    int before = __builtin_amdgcn_update_dpp(key, key, DPP_WAVE_SHR1, 0xF, 0xF, false);
+    int after = __builtin_amdgcn_update_dpp(key, key, DPP_WAVE_SHL1, 0xF, 0xF, false);
     bool head = lane == 0 || before != key;
     bool tail = lane == 63 || after != key;
     unsigned long long heads = __ballot(head) & ((2ull << lane) - 1ull);  // lane 63: every lane
     int start = 63 - __clzll((long long)heads);
+    // Segmented inclusive max scan: within each row of 16 lanes by shifts of 1, 2, 4 and 8, then across rows (the
+    // last lane of row 0 to row 1 and of row 2 to row 3, then the last lane of row 1 to rows 2 and 3), each step
+    // taking a lane's value only from a lane of its own run. A lane the step does not feed keeps its own value, and
+    // the maximum with itself changes nothing. Every lane ends with the maximum over its run up to itself.
     long long most = mine;
-    for (int d = 1; d < 64; d <<= 1) {
-        long long t = __shfl_up(most, d, 64);
-        if (lane - d >= start) most = max(most, t);
-    }
+    most = scan_step<DPP_ROW_SHR1, 0xF>(most, lane - 1 >= start);
+    most = scan_step<DPP_ROW_SHR2, 0xF>(most, lane - 2 >= start);
+    most = scan_step<DPP_ROW_SHR4, 0xF>(most, lane - 4 >= start);
+    most = scan_step<DPP_ROW_SHR8, 0xF>(most, lane - 8 >= start);
+    most = scan_step<DPP_ROW_BCAST15, 0xA>(most, (lane & 16) != 0 && (lane & ~15) - 1 >= start);
+    most = scan_step<DPP_ROW_BCAST31, 0xC>(most, lane >= 32 && 31 >= start);
     if (!claims) return;
     cleared[o] = 0;
     atomicMax(&claimed[o], mine);
@@ -660,7 +683,7 @@ __device__ __forceinline__ void append_open(bool open, const int4 &e, int4 *out,
     int leader = __ffsll((long long)mask) - 1;
     int base = 0;
     if (lane == leader) base = atomicAdd(count, __popcll(mask));
-    base = __shfl(base, leader, 64);
+    base = __builtin_amdgcn_readlane(base, leader);
     if (open) out[base + __popcll(mask & ((1ull << lane) - 1ull))] = e;
 }
 
