@@ -36,6 +36,7 @@ struct BoxDev {
     // thread that sweeps the body (shape_first), so a wave's loads of one word take neighbouring words
     int32_t *wallPlace;
     int32_t *keys, *order, *starts;
+    float4 *broad;  // 2 n: the broad phase's record of each place of the cell order (k_broad_records)
     int32_t *pairs, *pairsWas, *contacts, *rounds;
     float *manifold, *manifoldWas, *impulses, *impulsesWas;
     // n words each, one a body, bit k for the body's slot k, set where the buffer beside it may hold a word other

@@ -844,6 +844,19 @@ __device__ __forceinline__ void reach_of(const BoxDev &D, int s, float *out) {  
         for (int a = 0; a < 3; a++) out[a] = h[0];
 }
 
+// The broad phase's record of the body at place t of the cell order: (centre, the body's index as bits), (its reach
+// along each axis, BoxBroad.reach, 0). Written once a step after the sort, so the list's walk over a cell's bodies
+// reads one record a candidate, neighbouring records for neighbouring places, not 11 state fields a candidate.
+__global__ void k_broad_records(BoxDev D) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.n) return;
+    int s = D.order[t];
+    float reach[3];
+    reach_of(D, s, reach);
+    D.broad[2 * (size_t)t] = make_float4(D.x[0][s], D.x[1][s], D.x[2][s], __int_as_float(s));
+    D.broad[2 * (size_t)t + 1] = make_float4(reach[0], reach[1], reach[2], 0.f);
+}
+
 // BoxBroad.list, BoxBroad.java:103-164: a body's partners of higher index, in ascending order, into its 32 slots.
 __global__ void __launch_bounds__(THREADS) k_list(BoxDev D) {
     // A thread's partners so far, sorted, in LDS: word k of thread i at k * THREADS + i, so lanes at different places
@@ -852,12 +865,13 @@ __global__ void __launch_bounds__(THREADS) k_list(BoxDev D) {
     int *part = parts + threadIdx.x;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= D.n) return;
-    int s = D.order[t];  // threads take bodies in cell order, so a wave walks neighbouring cells together
+    // threads take bodies in cell order, so a wave walks neighbouring cells together
+    const float4 own = D.broad[2 * (size_t)t], ownReach = D.broad[2 * (size_t)t + 1];
+    int s = __float_as_int(own.w);
     size_t base = (size_t)s * SLOTS;
     float margin = D.params[MARGIN];
-    float centre[3] = {D.x[0][s], D.x[1][s], D.x[2][s]};
-    float rs[3];
-    reach_of(D, s, rs);
+    float centre[3] = {own.x, own.y, own.z};
+    float rs[3] = {ownReach.x, ownReach.y, ownReach.z};
     int found = 0;
     const SceneDev sc = D.sceneInfo[D.scene[s]];
     int key = D.keys[s] - sc.cellBase;
@@ -868,13 +882,14 @@ __global__ void __launch_bounds__(THREADS) k_list(BoxDev D) {
         int cell = sc.cellBase + (jx + sc.nx * (jy + sc.ny * jz));
         int end = D.starts[cell + 1];
         for (int k = D.starts[cell]; k < end; k++) {
-            int o = D.order[k];
+            const float4 at = D.broad[2 * (size_t)k];
+            int o = __float_as_int(at.w);
             if (!(o > s)) continue;
-            float other[3];
-            reach_of(D, o, other);
+            const float4 reach = D.broad[2 * (size_t)k + 1];
+            float where[3] = {at.x, at.y, at.z}, other[3] = {reach.x, reach.y, reach.z};
             bool close = true;
             for (int a = 0; a < 3; a++)
-                close = close && !(fabsf(D.x[a][o] - centre[a]) > (rs[a] + other[a]) + margin);
+                close = close && !(fabsf(where[a] - centre[a]) > (rs[a] + other[a]) + margin);
             if (!close) continue;
             if (found < SLOTS) {
                 int place = found;
@@ -1344,6 +1359,7 @@ void launch_bin(const BoxDev &D, hipStream_t s) {
 }
 
 void launch_list(const BoxDev &D, hipStream_t s) {
+    k_broad_records<<<blocks_for(D.n), THREADS, 0, s>>>(D);
     k_list<<<blocks_for(D.n), THREADS, 0, s>>>(D);
     check_launch("box list");
 }

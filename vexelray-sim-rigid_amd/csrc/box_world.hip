@@ -153,6 +153,7 @@ struct BoxWorld {
         D.shown = alloc<float>((int64_t)SHOWN_STRIDE * n);
         D.cornerHeld = alloc<uint64_t>(n);
         D.frame = alloc<float4>((int64_t)FRAME_VECTORS * n);
+        D.broad = alloc<float4>(2 * (int64_t)n);
         D.motion = alloc<float4>((int64_t)MOTION_VECTORS * n);
         D.wallPlace = alloc<int32_t>(n);
         wallPlaceNext = alloc<int32_t>(n);
