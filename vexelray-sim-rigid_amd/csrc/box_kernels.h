@@ -37,6 +37,9 @@ struct BoxDev {
     int32_t *wallPlace;
     int32_t *keys, *order, *starts;
     float4 *broad;  // 2 n: the broad phase's record of each place of the cell order (k_broad_records)
+    // The slots with a partner, listed densely for the narrow phase (launch_collide): collideFirst (n) each body's
+    // first place in collideList (slots), collideTotal (1) how many, collideScan the scan's scratch.
+    int32_t *collideFirst, *collideList, *collideTotal, *collideScan;
     int32_t *pairs, *pairsWas, *contacts, *rounds;
     float *manifold, *manifoldWas, *impulses, *impulsesWas;
     // n words each, one a body, bit k for the body's slot k, set where the buffer beside it may hold a word other

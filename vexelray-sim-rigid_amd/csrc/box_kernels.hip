@@ -989,20 +989,45 @@ __device__ __forceinline__ void store_held(uint32_t *words, int p, bool held) {
     if (p % SLOTS == 0) words[p / SLOTS] = (uint32_t)(mask >> (lane & 32));
 }
 
-__global__ void __launch_bounds__(THREADS) k_collide(BoxDev D) {
-    static_assert(SLOTS == 32 && THREADS % 64 == 0, "a body's slots are half a wave");
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= D.slots) return;  // slots is bodies * 32: a body's slots are all in or all out
-    float *m = D.manifold + (size_t)p * MANIFOLD_WORDS;
-    int partner = D.pairs[2 * (size_t)p + 1];
-    bool held = (D.manifoldHeld[p / SLOTS] >> (p % SLOTS)) & 1u;
-    store_held(D.manifoldHeld, p, partner >= 0);
-    if (partner < 0) {
-        if (held) m[COUNT] = 0;  // a count of 0 already there is left as it is
-        return;
+// The narrow phase over the slots with a partner only: a body's slots 0 .. min(found, 32) - 1 (k_list fills them in
+// order). Per body, a slot that may hold a count and has no partner now gets count 0 (BoxContact.collide's first
+// write; a count of 0 already there is left as it is) and the held word becomes the listed slots; the listed slots'
+// count, scanned, places each in a dense list, so the separating-axis test runs on full waves (a body's 32 slots in
+// half a wave, most without a partner, left most lanes idle through it).
+__global__ void k_collide_bodies(BoxDev D) {
+    static_assert(SLOTS == 32, "a body's slots are one word of held bits");
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    int f = min(max(D.contacts[2 * (size_t)D.slots + s], 0), SLOTS);
+    uint32_t listed = f >= SLOTS ? 0xffffffffu : ((1u << f) - 1u);
+    uint32_t stale = D.manifoldHeld[s] & ~listed;
+    while (stale) {
+        int k = __ffs(stale) - 1;
+        stale &= stale - 1;
+        D.manifold[((size_t)s * SLOTS + k) * MANIFOLD_WORDS + COUNT] = 0;
     }
-    m[COUNT] = 0;
-    collide_pair(D, D.pairs[2 * (size_t)p], partner, m);
+    D.manifoldHeld[s] = listed;
+    D.collideFirst[s] = f;
+}
+
+// After the scan of the listed counts: body s's listed slots at collideFirst[s] onwards of the dense list.
+__global__ void k_collide_place(BoxDev D) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= D.n) return;
+    int f = min(max(D.contacts[2 * (size_t)D.slots + s], 0), SLOTS);
+    int first = D.collideFirst[s];
+    for (int k = 0; k < f; k++) D.collideList[first + k] = s * SLOTS + k;
+}
+
+// BoxContact.collide for the listed slots, each thread taking every (grid)-th of the dense list.
+__global__ void __launch_bounds__(THREADS) k_collide(BoxDev D) {
+    int total = *D.collideTotal;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        int p = D.collideList[i];
+        float *m = D.manifold + (size_t)p * MANIFOLD_WORDS;
+        m[COUNT] = 0;
+        collide_pair(D, D.pairs[2 * (size_t)p], D.pairs[2 * (size_t)p + 1], m);
+    }
 }
 
 // The narrow phase alone over given pairs (what the reference's tests run as a collide-only program).
@@ -1364,8 +1389,14 @@ void launch_list(const BoxDev &D, hipStream_t s) {
     check_launch("box list");
 }
 
+// The dense collide's grid: enough workgroups to fill the chip a few times over.
+constexpr int COLLIDE_BLOCKS = 4096;
+
 void launch_collide(const BoxDev &D, hipStream_t s) {
-    k_collide<<<blocks_for(D.slots), THREADS, 0, s>>>(D);
+    k_collide_bodies<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    vr::scan_exclusive(D.collideFirst, D.collideFirst, D.n, D.collideTotal, D.collideScan, s);
+    k_collide_place<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+    k_collide<<<(int)std::min<int64_t>(blocks_for(D.slots), COLLIDE_BLOCKS), THREADS, 0, s>>>(D);
     check_launch("box collide");
 }
 

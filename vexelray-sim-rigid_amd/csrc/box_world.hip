@@ -154,6 +154,9 @@ struct BoxWorld {
         D.cornerHeld = alloc<uint64_t>(n);
         D.frame = alloc<float4>((int64_t)FRAME_VECTORS * n);
         D.broad = alloc<float4>(2 * (int64_t)n);
+        D.collideFirst = alloc<int32_t>(n);
+        D.collideTotal = alloc<int32_t>(1);
+        D.collideScan = alloc<int32_t>(vr::scan_tmp_words(n) + 64);
         D.motion = alloc<float4>((int64_t)MOTION_VECTORS * n);
         D.wallPlace = alloc<int32_t>(n);
         wallPlaceNext = alloc<int32_t>(n);
@@ -171,6 +174,7 @@ struct BoxWorld {
             reg("starts", D.starts, (int64_t)D.cells + 1, VR_I32);
             int64_t slots = D.slots;
             D.pairs = alloc<int32_t>(2 * slots);
+            D.collideList = alloc<int32_t>(slots);
             D.pairsWas = alloc<int32_t>(2 * slots);
             D.manifold = alloc<float>((int64_t)MANIFOLD_WORDS * slots);
             D.manifoldWas = alloc<float>((int64_t)MANIFOLD_WORDS * slots);
