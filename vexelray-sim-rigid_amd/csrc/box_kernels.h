@@ -108,10 +108,12 @@ void launch_colour_steps(const BoxDev &D, const Colouring &C, int touchingPairs,
                          hipStream_t s);
 void launch_colour_finish(const BoxDev &D, const Colouring &C, int touchingPairs, hipStream_t s);
 
-void launch_velocities(const BoxDev &D, hipStream_t s);
+// With `motion`, also the motion records (box_math.h) from the state.
+void launch_velocities(const BoxDev &D, bool motion, hipStream_t s);
 // A world of one scene walls it with the params' box: copied into the scene before each step.
 void launch_scene_from_params(const BoxDev &D, SceneDev *scene, hipStream_t s);
-void launch_positions(const BoxDev &D, hipStream_t s);
+// With `frame`, also the moving bodies' frame records' position and turn.
+void launch_positions(const BoxDev &D, bool frame, hipStream_t s);
 void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s);
 // The wall sweep's places for the bodies' shapes into `place` (n words), as the sweep takes the bodies.
 void launch_wall_place(const BoxDev &D, int32_t *place, hipStream_t s);

@@ -51,9 +51,16 @@ __device__ __forceinline__ void velocity_of(const BoxDev &D, int s) {
     for (int a = 0; a < 3; a++) D.vel[a][s] = D.vel[a][s] + h * g[a];
 }
 
-__global__ void k_velocities(BoxDev D) {
+// With `motion`, the body's motion record is made from the state as well (the sweeps' copy of velocity and momentum).
+__global__ void k_velocities(BoxDev D, int motion) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < D.n) velocity_of(D, s);
+    if (s >= D.n) return;
+    velocity_of(D, s);
+    if (motion) {
+        float4 *v = D.motion + (size_t)s * MOTION_VECTORS;
+        v[0] = make_float4(D.vel[0][s], D.vel[1][s], D.vel[2][s], 0.f);
+        v[1] = make_float4(D.l[0][s], D.l[1][s], D.l[2][s], 0.f);
+    }
 }
 
 // Boxes.positions, Boxes.java:299-342, for body s: the midpoint turn, with the angular momentum as the state.
@@ -79,9 +86,17 @@ __device__ __forceinline__ void position_of(const BoxDev &D, int s) {
     for (int k = 0; k < 4; k++) B.q[k][s] = end[k];
 }
 
-__global__ void k_positions(BoxDev D) {
+// With `frame`, a moving body's frame record takes its new position and turn (the rest of the record is fixed over a
+// step).
+__global__ void k_positions(BoxDev D, int frame) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < D.n) position_of(D, s);
+    if (s >= D.n) return;
+    position_of(D, s);
+    if (frame && D.im[s] > 0) {
+        float4 *f = D.frame + (size_t)s * FRAME_VECTORS;
+        f[0] = make_float4(D.x[0][s], D.x[1][s], D.x[2][s], D.im[s]);
+        f[1] = make_float4(D.q[0][s], D.q[1][s], D.q[2][s], D.q[3][s]);
+    }
 }
 
 // The body this thread takes among the workgroup's THREADS consecutive ones, boxes first and then balls, each in
@@ -135,6 +150,12 @@ __device__ unsigned long long ball_walls(const Bufs &B, Moving &box, const float
                                          size_t n, int place, const Soft &soft, const float *P) {
     const Sweep sweep = (Sweep)SWEEP;
     bool still = !(P[MU_ROLLING] > 0);  // Boxes.roll's turning stop is taken only with rolling friction
+    // Moving::turn by a zero change adds the zero change to the momentum and its inverse-inertia product to the spin;
+    // that product depends only on the turn and the inertia, which the sweep leaves as they are, so it is worked out
+    // once for all of the ball's quiet walls.
+    const float none[3] = {0.f, 0.f, 0.f};
+    float spinOfNone[3];
+    box.inverseInertiaTimes(none, spinOfNone);
     for (int order = 0; order < WALLS; order++) {
         int wall = REVERSE ? WALLS - 1 - order : order;
         int axis = wall / 2;
@@ -147,8 +168,10 @@ __device__ unsigned long long ball_walls(const Bufs &B, Moving &box, const float
         float gap = high ? pick(extent, axis) - reach : reach;
         bool quiet = !((held >> wall) & 1ull);
         if (quiet && (sweep == WARM || (still && !(gap < soft.margin)))) {
-            const float none[3] = {0.f, 0.f, 0.f};
-            box.turn(none);
+            for (int a = 0; a < 3; a++) {  // box.turn(none)
+                box.mom[a] = box.mom[a] + none[a];
+                box.omega[a] = box.omega[a] + spinOfNone[a];
+            }
             continue;
         }
         size_t at = (size_t)firstSlot * n + place;
@@ -1293,13 +1316,13 @@ void launch_scene_from_params(const BoxDev &D, SceneDev *scene, hipStream_t s) {
     check_launch("scene from params");
 }
 
-void launch_velocities(const BoxDev &D, hipStream_t s) {
-    k_velocities<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+void launch_velocities(const BoxDev &D, bool motion, hipStream_t s) {
+    k_velocities<<<blocks_for(D.n), THREADS, 0, s>>>(D, motion ? 1 : 0);
     check_launch("velocities");
 }
 
-void launch_positions(const BoxDev &D, hipStream_t s) {
-    k_positions<<<blocks_for(D.n), THREADS, 0, s>>>(D);
+void launch_positions(const BoxDev &D, bool frame, hipStream_t s) {
+    k_positions<<<blocks_for(D.n), THREADS, 0, s>>>(D, frame ? 1 : 0);
     check_launch("positions");
 }
 

@@ -464,13 +464,13 @@ struct BoxWorld {
                 launch_pack(D, C, touching, stream);
             }
             for (int sub = 0; sub < d.substeps; sub++) {  // the sweeps work on the frame and motion records
-                launch_velocities(D, stream);
-                launch_frame_in(D, true, stream);
+                // The frame records are made whole at the first substep; later only positions change them.
+                launch_velocities(D, sub > 0, stream);
+                if (sub == 0) launch_frame_in(D, true, stream);
                 sweep(WARM, false, used);
                 for (int k = 0; k < d.sweeps; k++) sweep(SOLVE, d.symmetric && k % 2 == 1, used);
                 launch_motion_out(D, stream);
-                launch_positions(D, stream);
-                launch_frame_in(D, false, stream);
+                launch_positions(D, true, stream);
                 for (int k = 0; k < d.sweeps; k++) sweep(RELAX, d.symmetric && k % 2 == 1, used);
                 launch_motion_out(D, stream);
             }
