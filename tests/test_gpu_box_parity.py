@@ -82,7 +82,8 @@ def run_both(make, steps):
         for _ in range(steps):
             r = s.sim.stepAndWait()
             reps.append((r.touching, r.points, r.rounds, r.unplaced, r.unlisted))
-        out[backend] = ({k: s.floats(k).copy() for k in STATE + WALL_IMPULSES}, reps, s.ints("contacts").copy(),
+        out[backend] = ({k: s.floats(k).copy() for k in STATE + WALL_IMPULSES + ("shown",)}, reps,
+                        s.ints("contacts").copy(),
                         s.ints("rounds").copy())
         s.close()
     return out["oracle"], out["gpu"]
@@ -98,6 +99,8 @@ def test_mixed_pile_matches_oracle(spheres, rolling):
     for k in WALL_IMPULSES:
         assert np.any(sa[k] != 0) or (k == "roll" and not (spheres and rolling)), k  # rolling resistance is for balls
         assert np.array_equal(sa[k].view(np.int32), sb[k].view(np.int32)), k
+    assert np.any(sa["shown"] != 0)
+    assert np.array_equal(sa["shown"].view(np.int32), sb["shown"].view(np.int32)), "shown"  # Boxes.show's record
     for k in STATE:
         scale = np.maximum(np.abs(sa[k]), 1e-3)
         assert float(np.max(np.abs(sa[k] - sb[k]) / scale)) <= 1e-4, k

@@ -1126,21 +1126,20 @@ __global__ void __launch_bounds__(THREADS) k_carry_counts(BoxDev D) {
     D.contacts[(size_t)D.slots + p] = count;
 }
 
-// Boxes.show, Boxes.java:801-820.
+// Boxes.show, Boxes.java:801-820. A body's 20 words are five 16-byte vectors, read and written whole (words 7 and 11,
+// which the show leaves as they are, go back as they were): neighbouring lanes' records are 80 bytes apart, and a
+// word at a time made each store touch a cache line a lane.
 __global__ void k_show(BoxDev D) {
+    static_assert(SHOWN_STRIDE == 20, "five 16-byte vectors a body");
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= D.n) return;
-    float *b = D.shown + (size_t)s * SHOWN_STRIDE;
-    for (int a = 0; a < 3; a++) {
-        b[a] = b[4 + a];
-        b[4 + a] = D.x[a][s];
-        b[8 + a] = D.half[a][s];
-    }
-    for (int k = 0; k < 4; k++) {
-        b[12 + k] = b[16 + k];
-        b[16 + k] = D.q[k][s];
-    }
-    b[3] = D.shape[s];
+    float4 *b = reinterpret_cast<float4 *>(D.shown + (size_t)s * SHOWN_STRIDE);
+    float4 at = b[1], size = b[2], turn = b[4];  // words 4-7, 8-11, 16-19 as they were
+    b[0] = make_float4(at.x, at.y, at.z, D.shape[s]);
+    b[1] = make_float4(D.x[0][s], D.x[1][s], D.x[2][s], at.w);
+    b[2] = make_float4(D.half[0][s], D.half[1][s], D.half[2][s], size.w);
+    b[3] = turn;
+    b[4] = make_float4(D.q[0][s], D.q[1][s], D.q[2][s], D.q[3][s]);
 }
 
 }  // namespace
