@@ -23,6 +23,7 @@
  *                     laid out as Spheres.params (core/.../sphere/Spheres.java:173-226); for boxes 18 words laid
  *                     out as Boxes.params (core/.../box/Boxes.java:212-234), BoxSimulation.params.
  *   vr_world_friction SphereStepper.friction(boolean) (core/.../sphere/SphereStepper.java:110-112).
+ *   vr_world_slice    SphereSimulation.slice(int) (gui/.../SphereSimulation.java:136-148).
  *   vr_world_step     SphereStepper.step(Runner) -> Report (SphereStepper.java:66-103) as driven by
  *                     SphereSimulation.step()/stepAndWait() (SphereSimulation.java:190-229); for boxes
  *                     BoxStepper.step(Runner) -> Report (core/.../box/BoxStepper.java:56-100) as driven by
@@ -109,6 +110,13 @@ typedef struct vr_step_report {
     int32_t reserved;
     int64_t wall_ns;
     int64_t gpu_ns;
+    /*
+     * SphereSimulation.StepReport.submissions (SphereSimulation.java:48, submit :231-248): the submissions the
+     * reference makes for the same step, each list its stepper runs cut into pieces of at most vr_world_slice's
+     * dispatches. For boxes, BoxSimulation's count (BoxSimulation.java:116-142): one a list, no slicing.
+     */
+    int32_t submissions;
+    int32_t unused;
 } vr_step_report;
 
 /*
@@ -147,6 +155,13 @@ int vr_world_read(vr_world *w, const char *name, void *dst, int64_t count);
 int vr_world_buffer_info(vr_world *w, const char *name, int64_t *count, int32_t *dtype);
 int vr_world_params(vr_world *w, const int32_t *words, int32_t n);
 int vr_world_friction(vr_world *w, int32_t on);
+/*
+ * SphereSimulation.slice(int) (gui/.../SphereSimulation.java:136-148): at most `dispatches` dispatches a submission,
+ * 0 for no bound; below 0 is VR_ERR_ARGUMENT. This library enqueues a step's kernels on one stream, each its own
+ * dispatch, whatever the slice: the slice changes only the submissions the report counts, as it changes nothing in
+ * the reference's results. A box world has no slice (VR_ERR_STATE).
+ */
+int vr_world_slice(vr_world *w, int32_t dispatches);
 int vr_world_step(vr_world *w, int32_t wait, vr_step_report *report);
 int vr_world_finish(vr_world *w);
 int vr_world_destroy(vr_world *w);

@@ -1270,6 +1270,7 @@ struct BoxWorld {
             rep->rounds = used;
             rep->most_rounds = used;
             rep->touching = touching;
+            rep->submissions = slots > 0 ? 2 : 1;  // BoxStepper.step: collide read back, then the step
             rep->points = points;
             rep->unplaced = unplaced;
             rep->unlisted = unlisted;

@@ -105,6 +105,15 @@ struct World {
     int n = 0, cap = 0, cells = 0;
     bool gs = false, grid = false, wide = false;
     bool friction = true;
+    int slice = 0;  // SphereSimulation.slice (SphereSimulation.java:136-148)
+
+    // SphereSimulation.submit (SphereSimulation.java:231-248) over SphereStep's lists (SphereStep.java:146-208,
+    // 216-243, 348-392): predict 1; prepare = bin, CountingSort.scan (SupirVast's, not in the reference: taken as
+    // three passes), order, and for Gauss-Seidel warmWalls, keepContacts, clearContacts, listContacts; velocity,
+    // bounce, bounced; walls, show and a report 1 each. A list is cut into pieces of at most `slice` dispatches.
+    static constexpr int SCAN_PASSES = 3, VELOCITY_PASSES = 3;
+    int prepare_passes() const { return grid ? 1 + SCAN_PASSES + 1 + (gs ? 4 : 0) : 0; }
+    int pieces(int passes) const { return slice == 0 ? 1 : (passes + slice - 1) / slice; }
     std::vector<int> opening, enough;  // SphereStepper.java:52-53
 
     std::map<std::string, std::vector<float>> F;
@@ -976,7 +985,7 @@ struct World {
 
     // SphereStepper.step, SphereStepper.java:66-103, with SphereStep.opening/more/closing (:348-392).
     void step_until_done(vr_step_report *rep) {
-        int waits = 0, rounds = 0, most = 0, unlisted = 0, rubbed = 0;
+        int waits = 0, rounds = 0, most = 0, unlisted = 0, rubbed = 0, submissions = 0;
         int limit = cap + 2 * CYCLE;
         for (int sub = 0; sub < d.substeps; sub++) {
             for (int it = 0; it < d.iterations; it++) {
@@ -994,12 +1003,17 @@ struct World {
                 }
                 int listed = 0;
                 int open = add_rounds(it, 0, k, &listed);
+                // SphereStep.opening: the rubbing rounds, the walls before (or the substep's close, predict and
+                // prepare), the rounds and a report.
+                int prefix = it > 0 ? 1 : (sub > 0 ? 1 + VELOCITY_PASSES : 0) + 1 + prepare_passes();
+                submissions += pieces(rubbed + prefix + k + 1);
                 waits++;
                 while (open != 0) {
                     if (k > limit)
                         throw StateError("a pass is still open after " + std::to_string(k) +
                                          " rounds, on a list of " + std::to_string(listed));
                     open = add_rounds(it, k, BATCH, &listed);
+                    submissions += pieces(BATCH + 1);  // SphereStep.more
                     k += BATCH;
                     waits++;
                 }
@@ -1016,6 +1030,7 @@ struct World {
         show();
         if (rep) {
             rep->waits = waits;
+            rep->submissions = submissions + pieces(rubbed + 1 + VELOCITY_PASSES + 1);  // SphereStep.closing
             rep->rounds = rounds;
             rep->most_rounds = most;
             rep->unlisted = unlisted;
@@ -1068,8 +1083,15 @@ struct World {
     void step(vr_step_report *rep) {
         if (rep) std::memset(rep, 0, sizeof(*rep));
         auto t0 = std::chrono::steady_clock::now();
-        if (gs && !d.fixed_rounds) step_until_done(rep);
-        else step_list();
+        if (gs && !d.fixed_rounds) {
+            step_until_done(rep);
+        } else {
+            step_list();
+            int iteration = gs ? d.rounds + 2 : 2;  // SphereStep.step(), one list
+            if (rep)
+                rep->submissions =
+                    pieces(d.substeps * (1 + prepare_passes() + d.iterations * iteration + VELOCITY_PASSES) + 1);
+        }
         auto t1 = std::chrono::steady_clock::now();
         if (rep) rep->wall_ns = std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
     }
@@ -1300,6 +1322,16 @@ int vro_world_friction(vro_world *p, int32_t on) {
     return VR_OK;
 }
 
+int vro_world_slice(vro_world *p, int32_t dispatches) {  // SphereSimulation.java:136-148
+    if (p->kind == VR_BOXES || !p->batch.empty())
+        return fail(VR_ERR_STATE, "a box world submits each list whole: it has no slice");
+    if (dispatches < 0)
+        return fail(VR_ERR_ARGUMENT,
+                    "a slice is some dispatches, or zero for no bound; got " + std::to_string(dispatches));
+    p->w.slice = dispatches;
+    return VR_OK;
+}
+
 int vro_world_step(vro_world *p, int32_t wait, vr_step_report *rep) {
     (void)wait;
     if (!p->batch.empty()) {
@@ -1316,6 +1348,7 @@ int vro_world_step(vro_world *p, int32_t wait, vr_step_report *rep) {
                 r->unlisted += one.unlisted;
                 r->rounds = std::max(r->rounds, one.rounds);
                 r->most_rounds = std::max(r->most_rounds, one.most_rounds);
+                r->submissions = std::max(r->submissions, one.submissions);
             }
         });
     }

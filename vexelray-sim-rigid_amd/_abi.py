@@ -24,7 +24,7 @@ VR_I32 = 2
 
 # Every entry point the header declares, for the export check.
 ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "world_buffer_info", "world_params",
-                "world_friction", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
+                "world_friction", "world_slice", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
                 "world_profile_read", "box_collide", "batch_create", "mlp_create", "mlp_destroy", "mlp_read",
                 "mlp_write", "mlp_forward", "mlp_gradient", "mlp_train", "mlp_last_error", "box_log",
                 "box_samples", "box_learned", "world_diagnostics")
@@ -43,7 +43,7 @@ class StepReport(ctypes.Structure):
     _fields_ = [("waits", ctypes.c_int32), ("rounds", ctypes.c_int32), ("most_rounds", ctypes.c_int32),
                 ("unlisted", ctypes.c_int32), ("touching", ctypes.c_int32), ("points", ctypes.c_int32),
                 ("unplaced", ctypes.c_int32), ("reserved", ctypes.c_int32), ("wall_ns", ctypes.c_int64),
-                ("gpu_ns", ctypes.c_int64)]
+                ("gpu_ns", ctypes.c_int64), ("submissions", ctypes.c_int32), ("unused", ctypes.c_int32)]
 
 
 class Profile(ctypes.Structure):
@@ -71,6 +71,7 @@ class Abi:
                                    ctypes.POINTER(ctypes.c_int32)]),
             "world_params": ([p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
             "world_friction": ([p, ctypes.c_int32]),
+            "world_slice": ([p, ctypes.c_int32]),
             "world_step": ([p, ctypes.c_int32, ctypes.POINTER(StepReport)]),
             "world_finish": ([p]),
             "world_destroy": ([p]),
@@ -193,6 +194,9 @@ class World:
 
     def friction(self, on: bool) -> None:
         self._check(self.abi.fn("world_friction")(self.handle, 1 if on else 0))
+
+    def slice(self, dispatches: int) -> None:
+        self._check(self.abi.fn("world_slice")(self.handle, int(dispatches)))
 
     def step(self, wait: bool) -> StepReport:
         rep = StepReport()

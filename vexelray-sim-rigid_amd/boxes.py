@@ -199,6 +199,7 @@ class BoxSimulation:
         rep = self._world.step(True)
         self._last = BoxReport(rep.touching, rep.points, rep.rounds, rep.unplaced, rep.unlisted)
         self.wall_ns, self.gpu_ns = rep.wall_ns, rep.gpu_ns
+        self.submissions = rep.submissions  # StepReport.submissions (BoxSimulation.java:116-142)
         self.learnedPoints = rep.reserved
         return self._last
 

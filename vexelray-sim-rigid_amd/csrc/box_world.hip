@@ -502,6 +502,7 @@ struct BoxWorld {
         rep->rounds = used;
         rep->most_rounds = used;
         rep->touching = touching;
+        rep->submissions = grid ? 2 : 1;  // BoxStepper.step: the collide list read back, then the step's list
         rep->points = points;
         rep->unplaced = unplaced;
         rep->unlisted = unlisted;

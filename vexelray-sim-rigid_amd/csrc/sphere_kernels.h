@@ -61,6 +61,7 @@ struct PassCtl {
     int k;  // rounds the last parting pass ran, as the reference's stepper counts them
     int error, errRounds, errListed;
     int waits, rounds, most, unlisted;  // SphereStepper.Report, added up over the step
+    int submissions;                    // the reference's submissions for the step's opening and more lists
     long long listedSum;                // contacts listed, added up over every pass (parting and rubbing)
     int wideRan[MAX_WIDE];              // per pass of a substep: the most rounds run wide this step
 };
@@ -88,8 +89,16 @@ void launch_list(const SphereDev &D, const SortScratch &S, hipStream_t s);
 void launch_round(const SphereDev &D, const RoundConst &rc, hipStream_t s);
 // A pass run until done (parting) or for the parting pass's rounds (rubbing), with no host wait: `wide` rounds of
 // the whole chip while many contacts are open, then one workgroup finishes the pass.
+// How a parting pass's lists count as the reference's submissions (SphereSimulation.submit, SphereSimulation.java:
+// 231-248, of the lists SphereStep.opening and more make, SphereStep.java:348-381): `prefix` passes open its opening
+// list before its rounds, after the rubbing rounds of the pass before when `carry`; pieces of at most `slice`
+// dispatches (0 for no bound).
+struct PassSubmit {
+    int prefix, carry, slice;
+};
+
 void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLists &L, int32_t *opening,
-                 int32_t *enough, int it, int iterations, bool rub, int wide, hipStream_t s);
+                 int32_t *enough, int it, int iterations, bool rub, int wide, const PassSubmit &submit, hipStream_t s);
 void launch_step_begin(PassCtl *ctl, PassSlot *slots, int nslots, hipStream_t s);
 void launch_walls(const SphereDev &D, hipStream_t s);
 void launch_velocity(const SphereDev &D, hipStream_t s);

@@ -693,6 +693,7 @@ __global__ void k_step_begin(PassCtl *ctl, PassSlot *slots, int words) {
     if (q == 0) {
         ctl->error = 0;
         ctl->waits = 0;
+        ctl->submissions = 0;
         ctl->rounds = 0;
         ctl->most = 0;
         ctl->unlisted = 0;
@@ -772,7 +773,7 @@ constexpr int TAIL = 512;
 template <bool RUB>
 __global__ void __launch_bounds__(TAIL) k_round_tail(SphereDev D, PassCtl *ctl, const PassSlot *slot, OpenLists L,
                                                      int32_t *opening, int32_t *enough, int it, int iterations,
-                                                     int wide, int limit) {
+                                                     int wide, int limit, PassSubmit submit) {
     __shared__ int sCount;
     int listed = min(D.count[LISTED], D.cap);
     int j = 0, n = listed;
@@ -836,8 +837,14 @@ __global__ void __launch_bounds__(TAIL) k_round_tail(SphereDev D, PassCtl *ctl, 
         ctl->errRounds = k;
         ctl->errListed = D.count[LISTED];
     }
+    // The opening list (the pass before's rubbing rounds, its prefix, its first rounds and a report) and the batches
+    // of `more` after it, each cut into pieces of at most submit.slice dispatches.
+    int more = k > open0 ? (k - open0) / BATCH : 0;
+    int opened = (submit.carry ? ctl->k : 0) + submit.prefix + open0 + 1;
+    int cut = submit.slice;
+    ctl->submissions += cut == 0 ? 1 + more : (opened + cut - 1) / cut + more * ((BATCH + 1 + cut - 1) / cut);
     ctl->k = k;
-    ctl->waits += 1 + (k > open0 ? (k - open0) / BATCH : 0);
+    ctl->waits += 1 + more;
     ctl->rounds += k;
     ctl->most = max(ctl->most, k);
     ctl->unlisted += max(0, D.count[LISTED] - D.cap);
@@ -1374,7 +1381,8 @@ void launch_step_begin(PassCtl *ctl, PassSlot *slots, int nslots, hipStream_t s)
 }
 
 void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLists &L, int32_t *opening,
-                 int32_t *enough, int it, int iterations, bool rub, int wide, hipStream_t s) {
+                 int32_t *enough, int it, int iterations, bool rub, int wide, const PassSubmit &submit,
+                 hipStream_t s) {
     if (wide < 0 || wide > MAX_WIDE) throw std::runtime_error("pass: wide rounds out of range");
     if ((int64_t)L.segments * SEG < D.cap) throw std::runtime_error("pass: open lists shorter than the contact list");
     int limit = D.cap + 2 * CYCLE;
@@ -1389,8 +1397,10 @@ void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLis
                           : (j == 0 ? k_round_wide<false, true> : k_round_wide<false, false>);
         kernel<<<L.segments, THREADS, 0, s>>>(D, ctl, slot, in, segIn, out, segOut, it, iterations, j, last);
     }
-    if (rub) k_round_tail<true><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide, limit);
-    else k_round_tail<false><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide, limit);
+    if (rub)
+        k_round_tail<true><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide, limit, submit);
+    else
+        k_round_tail<false><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide, limit, submit);
     check_launch("pass");
 }
 

@@ -74,6 +74,7 @@ struct vr_world {
     int32_t *devOpening = nullptr, *devEnough = nullptr;  // SphereStepper's opening batches, kept on the device
     std::vector<int> wide, wideRub;    // wide rounds to launch per pass, from what the last step used
     bool hostWaits = false;            // run the reference's launch model: host reads each report
+    int slice = 0;                     // SphereSimulation.slice: most dispatches a submission, 0 for no bound
     double *diagPartials = nullptr, *diagOut = nullptr;  // device scratch of vr_world_diagnostics, made on first use
     bool profiling = false;
     vr_profile prof{};
@@ -331,7 +332,26 @@ struct vr_world {
         }
     }
 
-    void pass(int q, int it, bool rub, int wideRounds) {
+    // The lengths of SphereStep's lists (SphereStep.java:146-208, 216-243, 348-392), which the reference's submissions
+    // count: predict; prepare (bin, the counting sort's scan, order, and for Gauss-Seidel warmWalls, keepContacts,
+    // clearContacts, listContacts); velocity, bounce, bounced; walls, show and a report one dispatch each.
+    // CountingSort.scan's passes belong to the SupirVast library, which is not part of the reference; they are taken
+    // as the three of a two-level scan.
+    static constexpr int SCAN_PASSES = 3, VELOCITY_PASSES = 3;
+    int prepare_passes() const { return grid ? 1 + SCAN_PASSES + 1 + (gs ? 4 : 0) : 0; }
+    // The submissions a list of `passes` dispatches is cut into (SphereSimulation.submit).
+    int pieces(int passes) const { return slice == 0 ? 1 : (passes + slice - 1) / slice; }
+    // What opens pass `it` of substep `sub` before its rounds (SphereStep.opening): the walls of the pass before, or
+    // for a substep's first pass the close of the substep before (walls, velocity) unless it is the first, then
+    // predict and prepare.
+    int opening_prefix(int sub, int it) const {
+        if (it > 0) return 1;
+        return (sub > 0 ? 1 + VELOCITY_PASSES : 0) + 1 + prepare_passes();
+    }
+    // SphereStep.closing: the last pass's rubbing rounds, the walls, velocity and show.
+    int closing_passes(int rubbed) const { return rubbed + 1 + VELOCITY_PASSES + 1; }
+
+    void pass(int q, int it, bool rub, int wideRounds, const PassSubmit &submit = PassSubmit{0, 0, 0}) {
         if (profiling) {
             while ((int)passEvents.size() < 2 * (q + 1)) {
                 hipEvent_t e;
@@ -340,7 +360,8 @@ struct vr_world {
             }
             HIP_OK(hipEventRecord(passEvents[2 * q], stream));
         }
-        launch_pass(D, ctl, slots + q, openLists, devOpening, devEnough, it, d.iterations, rub, wideRounds, stream);
+        launch_pass(D, ctl, slots + q, openLists, devOpening, devEnough, it, d.iterations, rub, wideRounds, submit,
+                    stream);
         launchesThisStep += wideRounds + 1;
         if (profiling) HIP_OK(hipEventRecord(passEvents[2 * q + 1], stream));
     }
@@ -369,7 +390,8 @@ struct vr_world {
                     predict();
                     prepare();
                 }
-                pass(q++, it, false, wide[it]);
+                PassSubmit submit{opening_prefix(sub, it), friction && !(sub == 0 && it == 0) ? 1 : 0, slice};
+                pass(q++, it, false, wide[it], submit);
             }
         }
         if (friction)
@@ -397,6 +419,7 @@ struct vr_world {
             prof.enqueue_ns += enqueued;
         }
         rep->waits = ctlHost->waits;
+        rep->submissions = ctlHost->submissions + pieces(closing_passes(friction ? ctlHost->k : 0));
         rep->rounds = ctlHost->rounds;
         rep->most_rounds = ctlHost->most;
         rep->unlisted = ctlHost->unlisted;
@@ -409,7 +432,7 @@ struct vr_world {
     }
 
     void step_until_done(vr_step_report *rep) {  // SphereStepper.java:66-103
-        int waits = 0, rounds = 0, most = 0, unlisted = 0, rubbed = 0;
+        int waits = 0, rounds = 0, most = 0, unlisted = 0, rubbed = 0, submissions = 0;
         int limit = cap + 2 * CYCLE;
         for (int sub = 0; sub < d.substeps; sub++) {
             for (int it = 0; it < d.iterations; it++) {
@@ -427,12 +450,14 @@ struct vr_world {
                 }
                 int listed = 0;
                 int open = add_rounds(it, 0, k, &listed);
+                submissions += pieces(rubbed + opening_prefix(sub, it) + k + 1);
                 waits++;
                 while (open != 0) {
                     if (k > limit)
                         throw StateError("a pass is still open after " + std::to_string(k) +
                                          " rounds, on a list of " + std::to_string(listed));
                     open = add_rounds(it, k, BATCH, &listed);
+                    submissions += pieces(BATCH + 1);
                     k += BATCH;
                     waits++;
                 }
@@ -448,6 +473,7 @@ struct vr_world {
         velocity_list();
         launch_show(D, grid, stream);
         rep->waits = waits;
+        rep->submissions = submissions + pieces(closing_passes(rubbed));
         rep->rounds = rounds;
         rep->most_rounds = most;
         rep->unlisted = unlisted;
@@ -497,6 +523,10 @@ struct vr_world {
             else step_device(r);
         } else {
             step_list();
+            // SphereStep.step(), one list: per substep predict, prepare, each iteration's passes (solve and apply,
+            // or the fixed rounds and the walls), velocity; then show.
+            int iteration = gs ? d.rounds + 2 : 2;
+            r->submissions = pieces(d.substeps * (1 + prepare_passes() + d.iterations * iteration + VELOCITY_PASSES) + 1);
         }
         HIP_OK(hipEventRecord(t1, stream));
         if (wait) {
@@ -720,6 +750,16 @@ int vr_world_params(vr_world *w, const int32_t *words, int32_t n) {
 int vr_world_friction(vr_world *w, int32_t on) {
     if (!w) return fail(VR_ERR_ARGUMENT, "null world");
     w->friction = on != 0;
+    return VR_OK;
+}
+
+int vr_world_slice(vr_world *w, int32_t dispatches) {  // SphereSimulation.java:136-148
+    if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    if (w->box) return fail(VR_ERR_STATE, "a box world submits each list whole: it has no slice");
+    if (dispatches < 0)
+        return fail(VR_ERR_ARGUMENT,
+                    "a slice is some dispatches, or zero for no bound; got " + std::to_string(dispatches));
+    w->slice = dispatches;
     return VR_OK;
 }
 

@@ -268,6 +268,12 @@ class SphereSimulation:
         """``SphereStepper.friction`` (SphereStepper.java:110-112)."""
         self._world.friction(on)
 
+    def slice(self, dispatches: int) -> None:
+        """``SphereSimulation.slice`` (SphereSimulation.java:136-148): at most ``dispatches`` dispatches a submission,
+        0 for no bound; a negative count is an IllegalArgumentException. The step's kernels and results are the same
+        whatever the slice; the report's ``submissions`` counts the pieces the reference's lists are cut into."""
+        self._world.slice(dispatches)
+
     def step(self) -> StepReport:
         return self._run(False)
 
@@ -277,7 +283,7 @@ class SphereSimulation:
     def _run(self, wait: bool) -> StepReport:
         rep = self._world.step(wait)
         solve = Report(rep.waits, rep.rounds, rep.most_rounds, rep.unlisted)
-        return StepReport(solve, 1, rep.wall_ns, rep.gpu_ns)
+        return StepReport(solve, rep.submissions, rep.wall_ns, rep.gpu_ns)
 
     def finish(self) -> None:
         self._world.finish()
