@@ -19,6 +19,7 @@
  *                     SphereStep.java:157-160, 224-226, 419).
  *   vr_world_write    PassRunner.write(name, int[]|float[]) (ctest/.../sphere/Scene.java:127-140).
  *   vr_world_read     PassRunner.floats(name) / read(name) (Scene.java:176-204; SphereSimulation.state() :250-254).
+ *   vr_world_start    SphereSimulation.start(...) (SphereSimulation.java:154-178): clear, writes, params, shown.
  *   vr_world_params   SphereSimulation.params(int[]) (SphereSimulation.java:182-185): 13 words of raw float bits
  *                     laid out as Spheres.params (core/.../sphere/Spheres.java:173-226); for boxes 18 words laid
  *                     out as Boxes.params (core/.../box/Boxes.java:212-234), BoxSimulation.params.
@@ -154,6 +155,15 @@ int vr_world_read(vr_world *w, const char *name, void *dst, int64_t count);
 /* Length and element type (VR_F32 / VR_I32) of a named buffer. */
 int vr_world_buffer_info(vr_world *w, const char *name, int64_t *count, int32_t *dtype);
 int vr_world_params(vr_world *w, const int32_t *words, int32_t n);
+/*
+ * SphereSimulation.start(x, y, z, u, v, w, r, im, params) (gui/.../SphereSimulation.java:154-178) for a sphere
+ * world: clear and the step's constants, the eight fields (each as long as the sphere count), the 13 parameter words
+ * (with SphereStepper.friction(Spheres.rubs(params))), and the picture's "before" and "after" both this state. The
+ * same as vr_world_clear, vr_world_write of each field, vr_world_params and vr_world_write of "shown". A box world
+ * starts through those calls (BoxSimulation.start works out the angular momentum on the host): VR_ERR_STATE.
+ */
+int vr_world_start(vr_world *w, const float *x, const float *y, const float *z, const float *u, const float *v,
+                   const float *vw, const float *r, const float *im, const int32_t *params, int32_t nparams);
 int vr_world_friction(vr_world *w, int32_t on);
 /*
  * SphereSimulation.slice(int) (gui/.../SphereSimulation.java:136-148): at most `dispatches` dispatches a submission,

@@ -1317,6 +1317,32 @@ int vro_world_params(vro_world *p, const int32_t *words, int32_t n) {
     return VR_OK;
 }
 
+int vro_world_start(vro_world *p, const float *x, const float *y, const float *z, const float *u, const float *v,
+                    const float *vw, const float *r, const float *im, const int32_t *params, int32_t nparams) {
+    // SphereSimulation.start, SphereSimulation.java:154-178.
+    if (p->kind == VR_BOXES || !p->batch.empty())
+        return fail(VR_ERR_STATE, "a box world starts through clear, writes and params");
+    int64_t n = 0;
+    int32_t dtype = 0;
+    int rc = vro_world_buffer_info(p, "x", &n, &dtype);
+    if (rc == VR_OK) rc = vro_world_clear(p);
+    const char *names[8] = {"x", "y", "z", "u", "v", "w", "r", "im"};
+    const float *fields[8] = {x, y, z, u, v, vw, r, im};
+    for (int k = 0; k < 8 && rc == VR_OK; k++) rc = vro_world_write(p, names[k], fields[k], n);
+    if (rc == VR_OK) rc = vro_world_params(p, params, nparams);
+    if (rc != VR_OK) return rc;
+    std::vector<float> shown((size_t)SHOWN_STRIDE * (size_t)n, 0.f);
+    for (int64_t s = 0; s < n; s++) {
+        float *b = shown.data() + (size_t)SHOWN_STRIDE * (size_t)s;
+        b[0] = b[4] = x[s];
+        b[1] = b[5] = y[s];
+        b[2] = b[6] = z[s];
+        b[3] = r[s];
+        b[TURNED_BEFORE + 3] = b[TURNED_AFTER + 3] = 1.f;  // the turns as the identity
+    }
+    return vro_world_write(p, "shown", shown.data(), (int64_t)shown.size());
+}
+
 int vro_world_friction(vro_world *p, int32_t on) {
     p->w.friction = on != 0;
     return VR_OK;

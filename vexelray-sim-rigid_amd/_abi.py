@@ -24,7 +24,7 @@ VR_I32 = 2
 
 # Every entry point the header declares, for the export check.
 ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "world_buffer_info", "world_params",
-                "world_friction", "world_slice", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
+                "world_start", "world_friction", "world_slice", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
                 "world_profile_read", "box_collide", "batch_create", "mlp_create", "mlp_destroy", "mlp_read",
                 "mlp_write", "mlp_forward", "mlp_gradient", "mlp_train", "mlp_last_error", "box_log",
                 "box_samples", "box_learned", "world_diagnostics")
@@ -72,6 +72,7 @@ class Abi:
             "world_params": ([p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
             "world_friction": ([p, ctypes.c_int32]),
             "world_slice": ([p, ctypes.c_int32]),
+            "world_start": ([p] + [p] * 8 + [p, ctypes.c_int32]),
             "world_step": ([p, ctypes.c_int32, ctypes.POINTER(StepReport)]),
             "world_finish": ([p]),
             "world_destroy": ([p]),
@@ -194,6 +195,17 @@ class World:
 
     def friction(self, on: bool) -> None:
         self._check(self.abi.fn("world_friction")(self.handle, 1 if on else 0))
+
+    def start(self, fields, params) -> None:
+        """vr_world_start: the eight fields x, y, z, u, v, w, r, im (each as long as the world's bodies) and the
+        parameter words."""
+        count, _ = self.info("x")
+        arrs = [np.ascontiguousarray(f, dtype=np.float32) for f in fields]
+        if len(arrs) != 8 or any(a.shape != (count,) for a in arrs):
+            raise self.arg_error(f"a start takes 8 fields of {count} floats")
+        words = np.ascontiguousarray(params, dtype=np.int32)
+        self._check(self.abi.fn("world_start")(self.handle, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs],
+                                               words.ctypes.data_as(ctypes.c_void_p), int(words.size)))
 
     def slice(self, dispatches: int) -> None:
         self._check(self.abi.fn("world_slice")(self.handle, int(dispatches)))

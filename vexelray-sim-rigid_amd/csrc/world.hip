@@ -747,6 +747,34 @@ int vr_world_params(vr_world *w, const int32_t *words, int32_t n) {
     return guard([&] { w->params(words, n); });
 }
 
+int vr_world_start(vr_world *w, const float *x, const float *y, const float *z, const float *u, const float *v,
+                   const float *vw, const float *r, const float *im, const int32_t *params, int32_t nparams) {
+    if (!w || !x || !y || !z || !u || !v || !vw || !r || !im || !params) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (w->box) return fail(VR_ERR_STATE, "a box world starts through clear, writes and params");
+    int64_t n = 0;
+    int32_t dtype = 0;
+    int rc = vr_world_buffer_info(w, "x", &n, &dtype);
+    if (rc == VR_OK) rc = vr_world_clear(w);
+    const char *names[8] = {"x", "y", "z", "u", "v", "w", "r", "im"};
+    const float *fields[8] = {x, y, z, u, v, vw, r, im};
+    for (int k = 0; k < 8 && rc == VR_OK; k++) rc = vr_world_write(w, names[k], fields[k], n);
+    if (rc == VR_OK) rc = vr_world_params(w, params, nparams);
+    if (rc != VR_OK) return rc;
+    // The picture's before and after (Spheres.SHOWN_STRIDE words a sphere): centre and radius, then the turns'
+    // w words (Spheres.TURNED_BEFORE + 3, TURNED_AFTER + 3) as the identity.
+    constexpr int TURNED_BEFORE = 32, TURNED_AFTER = 36;
+    std::vector<float> shown((size_t)SHOWN_STRIDE * (size_t)n, 0.f);
+    for (int64_t s = 0; s < n; s++) {
+        float *b = shown.data() + (size_t)SHOWN_STRIDE * (size_t)s;
+        b[0] = b[4] = x[s];
+        b[1] = b[5] = y[s];
+        b[2] = b[6] = z[s];
+        b[3] = r[s];
+        b[TURNED_BEFORE + 3] = b[TURNED_AFTER + 3] = 1.f;
+    }
+    return vr_world_write(w, "shown", shown.data(), (int64_t)shown.size());
+}
+
 int vr_world_friction(vr_world *w, int32_t on) {
     if (!w) return fail(VR_ERR_ARGUMENT, "null world");
     w->friction = on != 0;

@@ -242,23 +242,20 @@ class SphereSimulation:
         self._world.clear()
 
     def start(self, x, y, z, u, v, w, r, im, params, ax=None, ay=None, az=None, shown=True) -> None:
-        """``SphereSimulation.start`` (SphereSimulation.java:154-178). The angular velocities, which the reference's
-        test harness writes after the constants (ctest/.../sphere/Scene.java:136-138), may be given too."""
-        n = self.step_.spheres
-        self._world.clear()
-        for name, values in (("x", x), ("y", y), ("z", z), ("u", u), ("v", v), ("w", w), ("r", r), ("im", im)):
-            self._world.write(name, np.asarray(values, dtype=np.float32))
-        self.params(params)
+        """``SphereSimulation.start`` (SphereSimulation.java:154-178), as ``vr_world_start``. The angular velocities,
+        which the reference's test harness writes after the constants (ctest/.../sphere/Scene.java:136-138), may be
+        given too; ``shown=False`` leaves the picture's buffer cleared, as that harness does."""
+        fields = tuple(np.asarray(a, dtype=np.float32) for a in (x, y, z, u, v, w, r, im))
+        if shown:
+            self._world.start(fields, np.asarray(params, dtype=np.int32))
+        else:
+            self._world.clear()
+            for name, values in zip(("x", "y", "z", "u", "v", "w", "r", "im"), fields):
+                self._world.write(name, values)
+            self.params(params)
         for name, values in (("ax", ax), ("ay", ay), ("az", az)):
             if values is not None:
                 self._world.write(name, np.asarray(values, dtype=np.float32))
-        if shown:
-            s = np.zeros((n, Spheres.SHOWN_STRIDE), dtype=np.float32)
-            for k, arr in enumerate((x, y, z)):
-                s[:, k] = s[:, 4 + k] = np.asarray(arr, dtype=np.float32)
-            s[:, 3] = np.asarray(r, dtype=np.float32)
-            s[:, Spheres.TURNED_BEFORE + 3] = s[:, Spheres.TURNED_AFTER + 3] = 1
-            self._world.write("shown", s.reshape(-1))
 
     def params(self, params) -> None:
         """``SphereSimulation.params`` (SphereSimulation.java:182-185); friction follows ``Spheres.rubs``."""
