@@ -76,10 +76,10 @@ def test_batch_refuses_bad_scenes():
 @pytest.mark.gpu
 @pytest.mark.parametrize("count", [96, 100])
 def test_many_scenes_match_the_oracle(count):
-    """A batch of many small scenes, which the HIP library steps a workgroup a scene through all of a step's
-    substeps rather than pass by pass over the whole chip, against the oracle's batch, bit for bit, step by step.
-    The 100-scene batch has scenes too big for that (up to 4906 bodies; the scene step takes at most 256) and steps
-    pass by pass, as a check that both ways match."""
+    """Batches against the oracle's, bit for bit, step by step. After a start the HIP library steps a batch once
+    pass by pass over the whole chip, once a workgroup a scene through all of a step's substeps, and keeps the faster,
+    so each run here takes both ways and the switches between them (which move the wall impulses to the other way's
+    places): 96 small scenes, and 100 scenes of up to 4,906 bodies."""
     if count == 96:
         spec = scenes.batched_2k(seed=9, scenes=count, fewest=1, most=60, substeps=3, sweeps=3)
     else:
@@ -103,3 +103,24 @@ def test_many_scenes_match_the_oracle(count):
     for t in range(len(sa)):
         for name in STATE:
             assert np.array_equal(sa[t][name].view(np.int32), sb[t][name].view(np.int32)), (t, name)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["scene", "pass"])
+def test_a_fixed_step_mode_matches_the_oracle(mode, monkeypatch):
+    """VR_BOX_STEP_MODE fixes the batch's step mode (no timing): each way alone, bit for bit against the oracle."""
+    monkeypatch.setenv("VR_BOX_STEP_MODE", mode)
+    spec = scenes.batched_2k(seed=9, scenes=96, fewest=1, most=60, substeps=3, sweeps=3)
+    out = {}
+    for backend in ("oracle", "gpu"):
+        sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps,
+                             abi=sc.abi_for(backend))
+        start(sim, spec)
+        reps = [sim.stepAndWait() for _ in range(4)]
+        out[backend] = ([(r.touching, r.points, r.rounds) for r in reps],
+                        {name: sim.floats(name).copy() for name in STATE})
+        sim.close()
+    (ra, sa), (rb, sb) = out["oracle"], out["gpu"]
+    assert ra == rb
+    for name in sa:
+        assert np.array_equal(sa[name].view(np.int32), sb[name].view(np.int32)), name

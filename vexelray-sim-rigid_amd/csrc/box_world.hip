@@ -12,6 +12,7 @@
 #include <climits>
 #include <cstring>
 #include <stdexcept>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,12 @@ struct BoxWorld {
     // one workgroup, which beats launches over the whole chip for batches of small scenes (2,048 scenes of 16-64
     // bodies: 19 against 32 ms a step) but not once a scene holds thousands (16-4,096 bodies: 207 against 97).
     static constexpr int SCENE_STEP_MOST = 256;
+    // A batch's step mode, tuned: both modes leave every scene as the reference does, so after each start the batch
+    // steps once pass by pass and once a workgroup a scene, each timed by the step's own events, and keeps the faster
+    // (tuneStep counts the steps since the start, up to 2). VR_BOX_STEP_MODE=scene or pass fixes the mode instead.
+    bool tunable = false;
+    int tuneStep = 0;
+    float passMs = 0, sceneMs = 0;
     Colouring C{};
     // The learned warm start (no counterpart in the reference): per-point data, a log of (features, targets), and
     // the network whose prediction starts each step's impulses, when one is set.
@@ -124,6 +131,12 @@ struct BoxWorld {
             most = std::max(most, sceneBodies[k]);
         }
         sceneStep = !wallsFromParams && sceneBodies.size() >= SCENE_STEP_LEAST && most <= SCENE_STEP_MOST;
+        tunable = !wallsFromParams && grid && sceneBodies.size() > 1;
+        if (const char *mode = std::getenv("VR_BOX_STEP_MODE")) {
+            if (std::string(mode) == "scene" && !wallsFromParams) sceneStep = true;
+            if (std::string(mode) == "pass") sceneStep = false;
+            tunable = false;
+        }
         sceneFirst = alloc<int32_t>((int64_t)first.size());
         HIP_OK(hipMemcpy(sceneFirst, first.data(), sizeof(int32_t) * first.size(), hipMemcpyHostToDevice));
         int32_t *sceneIdx = alloc<int32_t>(n);
@@ -250,6 +263,7 @@ struct BoxWorld {
     }
 
     void clear() {  // PassRunner.clear(), then BoxStep.constants(): qw = 1 (BoxStep.java:103-105)
+        tuneStep = 0;  // a new start: the step mode is timed again
         for (auto &kv : named) HIP_OK(hipMemsetAsync(kv.second.p, 0, (size_t)kv.second.count * 4, stream));
         if (grid) {
             size_t slots = D.slots;
@@ -271,21 +285,31 @@ struct BoxWorld {
 
     // The wall sweep's places for the shapes as they are; with `move`, the wall impulse words follow their bodies.
     // A batch stepped a workgroup a scene keeps body order (the scene step sweeps bodies in order).
+    // Either mode's places keep a body within its workgroup's block of places, so k_wall_move can take the words
+    // from one to the other.
     void place_walls(bool move) {
         if (sceneStep) {
-            launch_wall_identity();
-            return;
+            wall_identity(wallPlaceNext);
+        } else {
+            launch_wall_place(D, wallPlaceNext, stream);
         }
-        launch_wall_place(D, wallPlaceNext, stream);
         if (move) launch_wall_move(D, D.wallPlace, wallPlaceNext, stream);
         std::swap(D.wallPlace, wallPlaceNext);
+        HIP_OK(hipStreamSynchronize(stream));
     }
 
-    void launch_wall_identity() {
+    void wall_identity(int32_t *place) {
         std::vector<int32_t> id((size_t)n);
         for (int s = 0; s < n; s++) id[s] = s;
-        HIP_OK(hipMemcpyAsync(D.wallPlace, id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipMemcpyAsync(place, id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
         HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    // Steps from now on a workgroup a scene (or pass by pass), the wall impulses moved to that mode's places.
+    void scene_step(bool on) {
+        if (on == sceneStep) return;
+        sceneStep = on;
+        place_walls(true);
     }
 
     // A manifold or impulse buffer written from outside: any slot may hold words other than zero.
@@ -429,6 +453,7 @@ struct BoxWorld {
     }
 
     void step(vr_step_report *rep) {
+        if (tunable && tuneStep < 3) scene_step(tuneStep == 0 ? false : tuneStep == 1 ? true : sceneMs < passMs);
         auto w0 = std::chrono::steady_clock::now();
         HIP_OK(hipEventRecord(t0, stream));
         if (wallsFromParams) launch_scene_from_params(D, sceneDev, stream);
@@ -492,6 +517,11 @@ struct BoxWorld {
         HIP_OK(hipStreamSynchronize(stream));
         float ms = 0;
         HIP_OK(hipEventElapsedTime(&ms, t0, t1));
+        if (tunable && tuneStep < 3) {
+            if (tuneStep == 0) passMs = ms;
+            if (tuneStep == 1) sceneMs = ms;
+            tuneStep++;
+        }
         rep->reserved = 0;
         if (net && grid && touching > 0) {
             HIP_OK(hipMemcpyAsync(pinned + 48, L.used, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
