@@ -109,3 +109,17 @@ def test_hip_submissions_match_the_oracle(host_waits, monkeypatch):
             dispatches
         for k in STATE:
             assert np.array_equal(got_state[k].view(np.int32), want_state[k].view(np.int32)), (dispatches, k)
+
+
+def test_a_gridless_jacobi_list_is_cut_as_the_reference_cuts_it():
+    """Without a grid a Jacobi step is one list the reference fixes whole (SphereStep.java:146-208): per substep
+    predict, then solve and apply each iteration, then velocity, bounce and bounced; then show. So its submissions
+    under any slice are pinned: ceil(length / slice), one with no slice."""
+    s = sc.crowd()
+    assert s.solve == pkg.Solve.JACOBI and not s.grid
+    length = s.substeps * (1 + 2 * s.iterations + 3) + 1
+    for dispatches, want in ((0, 1), (1, length), (7, -(-length // 7)), (length, 1), (length + 1, 1)):
+        s.start("oracle")
+        s.sim.slice(dispatches)
+        assert s.sim.stepAndWait().submissions == want, dispatches
+        s.close()
