@@ -487,3 +487,34 @@ def test_crowded_body_fills_every_round_and_leaves_the_rest_unplaced(backend):
         # Round k holds the k-th small ball's pair with the big one (slot k * 32), and nothing else.
         assert firsts == [k * 32 for k in range(32)]
         assert all(int(rounds[k * width + 1]) == -1 for k in range(32))
+
+
+def test_the_picture_is_given_each_step_and_the_one_before():
+    """BoxShowTest.thePictureIsGivenEachStepAndTheOneBefore (core/src/test/.../box/BoxShowTest.java:18-52), on the
+    oracle: two boxes, one tumbling in free fall and one fixed; after three steps the "after" words hold the state
+    the step ended on, the "before" words what the step before left there, and the half-extents are each box's own
+    (Boxes.java:781-791: centre before 0, after 4, half 8, turn before 12, after 16). The HIP library's picture is
+    the oracle's bit for bit (test_gpu_box_parity.py)."""
+    centre_before, centre_after, half, turned_before, turned_after = 0, 4, 8, 12, 16
+    with BoxScene(2).uniform(0.05, 0.1, 0.15, 1) as s:
+        s.x[0] = s.y[0] = s.z[0] = 0.5
+        s.ax[0], s.ay[0], s.az[0] = 1, 2, 3
+        s.x[1], s.y[1], s.z[1] = 0.2, 0.1, 0.8
+        s.im[1] = 0
+        s.half[2][1] = 0.05
+        s.start("oracle").advance(2)
+        earlier = s.floats("shown").copy()
+        s.advance(1).read()
+        shown = s.floats("shown")
+        at = (s.x, s.y, s.z)
+        for b in range(2):
+            base = b * Boxes.SHOWN_STRIDE
+            for a in range(3):
+                assert shown[base + centre_after + a] == at[a][b], (b, a)
+                assert shown[base + centre_before + a] == earlier[base + centre_after + a], (b, a)
+                assert shown[base + half + a] == s.half[a][b], (b, a)
+            for k in range(4):
+                assert shown[base + turned_after + k] == s.q[k][b], (b, k)
+                assert shown[base + turned_before + k] == earlier[base + turned_after + k], (b, k)
+        assert shown[centre_after + 1] < shown[centre_before + 1], "the first box is falling"
+        assert shown[turned_after] != shown[turned_before], "and turning"

@@ -584,3 +584,63 @@ def test_oracle_world_rejects_bad_step():
     d.bodies = 0
     with pytest.raises(pkg.IllegalArgumentException):
         abi.create(1, d, pkg.IllegalArgumentException, pkg.IllegalStateException)
+
+
+def lattice(side, seed, spacing, y0, dy):
+    """PileTest's loose lattice (core/src/test/.../sphere/PileTest.java:39-50, 77-88): java.util.Random(seed)."""
+    rnd = sc.pkg.JavaRandom(seed)
+    s = Scene(side ** 3).uniform(0.03, 0.1)
+    k = 0
+    for i in range(side):
+        for j in range(side):
+            for l in range(side):
+                s.x[k] = np.float32(0.07 + spacing * i + 0.005 * rnd.nextDouble())
+                s.y[k] = np.float32(y0 + dy * j)
+                s.z[k] = np.float32(0.07 + spacing * l + 0.005 * rnd.nextDouble())
+                k += 1
+    return s
+
+
+@pytest.mark.parametrize("grid,solve,mu", [(False, JACOBI, 0.0), (True, JACOBI, 0.0), (True, GS, 0.0),
+                                            (True, GS, 0.5)])
+def test_a_dropped_pile_stays_inside_apart_and_loses_energy(grid, solve, mu):
+    """PileTest.aDroppedPileStaysInsideApartAndLosesEnergy (PileTest.java:30-68), on the oracle: 125 spheres dropped
+    into a box for three seconds. Nothing goes non-finite or leaves the box, no sphere passes into another by a fifth
+    of a radius, and the pile ends with less energy than it started with."""
+    with lattice(5, 11, 0.065, 0.2, 0.07) as s:
+        s.sx = s.sz = 0.4
+        s.sy = 1
+        s.substeps = 20
+        s.grid = grid
+        s.solve = solve
+        s.muStatic = s.muKinetic = mu
+        zero = np.zeros(s.n, dtype=np.float32)
+        start = sc.pkg.SphereDiagnostics.of(s.x, s.y, s.z, s.u, s.v, s.w, zero, zero, zero, s.r, s.im, s.gy, s.sx,
+                                            s.sy, s.sz)
+        s.start("oracle")
+        worst = 0.0
+        for second in range(3):
+            state = s.advanceSeconds(1).diagnostics()
+            assert not state.broken, f"non-finite after {second + 1} s"
+            assert state.maxWall < 1e-5, f"outside the box: {state}"
+            worst = max(worst, state.maxOverlap)
+            assert state.kinetic + state.potential < start.kinetic + start.potential, f"energy grew: {state}"
+        assert worst < 0.2, f"spheres passed into each other by {worst} of a radius"
+
+
+def test_a_pile_with_friction_comes_to_rest():
+    """PileTest.aPileWithFrictionComesToRest (PileTest.java:76-103), on the oracle: 343 spheres dropped with
+    friction 0.5 at 10 substeps settle for four seconds, then lose under 5e-3 J a second and keep under 1e-4 J
+    moving."""
+    side = 7
+    with lattice(side, 7, 0.06, 0.3, 0.07) as s:
+        s.solve = GS
+        s.muStatic = s.muKinetic = 0.5
+        s.sx = s.sz = 0.08 + 0.06 * side
+        s.sy = 0.5 + 0.07 * side
+        s.start("oracle").advanceSeconds(4)
+        settled = s.diagnostics().potential
+        later = s.advanceSeconds(2).diagnostics()
+        creep = (settled - later.potential) / 2
+        assert creep < 5e-3, f"still creeping, {creep} J a second: {later}"
+        assert later.kinetic < 1e-4, f"at rest: {later}"
