@@ -43,6 +43,7 @@ class BoxScene:
         self.unplaced = 0
         self.unlisted = 0
         self.last = None
+        self.step = None
         self.sim = None
 
     def sphere(self, s, r):
@@ -62,6 +63,7 @@ class BoxScene:
         widest = float(np.max(np.sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]))) if self.n else 0.0
         step = BoxStep(self.n, self.substeps, self.passes, self.symmetric,
                        BoxBroad.grid(self.sx, self.sy, self.sz, widest, self.margin))
+        self.step = step
         self.sim = BoxSimulation(step, abi=sc.abi_for(backend))
         self.sim.clear()
         self.sim.params(Boxes.params(self.dt / self.substeps, self.gx, self.gy, self.gz, self.sx, self.sy, self.sz,
@@ -114,6 +116,30 @@ class BoxScene:
         self.read()
         return BoxDiagnostics.of(self.x, self.y, self.z, self.u, self.v, self.w, self.l, self.q, self.half,
                                  self.shape, self.im)
+
+    def ownSpin(self, s):
+        """Body s's spin in its own frame, as the state was last read (BoxScene.java's ownSpin)."""
+        return BoxDiagnostics.rotate(BoxDiagnostics.conjugate(self.q[:, s].astype(np.float64)),
+                                     [float(self.ax[s]), float(self.ay[s]), float(self.az[s])])
+
+    def axis(self, s, k):
+        """Body s's own axis k in the world's frame, as the state was last read."""
+        unit = [0.0, 0.0, 0.0]
+        unit[k] = 1.0
+        return BoxDiagnostics.rotate(self.q[:, s].astype(np.float64), unit)
+
+    def slots(self):
+        return self.step.slots
+
+    def deepest(self):
+        """The deepest point of any manifold the last step left (the pile tests' own sweep over the slots)."""
+        m = self.floats("manifold")
+        most = 0.0
+        for p in range(self.slots()):
+            at = p * BoxContact.MANIFOLD_WORDS
+            for c in range(int(m[at + BoxContact.COUNT])):
+                most = max(most, float(m[at + BoxContact.POINTS + c * BoxContact.POINT_WORDS + BoxContact.DEPTH]))
+        return most
 
     def floats(self, name):
         return self.sim.floats(name)

@@ -1,0 +1,437 @@
+"""The box world's physical known answers, the rest of the reference's box test cases, on the oracle.
+
+ctest/box/... = vexelray-sim-rigid-core/src/test/java/dev/vexelray/sim/rigid/box/... Each case is the reference's own
+scene and bound: free spins (BoxFreeTest), a box on tilted floors (BoxWallTest), balls with boxes (BoxSphereTest) and
+stacks and piles (BoxStackTest). Closed forms where the reference gives one: the Dzhanibekov half period (Landau and
+Lifshitz, Mechanics, section 37), a sliding ball rolling at 5/7 of its speed, rolling resistance slowing at 5/7 mu_r g,
+a box sliding down a slope at g (sin t - mu cos t), a tall box tipping where tan t passes its width over its height.
+
+They run on the oracle, which is what they pin. The HIP library is held to the oracle bit for bit on seeded scenes of
+the same kinds (test_gpu_box_parity.py, test_gpu_full_size.py). BoxFreeTest.theExplicitGyroscopicTermGrowsTheAngular-
+Momentum is not ported: it integrates a form the step does not use, in host doubles, and touches neither world.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.special
+
+import scene as sc
+from boxscene import BoxScene, Boxes
+
+pkg = sc.pkg
+G = 9.81
+HX, HY, HZ = 0.05, 0.1, 0.2
+
+
+def lean(unit):
+    """How far a unit vector leans from upright, from its horizontal part (BoxWallTest.java:166-172)."""
+    return math.asin(min(1.0, math.hypot(unit[0], unit[2])))
+
+
+def tilt_floor(s, degrees):
+    """The floor tilted by turning gravity (BoxWallTest.java:162-166)."""
+    theta = math.radians(degrees)
+    s.gx = G * math.sin(theta)
+    s.gy = -G * math.cos(theta)
+
+
+def axis_angle(axis, angle):
+    length = math.sqrt(sum(a * a for a in axis))
+    sin = math.sin(angle / 2) / length
+    return [axis[0] * sin, axis[1] * sin, axis[2] * sin, math.cos(angle / 2)]
+
+
+# ---- BoxFreeTest.java ----
+
+def elliptic_k(k2):
+    """K(k) for k^2 < 1 by the arithmetic-geometric mean, pi / (2 AGM(1, sqrt(1 - k^2))) (BoxFreeTest.java:
+    227-237)."""
+    a, b = 1.0, math.sqrt(1 - k2)
+    for _ in range(64):
+        if abs(a - b) <= 1e-15 * a:
+            break
+        a, b = (a + b) / 2, math.sqrt(a * b)
+    return math.pi / (2 * a)
+
+
+def half_period(moment, omega):
+    """Half the period of a torque-free spin about the intermediate axis (BoxFreeTest.java:199-222): for
+    I1 < I2 < I3 and M^2 > 2E I2, 2 K(k) sqrt(I1 I2 I3 / ((I3 - I2)(M^2 - 2E I1))) with
+    k^2 = (I2 - I1)(2E I3 - M^2) / ((I3 - I2)(M^2 - 2E I1)); for M^2 < 2E I2 the first and third swap."""
+    i1, i2, i3 = sorted(moment)
+    m2 = sum((moment[a] * omega[a]) ** 2 for a in range(3))
+    two_e = sum(moment[a] * omega[a] * omega[a] for a in range(3))
+    if m2 < two_e * i2:
+        i1, i3 = i3, i1
+    k2 = (i2 - i1) * (two_e * i3 - m2) / ((i3 - i2) * (m2 - two_e * i1))
+    return 2 * elliptic_k(k2) * math.sqrt(i1 * i2 * i3 / ((i3 - i2) * (m2 - two_e * i1)))
+
+
+def test_the_elliptic_integral_is_right():
+    """BoxFreeTest.theEllipticIntegralIsRight (:241-245), and against scipy's own K over the range."""
+    assert abs(math.pi / 2 - elliptic_k(0)) <= 1e-15
+    assert abs(1.8540746773013719 - elliptic_k(0.5)) <= 1e-14
+    for k2 in (0.1, 0.9, 0.999, 1 - 1e-8):
+        assert abs(elliptic_k(k2) - scipy.special.ellipk(k2)) <= 1e-12 * scipy.special.ellipk(k2), k2
+
+
+def test_the_half_period_is_the_integrated_one():
+    """The closed form the flip test is held to, against the torque-free Euler equations integrated in doubles
+    (scipy's RK45 at a tight tolerance): the first sign change of the spin about the intermediate axis comes a half
+    period after the spin passes through its other extreme, so the gap between two sign changes is that half period."""
+    from scipy.integrate import solve_ivp
+
+    moment = Boxes.inertia(Boxes.BOX_SHAPE, 1.0, HX, HY, HZ)
+
+    def euler(_, w):
+        l = [moment[a] * w[a] for a in range(3)]
+        return [-(w[1] * l[2] - w[2] * l[1]) / moment[0], -(w[2] * l[0] - w[0] * l[2]) / moment[1],
+                -(w[0] * l[1] - w[1] * l[0]) / moment[2]]
+
+    def flip(_, w):
+        return w[1]
+
+    out = solve_ivp(euler, (0, 12), [0, 5, 0.1], rtol=1e-11, atol=1e-12, events=flip, method="DOP853")
+    crossings = out.t_events[0]
+    assert len(crossings) >= 3
+    expected = half_period(moment, [0, 5, 0.1])
+    assert abs(expected - 3.403) < 1e-3  # the reference's figure for this body and spin
+    for k in range(1, len(crossings)):
+        assert abs(crossings[k] - crossings[k - 1] - expected) <= 1e-6 * expected
+
+
+def test_a_nudged_spin_about_a_stable_axis_stays_near_it():
+    """BoxFreeTest.aNudgedSpinAboutAStableAxisStaysNearIt (:78-104): 5 rad/s about the smallest or the largest axis
+    and 0.1 rad/s about each other, 10 s; the angle between the spin and the axis starts at 0.028 rad and stays under
+    0.056."""
+    for axis in (0, 2):
+        with BoxScene(1).uniform(HX, HY, HZ, 1) as s:
+            s.gy = 0
+            s.x[0] = s.y[0] = s.z[0] = 0.5
+            for a, spin in enumerate((s.ax, s.ay, s.az)):
+                spin[0] = 5 if a == axis else np.float32(0.1)
+            s.start("oracle")
+            widest = 0.0
+            for _ in range(600):
+                own = s.advance(1).read().ownSpin(0)
+                widest = max(widest, math.acos(min(1.0, abs(own[axis]) / np.linalg.norm(own))))
+            assert widest < 0.056, (axis, widest)
+
+
+def test_a_spin_about_the_intermediate_axis_turns_over_at_its_period():
+    """BoxFreeTest.aSpinAboutTheIntermediateAxisTurnsOverAtItsPeriod (:150-193): 5 rad/s about the intermediate axis
+    and 0.1 rad/s about the smallest, 12 s; the sign changes of the spin about the intermediate axis, placed between
+    frames by linear interpolation, are the closed form's half period apart to 0.5%."""
+    moment = Boxes.inertia(Boxes.BOX_SHAPE, 1.0, HX, HY, HZ)
+    omega = (0.0, 5.0, 0.1)
+    expected = half_period(moment, omega)
+    with BoxScene(1).uniform(HX, HY, HZ, 1) as s:
+        s.gy = 0
+        s.x[0] = s.y[0] = s.z[0] = 0.5
+        s.ay[0] = omega[1]
+        s.az[0] = np.float32(omega[2])
+        s.start("oracle")
+        crossings = []
+        was = omega[1]
+        for frame in range(1, 721):
+            now = s.advance(1).read().ownSpin(0)[1]
+            if np.sign(now) != np.sign(was):
+                crossings.append((frame - 1 + was / (was - now)) * s.dt)
+            was = now
+        assert len(crossings) >= 3, crossings
+        for k in range(1, len(crossings)):
+            assert abs(crossings[k] - crossings[k - 1] - expected) <= 0.005 * expected, (k, crossings, expected)
+
+
+# ---- BoxWallTest.java ----
+
+def test_a_box_dropped_on_a_corner_settles_on_a_face():
+    """BoxWallTest.aBoxDroppedOnACornerSettlesOnAFace (:43-75): 0.1 x 0.2 x 0.4 m, turned 0.8 rad about
+    (1, 0.3, 0.7), dropped from 0.3 m, mu 0.5; after 3 s one axis is upright to 1e-3 rad, the box rests at that
+    half-extent's height to 1e-3 m, under 1e-5 J."""
+    with BoxScene(1).uniform(0.05, 0.1, 0.2, 1) as s:
+        s.sx = s.sz = 2
+        s.x[0] = s.z[0] = 1
+        s.y[0] = np.float32(0.3)
+        s.q[:, 0] = np.array(axis_angle((1, 0.3, 0.7), 0.8), dtype=np.float32)
+        s.muStatic = s.muKinetic = 0.5
+        st = s.start("oracle").advanceSeconds(3).diagnostics()
+        up = 0
+        for k in (1, 2):
+            if abs(s.axis(0, k)[1]) > abs(s.axis(0, up)[1]):
+                up = k
+        assert lean(s.axis(0, up)) < 1e-3, up
+        assert abs(s.half[up][0] - s.y[0]) <= 1e-3
+        assert st.kinetic < 1e-5
+
+
+def on_a_slope(degrees, mu):
+    """The flat box resting on a floor tilted by degrees, room to slide along x (BoxWallTest.java:150-160)."""
+    s = BoxScene(1).uniform(0.1, 0.05, 0.1, 1)
+    tilt_floor(s, degrees)
+    s.sx = 10
+    s.x[0] = 1
+    s.z[0] = np.float32(0.5)
+    s.y[0] = np.float32(0.05)
+    s.muStatic = s.muKinetic = mu
+    return s
+
+
+def test_a_box_rests_on_a_slope_its_friction_holds():
+    """BoxWallTest.aBoxRestsOnASlopeItsFrictionHolds (:77-97): 20 degrees, mu 0.5 > tan 20; settled within 1e-5 m
+    of where it was put, then still to 1e-5 m and 1e-5 m/s from 1 s to 3 s."""
+    with on_a_slope(20, 0.5) as s:
+        settled = float(s.start("oracle").advanceSeconds(1).read().x[0])
+        s.advanceSeconds(2).read()
+        assert abs(1 - settled) <= 1e-5
+        assert abs(settled - s.x[0]) <= 1e-5
+        assert abs(s.u[0]) <= 1e-5
+
+
+def test_a_box_slides_down_a_slope_its_friction_cannot_hold():
+    """BoxWallTest.aBoxSlidesDownASlopeItsFrictionCannotHold (:99-119): 30 degrees, mu 0.3 < tan 30; it speeds up
+    at g (sin t - mu cos t) to 1%, read between 0.5 s and 1 s, and does not turn (1e-3 rad)."""
+    theta = math.radians(30)
+    with on_a_slope(30, 0.3) as s:
+        early = float(s.start("oracle").advanceSeconds(0.5).read().u[0])
+        s.advanceSeconds(0.5).read()
+        rate = (s.u[0] - early) / 0.5
+        expected = G * (math.sin(theta) - 0.3 * math.cos(theta))
+        assert abs(expected - rate) <= 0.01 * expected, (rate, expected)
+        assert lean(s.axis(0, 1)) < 1e-3
+
+
+@pytest.mark.parametrize("degrees", [12, 16])
+def test_a_tall_box_tips_where_its_weight_leaves_its_base(degrees):
+    """BoxWallTest.aTallBoxTipsWhereItsWeightLeavesItsBase (:121-148): 0.1 x 0.4 x 0.1 m, mu 1; it tips past
+    atan(0.1 / 0.4) = 14.04 degrees. At 12 it stands upright to 1e-6, under 1e-6 J; at 16 it has fallen."""
+    with BoxScene(1).uniform(0.05, 0.2, 0.05, 1) as s:
+        tilt_floor(s, degrees)
+        s.sx = s.sz = 2
+        s.x[0] = s.z[0] = 1
+        s.y[0] = np.float32(0.2)
+        s.muStatic = s.muKinetic = 1
+        st = s.start("oracle").advanceSeconds(2).diagnostics()
+        upright = s.axis(0, 1)[1]
+        if degrees < 14:
+            assert upright > 1 - 1e-6, upright
+            assert st.kinetic < 1e-6
+        else:
+            assert upright < 0.5, upright
+
+
+# ---- BoxSphereTest.java ----
+
+def test_a_sliding_ball_settles_into_rolling_on_a_box():
+    """BoxSphereTest.aSlidingBallSettlesIntoRollingOnABox (:41-69): a 5 cm ball sliding at 2 m/s on a fixed slab,
+    mu 0.3; after 1 s it rolls at 5/7 of 2 m/s, with spin v / r, to 1%, on the slab to 5e-4 m."""
+    with BoxScene(2).uniform(1.8, 0.05, 0.3, 1) as s:
+        s.sphere(1, 0.05)
+        s.im[0] = 0
+        s.muStatic = s.muKinetic = 0.3
+        s.sx, s.sz = 4, 1
+        s.x[0] = 2
+        s.y[0] = np.float32(0.05)
+        s.z[0] = s.z[1] = np.float32(0.5)
+        s.x[1] = np.float32(0.4)
+        s.y[1] = np.float32(0.15)
+        s.u[1] = 2
+        s.start("oracle").advanceSeconds(1).read()
+        rolling = 2 * 5.0 / 7
+        assert abs(rolling - s.u[1]) <= 0.01 * rolling
+        assert abs(-rolling / 0.05 - s.az[1]) <= 0.01 * rolling / 0.05
+        assert abs(0.15 - s.y[1]) <= 5e-4
+
+
+@pytest.mark.parametrize("on_slab", [False, True])
+def test_a_rolling_ball_slows_at_five_sevenths_mu_rolling_g(on_slab):
+    """BoxSphereTest.aRollingBallSlowsAtFiveSeventhsMuRollingG (:71-108): a 5 cm ball rolling at 1 m/s, mu_r 0.1,
+    on the floor (the walls' contact) or a fixed slab (a pair's): 1 - 0.5 (5/7) 0.1 g m/s after 0.5 s to 1e-2, still
+    rolling to 0.2 rad/s."""
+    n = 2 if on_slab else 1
+    with BoxScene(n).uniform(1.8, 0.05, 0.3, 1) as s:
+        ball = n - 1
+        s.sphere(ball, 0.05)
+        s.muStatic = s.muKinetic = 0.5
+        s.muRolling = 0.1
+        s.sx, s.sz = 4, 1
+        if on_slab:
+            s.im[0] = 0
+            s.x[0] = 2
+            s.y[0] = np.float32(0.05)
+            s.z[0] = np.float32(0.5)
+        s.x[ball] = np.float32(0.4)
+        s.y[ball] = np.float32(0.15 if on_slab else 0.05)
+        s.z[ball] = np.float32(0.5)
+        s.u[ball] = 1
+        s.az[ball] = -20
+        s.start("oracle").advanceSeconds(0.5).read()
+        expected = 1 - 0.5 * 5.0 / 7 * 0.1 * G
+        assert abs(expected - s.u[ball]) <= 1e-2, (s.u[ball], expected)
+        assert abs(-s.u[ball] / 0.05 - s.az[ball]) <= 0.2
+
+
+def test_a_ball_into_a_box_keeps_momentum():
+    """BoxSphereTest.aBallIntoABoxKeepsMomentum (:110-130): 1 kg at 2 m/s into a free 1 kg box, no gravity; momentum
+    2 to 1e-6, both at 1 m/s to 1e-2."""
+    with BoxScene(2).uniform(0.05, 0.05, 0.05, 1) as s:
+        s.sphere(0, 0.05)
+        s.gy = 0
+        s.sx = s.sy = s.sz = 2
+        s.x[0] = np.float32(0.7)
+        s.x[1] = 1
+        s.y[:] = 1
+        s.z[:] = 1
+        s.u[0] = 2
+        st = s.start("oracle").advanceSeconds(0.5).diagnostics()
+        assert abs(2 - st.momentum[0]) <= 1e-6
+        assert abs(1 - s.u[0]) <= 1e-2 and abs(1 - s.u[1]) <= 1e-2
+
+
+def test_a_box_rests_on_four_balls():
+    """BoxSphereTest.aBoxRestsOnFourBalls (:132-160): a 20 x 10 x 20 cm box on four fixed 5 cm balls; it rests at
+    0.15 m to 5e-4, level to 1e-4, under 1e-6 J."""
+    with BoxScene(5).uniform(0.05, 0.05, 0.05, 1) as s:
+        for k, (x, z) in enumerate(((0.42, 0.58), (0.58, 0.42), (0.42, 0.42), (0.58, 0.58))):
+            s.sphere(k, 0.05)
+            s.im[k] = 0
+            s.x[k], s.z[k], s.y[k] = np.float32(x), np.float32(z), np.float32(0.05)
+        s.half[0][4] = s.half[2][4] = np.float32(0.1)
+        s.x[4] = s.z[4] = np.float32(0.5)
+        s.y[4] = np.float32(0.151)
+        s.muStatic = s.muKinetic = 0.5
+        st = s.start("oracle").advanceSeconds(2).diagnostics()
+        up = s.axis(4, 1)
+        assert abs(0.15 - s.y[4]) <= 5e-4
+        assert math.hypot(up[0], up[2]) < 1e-4
+        assert st.kinetic < 1e-6
+
+
+def test_a_mixed_pile_rests():
+    """BoxSphereTest.aMixedPileRests (:162-207): 64 balls and 64 cubes of 8 cm, alternating, jittered by
+    java.util.Random(3), mu 0.5, mu_r 0.3; after 4 s under 1e-5 J, no point deeper than 2 mm, nothing unplaced or
+    unlisted."""
+    side = 4
+    n = 2 * side ** 3
+    rnd = pkg.JavaRandom(3)
+    with BoxScene(n).uniform(0.04, 0.04, 0.04, 0.5) as s:
+        s.muStatic = s.muKinetic = 0.5
+        s.muRolling = 0.3
+        s.sx = s.sz = 1
+        s.sy = 2.5
+        k = 0
+        for i in range(side):
+            for j in range(2 * side):
+                for l in range(side):
+                    if k % 2 == 0:
+                        s.sphere(k, 0.04)
+                    s.x[k] = np.float32(0.35 + 0.1 * i + 0.02 * rnd.nextDouble())
+                    s.y[k] = np.float32(0.3 + 0.1 * j)
+                    s.z[k] = np.float32(0.35 + 0.1 * l + 0.02 * rnd.nextDouble())
+                    k += 1
+        st = s.start("oracle").advanceSeconds(4).diagnostics()
+        assert st.kinetic < 1e-5
+        assert s.deepest() < 2e-3
+        assert s.unplaced == 0
+        assert s.unlisted == 0
+
+
+# ---- BoxStackTest.java ----
+
+def column(n, most_drift, most_energy):
+    """BoxStackTest.column (:234-261): n cubes of 10 cm, 1 mm apart, mu 0.5, 3 s."""
+    s = BoxScene(n).uniform(0.05, 0.05, 0.05, 1)
+    s.sx = s.sz = 1
+    s.sy = 0.11 * n + 0.5
+    for k in range(n):
+        s.x[k] = s.z[k] = np.float32(0.5)
+        s.y[k] = np.float32(0.05 + 0.101 * k)
+    s.muStatic = s.muKinetic = 0.5
+    with s:
+        st = s.start("oracle").advanceSeconds(3).diagnostics()
+        assert math.hypot(s.x[n - 1] - 0.5, s.z[n - 1] - 0.5) < most_drift
+        assert st.kinetic < most_energy
+        assert s.unplaced == 0
+    return s
+
+
+def test_a_column_of_twenty_stands():
+    """BoxStackTest.aColumnOfTwentyStands (:54-62): the top within 2 cm, under 1e-4 J."""
+    column(20, 2e-2, 1e-4)
+
+
+@pytest.mark.parametrize("f", [0.95, 1.05])
+def test_the_harmonic_overhang_stands_short_of_its_limit_and_falls_past_it(f):
+    """BoxStackTest.theHarmonicOverhangStandsShortOfItsLimitAndFallsPastIt (:64-112): six blocks 40 cm long, each
+    past the one below by f L / 2k, k counted from the top. At 0.95 no block drops 2 mm or slides 1 mm in 3 s and the
+    stack is still (1e-6 J); at 1.05 a block drops more than 5 cm."""
+    n, length = 6, 0.4
+    with BoxScene(n).uniform(length / 2, 0.025, 0.1, 1) as s:
+        s.muStatic = s.muKinetic = 0.5
+        s.sx, s.sz, s.sy = 1.6, 0.5, 1
+        for i in range(n):
+            s.x[i] = np.float32(0.3) if i == 0 else np.float32(float(s.x[i - 1]) + f * length / (2 * (n - i)))
+            s.y[i] = np.float32(0.025 + 0.05 * i)
+            s.z[i] = np.float32(0.25)
+        x0, y0 = s.x.copy(), s.y.copy()
+        st = s.start("oracle").advanceSeconds(3).diagnostics()
+        drop = float(np.max(y0.astype(np.float64) - s.y))
+        slide = float(np.max(np.abs(s.x.astype(np.float64) - x0)))
+        if f < 1:
+            assert drop < 2e-3 and slide < 1e-3, (drop, slide)
+            assert st.kinetic < 1e-6
+        else:
+            assert drop > 0.05, drop
+
+
+def test_a_pyramid_of_ten_stands():
+    """BoxStackTest.aPyramidOfTenStandsOnTheCpu (:114-182): 55 cubes, ten at the base, each row half a cube along,
+    0.5 mm apart; after 3 s no cube has moved 1 mm, under 1e-6 J, nothing unplaced."""
+    base = 10
+    n = base * (base + 1) // 2
+    with BoxScene(n).uniform(0.05, 0.05, 0.05, 1) as s:
+        s.muStatic = s.muKinetic = 0.5
+        s.sx, s.sz, s.sy = 1.6, 0.5, 1.5
+        k = 0
+        for row in range(base):
+            for c in range(base - row):
+                s.x[k] = np.float32(0.3 + 0.1005 * (c + row / 2.0))
+                s.y[k] = np.float32(0.05 + 0.1 * row)
+                s.z[k] = np.float32(0.25)
+                k += 1
+        x0, y0 = s.x.astype(np.float64), s.y.astype(np.float64)
+        st = s.start("oracle").advanceSeconds(3).diagnostics()
+        moved = float(np.max(np.hypot(s.x - x0, s.y - y0)))
+        assert moved < 1e-3, moved
+        assert st.kinetic < 1e-6
+        assert s.unplaced == 0
+
+
+def test_a_pile_of_343_rests():
+    """BoxStackTest.aPileOf343RestsOnTheCpu (:184-232): 343 cubes of 8 cm, half a kilogram, a loose block 7 a side
+    jittered and turned up to 0.6 rad by java.util.Random(7), mu 0.5; after 4 s under 1e-6 J, no point deeper than
+    1 mm, nothing unplaced or unlisted."""
+    side = 7
+    n = side ** 3
+    rnd = pkg.JavaRandom(7)
+    with BoxScene(n).uniform(0.04, 0.04, 0.04, 0.5) as s:
+        s.muStatic = s.muKinetic = 0.5
+        s.sx = s.sz = 0.1 * side + 0.4
+        s.sy = 0.1 * side + 1.5
+        k = 0
+        for i in range(side):
+            for j in range(side):
+                for l in range(side):
+                    s.x[k] = np.float32(0.25 + 0.1 * i + 0.02 * rnd.nextDouble())
+                    s.y[k] = np.float32(0.3 + 0.1 * j)
+                    s.z[k] = np.float32(0.25 + 0.1 * l + 0.02 * rnd.nextDouble())
+                    axis = [rnd.nextDouble() - 0.5, rnd.nextDouble() - 0.5, rnd.nextDouble() - 0.5]
+                    s.q[:, k] = np.array(axis_angle(axis, 0.6 * rnd.nextDouble()), dtype=np.float32)
+                    k += 1
+        st = s.start("oracle").advanceSeconds(4).diagnostics()
+        assert st.kinetic < 1e-6
+        assert s.deepest() < 1e-3
+        assert s.unplaced == 0
+        assert s.unlisted == 0
