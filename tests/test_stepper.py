@@ -70,6 +70,33 @@ def test_a_pass_goes_on_a_batch_at_a_time_until_nothing_is_open():
         assert r.waits == 1 + (r.rounds - BATCH) // BATCH
 
 
+def listed_pairs(s):
+    """Every pair the list takes at the given positions, by brute force in f32: nearer than the radii's sum times
+    (1 + LIST_MARGIN) (Spheres.java:795 and the list pass)."""
+    m = np.float32(1) + np.float32(pkg.Spheres.LIST_MARGIN)
+    count = 0
+    for a in range(s.n):
+        d = [(c[a] - c[a + 1:]).astype(np.float32) for c in (s.x, s.y, s.z)]
+        d2 = d[0] * d[0] + (d[1] * d[1] + d[2] * d[2])
+        near = (s.r[a] + s.r[a + 1:]) * m
+        count += int(np.count_nonzero(d2 < near * near))
+    return count
+
+
+def test_contacts_the_list_had_no_room_for_are_reported():
+    """SphereStepperTest.contactsTheListHadNoRoomForAreReported (:154-160): each pass adds what its list found past
+    the list's capacity, so a substep of two passes reports it twice. The crowd is still and has no gravity, so the
+    first substep lists the pairs of its starting positions, counted here by brute force."""
+    s = spheres(1, 2, crowded=True)
+    s.contactsPerSphere = 1
+    listed = listed_pairs(s)
+    assert listed > s.n, "the list should overflow"
+    with s.start("oracle"):
+        r = s.sim.stepAndWait().solve
+        assert int(s.ints("contactCount")[pkg.Spheres.LISTED]) == listed
+        assert r.unlisted == 2 * (listed - s.n)
+
+
 def test_the_opening_grows_to_what_a_pass_needed_and_shrinks_slowly():
     """SphereStepperTest.theOpeningGrowsToWhatAPassNeededAndShrinksSlowly (:98-113): one substep of two passes. A
     crowded first step grows each pass's opening to the rounds it ran (at most CYCLE). Moved apart, the spheres need
