@@ -227,6 +227,42 @@ def test_on_a_slope_rolls_or_slides(backend):
         assert abs(-2.5 * mu * G * math.cos(theta) * t / 0.1 - s.az[0]) <= 0.05
 
 
+@pytest.mark.parametrize("mu_static,mu_kinetic", [(0.6, 0.2), (0.2, 0.6)])
+def test_a_slide_is_slowed_by_kinetic_friction_alone(mu_static, mu_kinetic):
+    """Static and kinetic friction apart, which no reference test sets apart (Spheres.friction, Spheres.java:
+    595-654), on the oracle: a sphere sliding at 2 m/s on the floor slows at mu_k g and spins up at 5/2 mu_k g / r
+    until it rolls, whatever mu_s is. Read at 0.05 s, before it rolls at either mu_k."""
+    v0, t = 2.0, 0.05
+    with _on_the_floor(0) as s:
+        s.muStatic, s.muKinetic = mu_static, mu_kinetic
+        s.u[0] = v0
+        s.start("oracle").advanceSeconds(t).read()
+        assert abs(v0 - mu_kinetic * G * t - s.u[0]) <= 1e-2
+        assert abs(-2.5 * mu_kinetic * G * t / 0.1 - s.az[0]) <= 0.1
+
+
+@pytest.mark.parametrize("mu_static,mu_kinetic,rolls", [(0.3, 0.05, True), (0.1, 0.05, False), (0.05, 0.1, False),
+                                                      (0.05, 0.3, True)])
+def test_on_a_slope_friction_decides_whether_it_rolls(mu_static, mu_kinetic, rolls):
+    """On a 30 degree slope rolling without slipping takes (2/7) tan 30 = 0.165 of the normal impulse, at 5/7 g sin t.
+    Friction holds it whole within mu_s, else cuts it to mu_k, but never past what stops the slip (Spheres.friction's
+    min(1, mu_k lambda_n / |whole|), Spheres.java:636-641). So the sphere rolls when either mu reaches 0.165 and
+    otherwise slides at g (sin t - mu_k cos t). On the oracle, to 1e-2 after 0.5 s."""
+    theta, t = math.radians(30), 0.5
+    with _on_the_floor(0) as s:
+        s.muStatic, s.muKinetic = mu_static, mu_kinetic
+        s.x[0] = 0.2
+        s.gx = G * math.sin(theta)
+        s.gy = -G * math.cos(theta)
+        s.start("oracle").advanceSeconds(t).read()
+        if rolls:
+            assert abs(5.0 / 7 * G * math.sin(theta) * t - s.u[0]) <= 1e-2
+            assert abs(-s.u[0] / 0.1 - s.az[0]) <= 1e-2
+        else:
+            assert abs(G * (math.sin(theta) - mu_kinetic * math.cos(theta)) * t - s.u[0]) <= 1e-2
+            assert abs(-2.5 * mu_kinetic * G * math.cos(theta) * t / 0.1 - s.az[0]) <= 0.05
+
+
 def _glancing(mu):
     s = Scene(2).uniform(0.1, 1)
     s.gy = 0
