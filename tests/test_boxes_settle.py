@@ -203,6 +203,34 @@ def test_a_box_slides_down_a_slope_its_friction_cannot_hold():
         assert lean(s.axis(0, 1)) < 1e-3
 
 
+@pytest.mark.parametrize("on_slab", [False, True])
+@pytest.mark.parametrize("mu_static,mu_kinetic", [(0.6, 0.2), (0.2, 0.4)])
+def test_a_sliding_box_slows_at_mu_kinetic_g(mu_static, mu_kinetic, on_slab):
+    """Static and kinetic friction apart, which no reference box test sets apart: past mu_s lambda_n the tangent
+    impulse is set to mu_k lambda_n (Boxes.java:525-533). A flat box sliding at 2 m/s, on the floor (the walls'
+    contact) or on a fixed slab (a pair's), slows at mu_k g whatever mu_s is: 2 - mu_k g t after 0.25 s, to 1e-2,
+    and does not turn."""
+    v0, t = 2.0, 0.25
+    n = 2 if on_slab else 1
+    with BoxScene(n).uniform(0.1, 0.05, 0.1, 1) as s:
+        box = n - 1
+        s.muStatic, s.muKinetic = mu_static, mu_kinetic
+        s.sx, s.sz = 4, 1
+        if on_slab:
+            s.half[0][0], s.half[2][0] = np.float32(1.8), np.float32(0.3)
+            s.im[0] = 0
+            s.x[0] = 2
+            s.y[0] = np.float32(0.05)
+            s.z[0] = np.float32(0.5)
+        s.x[box] = np.float32(0.4)
+        s.y[box] = np.float32(0.15 if on_slab else 0.05)
+        s.z[box] = np.float32(0.5)
+        s.u[box] = v0
+        s.start("oracle").advanceSeconds(t).read()
+        assert abs(v0 - mu_kinetic * G * t - s.u[box]) <= 1e-2, (s.u[box], v0 - mu_kinetic * G * t)
+        assert lean(s.axis(box, 1)) < 1e-3
+
+
 @pytest.mark.parametrize("degrees", [12, 16])
 def test_a_tall_box_tips_where_its_weight_leaves_its_base(degrees):
     """BoxWallTest.aTallBoxTipsWhereItsWeightLeavesItsBase (:121-148): 0.1 x 0.4 x 0.1 m, mu 1; it tips past
