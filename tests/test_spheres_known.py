@@ -263,6 +263,32 @@ def test_on_a_slope_friction_decides_whether_it_rolls(mu_static, mu_kinetic, rol
             assert abs(-2.5 * mu_kinetic * G * math.cos(theta) * t / 0.1 - s.az[0]) <= 0.05
 
 
+@pytest.mark.parametrize("omega", [1.0, 0.5])
+@pytest.mark.parametrize("averaged", [True, False])
+def test_a_jacobi_move_is_the_summed_shares_relaxed(omega, averaged):
+    """Spheres.Solving (Spheres.java:1259-1297), on the oracle: each overlap adds this sphere's share of it,
+    (reach - dist) w_s / w_sum along the separation, and the sum is scaled by omega, or by omega over the sphere's own
+    contact count when averaged (Spheres.java:75-88). Sphere A is pressed 1 cm by B along x and by C along y; B and C
+    are apart. One Jacobi pass, no gravity: A moves omega (0.005, 0.005) back, halved when averaged (two contacts),
+    and B and C each move omega 0.005 away (one contact each)."""
+    with Scene(3).uniform(0.1, 1) as s:
+        s.gy = 0
+        s.substeps = 1
+        s.solve = JACOBI
+        s.omega, s.averaged = omega, averaged
+        s.x[:], s.y[:], s.z[:] = 0.5, 0.5, 0.5
+        s.x[1] = np.float32(0.69)
+        s.y[2] = np.float32(0.69)
+        x0, y0 = s.x.astype(np.float64), s.y.astype(np.float64)
+        s.start("oracle").advance(1).read()
+        a = omega * 0.005 * (0.5 if averaged else 1.0)
+        b = omega * 0.005
+        moved = np.stack([s.x - x0, s.y - y0])
+        want = np.array([[-a, b, 0.0], [-a, 0.0, b]])
+        assert np.all(np.abs(moved - want) <= 1e-6), moved
+        assert np.all(s.z == np.float32(0.5))
+
+
 def _glancing(mu):
     s = Scene(2).uniform(0.1, 1)
     s.gy = 0
