@@ -184,6 +184,16 @@ int vr_last_error(char *buf, int32_t len);
  */
 int vr_box_collide(vr_world *w, const int32_t *pairs, int32_t npairs, float *manifold);
 /*
+ * The first half of BoxStepper.step (core/.../box/BoxStepper.java:62-95) on a box world or batch: the collide list
+ * BoxStep.collide() (core/.../box/BoxStep.java:138-153: keep, bin and sort, list, collide, carry, counts) run and
+ * its `contacts` read back, and the touching pairs coloured into `rounds`, with no substep after it: what the
+ * reference's Runner.runAndRead(step.collide(), "contacts") leaves, before runner.run(step.step(used)). The report
+ * holds the step's counts (touching, points, rounds, unplaced, unlisted). A later vr_world_step collides again from
+ * what this left, as the reference's next step would. Pairs out of 0 .. bodies - 1, or a negative count, given to
+ * vr_box_collide are VR_ERR_ARGUMENT, checked before anything runs.
+ */
+int vr_box_collide_list(vr_world *w, vr_step_report *report);
+/*
  * Diagnostics reduced on the device in f64 (SphereDiagnostics.of, core/.../sphere/SphereDiagnostics.java:34-83;
  * BoxDiagnostics.of, core/.../box/BoxDiagnostics.java:18-44, both host-side in the reference), into out[14]:
  * VR_DIAG_OVERLAP deepest sphere-sphere overlap over the smaller radius (spheres; through the grid, rebuilt at the
@@ -243,6 +253,8 @@ int vr_mlp_last_error(char *buf, int32_t len);
  * vr_box_samples copies up to `most` of them out. vr_box_learned sets the network (null: none) whose prediction
  * replaces the carried impulse at each point where it is usable (finite, normal part not negative); the report's
  * `reserved` word then counts the points it replaced.
+ * The world keeps only the network's handle: `m` must stay alive while it is set. vr_mlp_destroy refuses
+ * (VR_ERR_STATE) a network that a world still holds; vr_box_learned(w, NULL) or vr_world_destroy lets it go.
  */
 int vr_box_log(vr_world *w, int32_t on, int64_t capacity);
 int vr_box_samples(vr_world *w, float *x, float *t, int64_t most, int64_t *count);

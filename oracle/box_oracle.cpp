@@ -1226,8 +1226,9 @@ struct BoxWorld {
         }
     }
 
-    // BoxStepper.step, BoxStepper.java:56-100, with BoxStep.collide and BoxStep.step (BoxStep.java:187-214).
-    void step(vr_step_report *rep) {
+    // BoxStepper.step up to the rounds' write (BoxStepper.java:62-95): BoxStep.collide (BoxStep.java:138-153) and
+    // the host's first-fit colouring; the counts into rep. Returns the rounds used.
+    int prepare(vr_step_report *rep) {
         int used = 0, touching = 0, points = 0, unplaced = 0, unlisted = 0;
         if (slots > 0) {
             keep();
@@ -1258,6 +1259,21 @@ struct BoxWorld {
             }
             for (int s = 0; s < n; s++) unlisted += std::max(0, contacts[2 * (size_t)slots + s] - SLOTS);
         }
+        if (rep) {
+            rep->rounds = used;
+            rep->most_rounds = used;
+            rep->touching = touching;
+            rep->points = points;
+            rep->unplaced = unplaced;
+            rep->unlisted = unlisted;
+            rep->submissions = slots > 0 ? 1 : 0;
+        }
+        return used;
+    }
+
+    // BoxStepper.step, BoxStepper.java:56-100, with BoxStep.collide and BoxStep.step (BoxStep.java:187-214).
+    void step(vr_step_report *rep) {
+        int used = prepare(rep);
         for (int sub = 0; sub < d.substeps; sub++) {
             velocities();
             sweep(WARM, false, used);
@@ -1266,15 +1282,7 @@ struct BoxWorld {
             for (int k = 0; k < d.sweeps; k++) sweep(RELAX, d.symmetric && k % 2 == 1, used);
         }
         show();
-        if (rep) {
-            rep->rounds = used;
-            rep->most_rounds = used;
-            rep->touching = touching;
-            rep->submissions = slots > 0 ? 2 : 1;  // BoxStepper.step: collide read back, then the step
-            rep->points = points;
-            rep->unplaced = unplaced;
-            rep->unlisted = unlisted;
-        }
+        if (rep) rep->submissions = slots > 0 ? 2 : 1;  // BoxStepper.step: collide read back, then the step
     }
 };
 
@@ -1315,7 +1323,14 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n) {
 
 void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
 
+void box_collide_list(BoxWorld *w, vr_step_report *rep) { w->prepare(rep); }
+
 void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold) {
+    if (npairs < 0) throw std::invalid_argument("a count of pairs is at least 0, got " + std::to_string(npairs));
+    for (int64_t k = 0; k < 2 * (int64_t)npairs; k++)
+        if (pairs[k] < 0 || pairs[k] >= w->n)
+            throw std::invalid_argument("pair " + std::to_string(k / 2) + " names body " + std::to_string(pairs[k]) +
+                                        ", outside 0 .. " + std::to_string(w->n - 1));
     for (int p = 0; p < npairs; p++) {
         float *m = manifold + (size_t)p * MANIFOLD_WORDS;
         std::fill(m, m + MANIFOLD_WORDS, 0.f);

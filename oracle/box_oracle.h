@@ -16,6 +16,8 @@ void box_clear(BoxWorld *w);
 bool box_buffer(BoxWorld *w, const std::string &name, void **ptr, int64_t *count, int32_t *dtype);
 void box_params(BoxWorld *w, const int32_t *words, int32_t n);
 void box_step(BoxWorld *w, vr_step_report *rep);
+// BoxStepper.step up to the rounds' write: the collide list and the colouring, no substep.
+void box_collide_list(BoxWorld *w, vr_step_report *rep);
 // The narrow phase alone over pairs (first, second), MANIFOLD_WORDS floats a pair out.
 void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold);
 

@@ -1408,8 +1408,36 @@ int vro_last_error(char *buf, int32_t len) {
 
 int vro_box_collide(vro_world *p, const int32_t *pairs, int32_t npairs, float *manifold) {
     if (p->kind != VR_BOXES || !p->box) return fail(VR_ERR_STATE, "not a box world of one scene");
-    vro::box_collide(p->box, pairs, npairs, manifold);
+    try {
+        vro::box_collide(p->box, pairs, npairs, manifold);
+    } catch (const std::invalid_argument &e) {
+        return fail(VR_ERR_ARGUMENT, e.what());
+    }
     return VR_OK;
+}
+
+int vro_box_collide_list(vro_world *p, vr_step_report *rep) {
+    if (p->kind != VR_BOXES) return fail(VR_ERR_STATE, "not a box world");
+    return guard([&] {
+        vr_step_report local{};
+        vr_step_report *r = rep ? rep : &local;
+        std::memset(r, 0, sizeof(*r));
+        if (p->batch.empty()) {
+            vro::box_collide_list(p->box, r);
+            return;
+        }
+        for (auto *b : p->batch) {
+            vr_step_report one{};
+            vro::box_collide_list(b, &one);
+            r->touching += one.touching;
+            r->points += one.points;
+            r->unplaced += one.unplaced;
+            r->unlisted += one.unlisted;
+            r->rounds = std::max(r->rounds, one.rounds);
+            r->most_rounds = std::max(r->most_rounds, one.most_rounds);
+            r->submissions = std::max(r->submissions, one.submissions);
+        }
+    });
 }
 
 // The oracle keeps no pass timing; its profile reads as zeros.

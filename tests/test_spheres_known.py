@@ -227,34 +227,36 @@ def test_on_a_slope_rolls_or_slides(backend):
         assert abs(-2.5 * mu * G * math.cos(theta) * t / 0.1 - s.az[0]) <= 0.05
 
 
+@pytest.mark.parametrize("backend", BACKENDS)
 @pytest.mark.parametrize("mu_static,mu_kinetic", [(0.6, 0.2), (0.2, 0.6)])
-def test_a_slide_is_slowed_by_kinetic_friction_alone(mu_static, mu_kinetic):
+def test_a_slide_is_slowed_by_kinetic_friction_alone(mu_static, mu_kinetic, backend):
     """Static and kinetic friction apart, which no reference test sets apart (Spheres.friction, Spheres.java:
-    595-654), on the oracle: a sphere sliding at 2 m/s on the floor slows at mu_k g and spins up at 5/2 mu_k g / r
+    595-654), on each backend: a sphere sliding at 2 m/s on the floor slows at mu_k g and spins up at 5/2 mu_k g / r
     until it rolls, whatever mu_s is. Read at 0.05 s, before it rolls at either mu_k."""
     v0, t = 2.0, 0.05
     with _on_the_floor(0) as s:
         s.muStatic, s.muKinetic = mu_static, mu_kinetic
         s.u[0] = v0
-        s.start("oracle").advanceSeconds(t).read()
+        s.start(backend).advanceSeconds(t).read()
         assert abs(v0 - mu_kinetic * G * t - s.u[0]) <= 1e-2
         assert abs(-2.5 * mu_kinetic * G * t / 0.1 - s.az[0]) <= 0.1
 
 
 @pytest.mark.parametrize("mu_static,mu_kinetic,rolls", [(0.3, 0.05, True), (0.1, 0.05, False), (0.05, 0.1, False),
                                                       (0.05, 0.3, True)])
-def test_on_a_slope_friction_decides_whether_it_rolls(mu_static, mu_kinetic, rolls):
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_on_a_slope_friction_decides_whether_it_rolls(mu_static, mu_kinetic, rolls, backend):
     """On a 30 degree slope rolling without slipping takes (2/7) tan 30 = 0.165 of the normal impulse, at 5/7 g sin t.
     Friction holds it whole within mu_s, else cuts it to mu_k, but never past what stops the slip (Spheres.friction's
     min(1, mu_k lambda_n / |whole|), Spheres.java:636-641). So the sphere rolls when either mu reaches 0.165 and
-    otherwise slides at g (sin t - mu_k cos t). On the oracle, to 1e-2 after 0.5 s."""
+    otherwise slides at g (sin t - mu_k cos t). On each backend, to 1e-2 after 0.5 s."""
     theta, t = math.radians(30), 0.5
     with _on_the_floor(0) as s:
         s.muStatic, s.muKinetic = mu_static, mu_kinetic
         s.x[0] = 0.2
         s.gx = G * math.sin(theta)
         s.gy = -G * math.cos(theta)
-        s.start("oracle").advanceSeconds(t).read()
+        s.start(backend).advanceSeconds(t).read()
         if rolls:
             assert abs(5.0 / 7 * G * math.sin(theta) * t - s.u[0]) <= 1e-2
             assert abs(-s.u[0] / 0.1 - s.az[0]) <= 1e-2
@@ -303,6 +305,37 @@ def _glancing(mu):
     s.solve = GS
     s.muStatic = s.muKinetic = mu
     return s
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_a_crowd_of_600_keeps_its_momentum_through_a_step(backend):
+    """docs/TODO.md:123-125: through one step of 600 crowded spheres of mixed masses, Gauss-Seidel until done, the
+    momentum stays at zero to 1e-6, so no two contacts that share a sphere were solved in the same round (two such
+    solved side by side on the device would each read the sphere before the other moved it, and the pair of moves
+    would no longer be equal and opposite by mass). GridTest.crowd's draws (radius 0.02-0.05 m, mass 0.1-1.1 kg,
+    java.util.Random(5)) packed into a 0.4 m cube at the middle of a 6 m box, far enough from the walls that nothing
+    reaches them within the step, no gravity, at rest. The spheres fly apart at up to ~110 m/s, so the sum is judged
+    against the momentum's own scale, sum m |v| (f32 velocities carry about 1e-7 of it each): 1e-6 of it."""
+    n = 600
+    rnd = pkg.JavaRandom(5)
+    with Scene(n) as s:
+        s.sx = s.sy = s.sz = 6.0
+        s.gy = 0.0
+        for k in range(n):
+            s.r[k] = np.float32(0.02 + 0.03 * rnd.nextDouble())
+            s.im[k] = np.float32(1 / (0.1 + rnd.nextDouble()))
+            s.x[k] = np.float32(2.8 + 0.4 * rnd.nextDouble())
+            s.y[k] = np.float32(2.8 + 0.4 * rnd.nextDouble())
+            s.z[k] = np.float32(2.8 + 0.4 * rnd.nextDouble())
+        s.solve = GS
+        s.start(backend).advance(1).read()
+        m = 1 / s.im.astype(np.float64)
+        assert s.x.min() > 2 * 0.05 and s.x.max() < 6 - 2 * 0.05, "a sphere reached a wall"
+        for name in ("u", "v", "w"):
+            vel = getattr(s, name).astype(np.float64)
+            scale = float(np.sum(m * np.abs(vel)))
+            assert scale > 100.0, "the crowd did not part"
+            assert abs(float(np.sum(m * vel))) <= 1e-6 * scale, (name, float(np.sum(m * vel)), scale)
 
 
 @pytest.mark.parametrize("backend", BACKENDS)

@@ -27,7 +27,7 @@ ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "wor
                 "world_start", "world_friction", "world_slice", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
                 "world_profile_read", "box_collide", "batch_create", "mlp_create", "mlp_destroy", "mlp_read",
                 "mlp_write", "mlp_forward", "mlp_gradient", "mlp_train", "mlp_last_error", "box_log",
-                "box_samples", "box_learned", "world_diagnostics")
+                "box_samples", "box_learned", "world_diagnostics", "box_collide_list")
 
 
 class StepDesc(ctypes.Structure):
@@ -80,6 +80,7 @@ class Abi:
             "world_profile": ([p, ctypes.c_int32]),
             "world_profile_read": ([p, ctypes.POINTER(Profile)]),
             "box_collide": ([p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
+            "box_collide_list": ([p, ctypes.POINTER(StepReport)]),
             "mlp_create": ([ctypes.c_uint64, ctypes.POINTER(p)]),
             "mlp_destroy": ([p]),
             "mlp_read": ([p, ctypes.POINTER(ctypes.c_float)]),
@@ -238,6 +239,11 @@ class World:
         self._check(self.abi.fn("box_collide")(self.handle, pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                                pairs.size // 2, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
+
+    def collide_list(self) -> StepReport:
+        rep = StepReport()
+        self._check(self.abi.fn("box_collide_list")(self.handle, ctypes.byref(rep)))
+        return rep
 
     def destroy(self) -> None:
         if self.handle:

@@ -206,6 +206,12 @@ class BoxSimulation:
     def last(self) -> BoxReport:
         return self._last
 
+    def collideList(self) -> BoxReport:
+        """The first half of ``BoxStepper.step`` (BoxStepper.java:62-95): the collide list (keep, broad phase,
+        narrow phase, carry, counts) and the colouring, with no substep run (vr_box_collide_list)."""
+        rep = self._world.collide_list()
+        return BoxReport(rep.touching, rep.points, rep.rounds, rep.unplaced, rep.unlisted)
+
     def collide(self, pairs) -> np.ndarray:
         """The narrow phase alone over ``pairs`` (first, second, first, second, ...)."""
         return self._world.collide(pairs)
@@ -229,8 +235,12 @@ class BoxSimulation:
 
     def learned(self, net) -> None:
         """Starts each step's impulses from `net`'s prediction (an Mlp of the same library), or from the carry
-        (None), as the reference does."""
+        (None), as the reference does. The world holds the network until learned(None) or close(), so it is kept
+        here as long: the library keeps only its handle."""
+        if net is not None and getattr(net, "_h", None) is None:
+            raise IllegalArgumentException("the network is closed")
         self._world._check(self._abi.fn("box_learned")(self._world.handle, net._h if net is not None else None))
+        self._net = net
 
     def diagnostics(self):
         """BoxDiagnostics.of on the device (vr_world_diagnostics), with the top speed, whether any state is not
@@ -253,6 +263,7 @@ class BoxSimulation:
         if self._world is not None:
             self._world.destroy()
             self._world = None
+        self._net = None
 
     def __enter__(self):
         return self

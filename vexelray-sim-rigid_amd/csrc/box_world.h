@@ -35,6 +35,8 @@ bool box_find(BoxWorld *w, const std::string &name, BoxBuf *out);
 hipStream_t box_stream(BoxWorld *w);
 void box_params(BoxWorld *w, const int32_t *words, int32_t n);
 void box_step(BoxWorld *w, vr_step_report *rep);
+// BoxStepper.step up to the rounds' write: the collide list and the colouring, no substep (vr_box_collide_list).
+void box_collide_list(BoxWorld *w, vr_step_report *rep);
 void box_wall_impulses_written(BoxWorld *w);  // after a write of lambda or rub through the ABI
 void box_shapes_written(BoxWorld *w);         // after a write of shape: the wall impulses follow their bodies
 void box_pair_buffer_written(BoxWorld *w, const std::string &name);  // after a write of a manifold or impulse buffer

@@ -452,12 +452,17 @@ struct BoxWorld {
         if (!reverse) launch_walls(D, kind, reverse, stream);
     }
 
-    void step(vr_step_report *rep) {
-        if (tunable && tuneStep < 3) scene_step(tuneStep == 0 ? false : tuneStep == 1 ? true : sceneMs < passMs);
-        auto w0 = std::chrono::steady_clock::now();
-        HIP_OK(hipEventRecord(t0, stream));
-        if (wallsFromParams) launch_scene_from_params(D, sceneDev, stream);
+    // The counts BoxStepper.step reports for the step's collide (BoxStepper.java:40, 62-90).
+    struct Prepared {
         int used = 0, touching = 0, points = 0, unplaced = 0, unlisted = 0;
+    };
+
+    // BoxStepper.step up to the rounds' write (BoxStepper.java:62-95): BoxStep.collide (keep, bin and sort, list,
+    // collide, carry, counts; BoxStep.java:138-153), the touching pairs coloured into rounds, and, with a network
+    // set, its prediction in place of the carried impulses.
+    Prepared prepare() {
+        Prepared p;
+        if (wallsFromParams) launch_scene_from_params(D, sceneDev, stream);
         if (grid) {
             // BoxPairs.keep: the step's pairs, manifolds and impulses become the kept ones, which carry reads.
             std::swap(D.pairs, D.pairsWas);
@@ -471,16 +476,50 @@ struct BoxWorld {
             launch_collide(D, stream);
             launch_carry_counts(D, stream);
             // BoxStepper's colouring (first fit in slot order), on the device; the host reads back the counts.
-            colour(&used, &touching, &points, &unplaced, &unlisted);
-            if ((net || logging) && touching > 0) {
-                points_room(points);
-                launch_point_features(D, C, L, touching, stream);
+            colour(&p.used, &p.touching, &p.points, &p.unplaced, &p.unlisted);
+            if ((net || logging) && p.touching > 0) {
+                points_room(p.points);
+                launch_point_features(D, C, L, p.touching, stream);
                 if (net) {
-                    net->forward(L.x, points, L.y, stream);
-                    launch_apply_prediction(D, C, L, touching, stream);
+                    net->forward(L.x, p.points, L.y, stream);
+                    launch_apply_prediction(D, C, L, p.touching, stream);
                 }
             }
         }
+        return p;
+    }
+
+    void report_counts(const Prepared &p, vr_step_report *rep) {
+        rep->rounds = p.used;
+        rep->most_rounds = p.used;
+        rep->touching = p.touching;
+        rep->points = p.points;
+        rep->unplaced = p.unplaced;
+        rep->unlisted = p.unlisted;
+    }
+
+    // The collide list alone, read back and coloured, with no substep run: what a step's impulses start from.
+    void collide_list(vr_step_report *rep) {
+        auto w0 = std::chrono::steady_clock::now();
+        HIP_OK(hipEventRecord(t0, stream));
+        Prepared p = prepare();
+        HIP_OK(hipEventRecord(t1, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, t0, t1));
+        report_counts(p, rep);
+        rep->submissions = grid ? 1 : 0;
+        rep->gpu_ns = (int64_t)(ms * 1e6);
+        rep->wall_ns =
+            std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - w0).count();
+    }
+
+    void step(vr_step_report *rep) {
+        if (tunable && tuneStep < 3) scene_step(tuneStep == 0 ? false : tuneStep == 1 ? true : sceneMs < passMs);
+        auto w0 = std::chrono::steady_clock::now();
+        HIP_OK(hipEventRecord(t0, stream));
+        Prepared p = prepare();
+        int used = p.used, touching = p.touching, points = p.points;
         if (sceneStep) {
             launch_scene_step(D, sceneFirst, C.starts, used, d.substeps, d.sweeps, d.symmetric != 0, stream);
         } else {
@@ -529,19 +568,20 @@ struct BoxWorld {
             HIP_OK(hipStreamSynchronize(stream));
             rep->reserved = pinned[48];
         }
-        rep->rounds = used;
-        rep->most_rounds = used;
-        rep->touching = touching;
+        report_counts(p, rep);
         rep->submissions = grid ? 2 : 1;  // BoxStepper.step: the collide list read back, then the step's list
-        rep->points = points;
-        rep->unplaced = unplaced;
-        rep->unlisted = unlisted;
         rep->gpu_ns = (int64_t)(ms * 1e6);
         rep->wall_ns =
             std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - w0).count();
     }
 
     void collide_pairs(const int32_t *pairs, int npairs, float *manifold) {
+        // Checked before anything is allocated or launched: the kernel loads both bodies of every pair unguarded.
+        if (npairs < 0) throw std::invalid_argument("a count of pairs is at least 0, got " + std::to_string(npairs));
+        for (int64_t k = 0; k < 2 * (int64_t)npairs; k++)
+            if (pairs[k] < 0 || pairs[k] >= n)
+                throw std::invalid_argument("pair " + std::to_string(k / 2) + " names body " +
+                                            std::to_string(pairs[k]) + ", outside 0 .. " + std::to_string(n - 1));
         int32_t *dp = nullptr;
         float *dm = nullptr;
         HIP_OK(hipMalloc(&dp, (size_t)std::max(1, 2 * npairs) * 4));
@@ -555,6 +595,7 @@ struct BoxWorld {
     }
 
     ~BoxWorld() {
+        if (net) net->holders--;  // the network may be destroyed once no world starts from it
         if (stream) (void)hipStreamSynchronize(stream);
         if (pinned) (void)hipHostFree(pinned);
         for (void *p : {(void *)L.pointBase, (void *)L.x, (void *)L.y, (void *)L.t, (void *)L.used, (void *)logX,
@@ -624,6 +665,7 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n) {
 }
 
 void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
+void box_collide_list(BoxWorld *w, vr_step_report *rep) { w->collide_list(rep); }
 void box_wall_impulses_written(BoxWorld *w) { w->wall_impulses_written(); }
 void box_shapes_written(BoxWorld *w) {
     w->place_walls(true);
@@ -635,7 +677,11 @@ void box_pair_buffer_written(BoxWorld *w, const std::string &name) { w->pair_buf
 void box_log(BoxWorld *w, bool on, int64_t capacity) { w->log(on, capacity); }
 void box_diagnostics(BoxWorld *w, double *out) { w->diagnostics(out); }
 int64_t box_samples(BoxWorld *w, float *x, float *t, int64_t most) { return w->samples(x, t, most); }
-void box_learned(BoxWorld *w, vrmlp::Net *net) { w->net = net; }
+void box_learned(BoxWorld *w, vrmlp::Net *net) {
+    if (net) net->holders++;
+    if (w->net) w->net->holders--;
+    w->net = net;
+}
 
 void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold) {
     w->collide_pairs(pairs, npairs, manifold);

@@ -20,6 +20,7 @@ struct Net {
     float *loss = nullptr;          // 1, added to by gradient()
     void *packed = nullptr;         // bf16 weights for forward()
     int steps = 0;                  // Adam steps taken
+    int holders = 0;                // box worlds that start their steps from this network (vr_box_learned)
     Net();
     ~Net();
     void pack(hipStream_t s);       // after params change

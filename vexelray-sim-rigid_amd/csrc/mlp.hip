@@ -428,6 +428,11 @@ int vr_mlp_create(uint64_t seed, vr_mlp **out) {
 }
 
 int vr_mlp_destroy(vr_mlp *m) {
+    if (m && m->net.holders > 0) {
+        g_mlpError = "the network is set on " + std::to_string(m->net.holders) +
+                      " box world(s); unset it (vr_box_learned with null) or destroy them first";
+        return VR_ERR_STATE;
+    }
     delete m;
     return VR_OK;
 }

@@ -817,7 +817,24 @@ int vr_world_destroy(vr_world *w) {
 int vr_box_collide(vr_world *w, const int32_t *pairs, int32_t npairs, float *manifold) {
     if (!w || (!pairs && npairs > 0) || (!manifold && npairs > 0)) return fail(VR_ERR_ARGUMENT, "null argument");
     if (!w->box) return fail(VR_ERR_STATE, "not a box world");
-    return guard([&] { vrbox::box_collide(w->box, pairs, npairs, manifold); });
+    return guard([&] {
+        try {
+            vrbox::box_collide(w->box, pairs, npairs, manifold);
+        } catch (const std::invalid_argument &e) {
+            throw ArgError(e.what());
+        }
+    });
+}
+
+int vr_box_collide_list(vr_world *w, vr_step_report *report) {
+    if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    if (!w->box) return fail(VR_ERR_STATE, "not a box world");
+    return guard([&] {
+        vr_step_report local{};
+        vr_step_report *r = report ? report : &local;
+        std::memset(r, 0, sizeof(*r));
+        vrbox::box_collide_list(w->box, r);
+    });
 }
 
 int vr_box_log(vr_world *w, int32_t on, int64_t capacity) {
