@@ -93,7 +93,7 @@ def batch_run(spec, backend, steps):
     for _ in range(steps):
         r = sim.stepAndWait()
         reps.append((r.touching, r.points, r.rounds, r.unplaced, r.unlisted))
-    state = {k: sim.floats(k).copy() for k in BOX_STATE + ("lambda", "rub", "roll")}
+    state = {k: sim.floats(k).copy() for k in BOX_STATE}
     sim.close()
     return reps, state
 
