@@ -214,6 +214,16 @@ int vr_box_collide_list(vr_world *w, vr_step_report *report);
 #define VR_DIAG_DEPTH 13
 #define VR_DIAG_WORDS 14
 int vr_world_diagnostics(vr_world *w, double *out, int32_t n);
+/*
+ * No counterpart in the reference. vr_world_diagnostics without allocating: for callers that cannot allocate device
+ * memory (a serving loop that must not call hipMalloc, a harness that forbids it). vr_world_diagnostics makes its
+ * per-workgroup partials and its 14 results in device memory on its first call; this one keeps them in a fixed
+ * 1 MiB array of the library's own on each GPU (held until the process ends), so it calls no hipMalloc or hipFree.
+ * It takes worlds of at most 2,396,160 bodies (VR_ERR_ARGUMENT past that, before anything runs) and returns what
+ * vr_world_diagnostics returns, word for word. Every call on one GPU shares that array, and nothing in it has to
+ * start at zero: calls on worlds of the same GPU from two threads at once race on it, so a caller keeps them apart.
+ */
+int vr_world_diagnostics_static(vr_world *w, double *out, int32_t n);
 /* Switches pass timing on or off and zeroes its counters. */
 int vr_world_profile(vr_world *w, int32_t on);
 int vr_world_profile_read(vr_world *w, vr_profile *out);

@@ -1548,6 +1548,9 @@ int vro_world_diagnostics(vro_world *p, double *out, int32_t n) {
     return VR_OK;
 }
 
+// The oracle holds no device scratch: its allocation-free diagnostics are its diagnostics.
+int vro_world_diagnostics_static(vro_world *p, double *out, int32_t n) { return vro_world_diagnostics(p, out, n); }
+
 // The learned warm start runs on the device only: the oracle's box worlds keep the reference's carry.
 int vro_box_log(vro_world *, int32_t, int64_t) { return fail(VR_ERR_STATE, "the oracle keeps no sample log"); }
 int vro_box_samples(vro_world *, float *, float *, int64_t, int64_t *) {

@@ -27,7 +27,8 @@ ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "wor
                 "world_start", "world_friction", "world_slice", "world_step", "world_finish", "world_destroy", "last_error", "world_profile",
                 "world_profile_read", "box_collide", "batch_create", "mlp_create", "mlp_destroy", "mlp_read",
                 "mlp_write", "mlp_forward", "mlp_gradient", "mlp_train", "mlp_last_error", "box_log",
-                "box_samples", "box_learned", "world_diagnostics", "box_collide_list")
+                "box_samples", "box_learned", "world_diagnostics", "box_collide_list",
+                "world_diagnostics_static")
 
 
 class StepDesc(ctypes.Structure):
@@ -97,6 +98,7 @@ class Abi:
                              ctypes.POINTER(ctypes.c_int64)]),
             "box_learned": ([p, p]),
             "world_diagnostics": ([p, ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
+            "world_diagnostics_static": ([p, ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
             "batch_create": ([ctypes.POINTER(StepDesc), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(p)]),

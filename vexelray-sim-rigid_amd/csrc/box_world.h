@@ -47,6 +47,9 @@ void box_log(BoxWorld *w, bool on, int64_t capacity);
 int64_t box_samples(BoxWorld *w, float *x, float *t, int64_t most);
 void box_learned(BoxWorld *w, vrmlp::Net *net);
 void box_diagnostics(BoxWorld *w, double *out);  // vr_world_diagnostics for a box world
+// The same into device scratch the caller gives: vrdiag::partial_words(bodies) partials, then D_WORDS results.
+void box_diagnostics_into(BoxWorld *w, double *partials, double *devOut, double *out);
+int box_bodies(BoxWorld *w);
 void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold);
 
 }  // namespace vrbox

@@ -401,8 +401,13 @@ struct BoxWorld {
             diagPartials = alloc<double>(vrdiag::partial_words(n));
             diagOut = alloc<double>(vrdiag::D_WORDS);
         }
-        vrdiag::launch_boxes(D, diagPartials, diagOut, stream);
-        HIP_OK(hipMemcpyAsync(out, diagOut, sizeof(double) * vrdiag::D_WORDS, hipMemcpyDeviceToHost, stream));
+        diagnostics_into(diagPartials, diagOut, out);
+    }
+
+    // The same into device scratch the caller provides (vrdiag::partial_words(n) partials, D_WORDS results).
+    void diagnostics_into(double *partials, double *devOut, double *out) {
+        vrdiag::launch_boxes(D, partials, devOut, stream);
+        HIP_OK(hipMemcpyAsync(out, devOut, sizeof(double) * vrdiag::D_WORDS, hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
     }
 
@@ -676,6 +681,10 @@ void box_wall_places(BoxWorld *w, std::vector<int32_t> &out) { w->wall_places(ou
 void box_pair_buffer_written(BoxWorld *w, const std::string &name) { w->pair_buffer_written(name); }
 void box_log(BoxWorld *w, bool on, int64_t capacity) { w->log(on, capacity); }
 void box_diagnostics(BoxWorld *w, double *out) { w->diagnostics(out); }
+void box_diagnostics_into(BoxWorld *w, double *partials, double *devOut, double *out) {
+    w->diagnostics_into(partials, devOut, out);
+}
+int box_bodies(BoxWorld *w) { return w->n; }
 int64_t box_samples(BoxWorld *w, float *x, float *t, int64_t most) { return w->samples(x, t, most); }
 void box_learned(BoxWorld *w, vrmlp::Net *net) {
     if (net) net->holders++;

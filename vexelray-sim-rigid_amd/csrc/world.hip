@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -52,6 +53,7 @@ struct Buf {
 
 struct vr_world {
     vrbox::BoxWorld *box = nullptr;  // set for a box world, which the calls below hand on to
+    int device = 0;                  // the GPU current when the world was made, whose stream it steps on
     vr_step_desc d{};
     int n = 0, cap = 0, cells = 0;
     bool gs = false, grid = false, friction = true;
@@ -545,12 +547,18 @@ struct vr_world {
             diagPartials = alloc<double>(vrdiag::partial_words(n));
             diagOut = alloc<double>(vrdiag::D_WORDS);
         }
+        diagnostics_into(diagPartials, diagOut, out);
+    }
+
+    // The same into device scratch the caller provides: `partials` of vrdiag::partial_words(n) doubles and
+    // `devOut` of D_WORDS.
+    void diagnostics_into(double *partials, double *devOut, double *out) {
         if (grid) {
             launch_bin(D, stream);
             launch_sort(D, S, stream);
         }
-        vrdiag::launch_spheres(D, grid, diagPartials, diagOut, stream);
-        HIP_OK(hipMemcpyAsync(out, diagOut, sizeof(double) * vrdiag::D_WORDS, hipMemcpyDeviceToHost, stream));
+        vrdiag::launch_spheres(D, grid, partials, devOut, stream);
+        HIP_OK(hipMemcpyAsync(out, devOut, sizeof(double) * vrdiag::D_WORDS, hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
     }
 
@@ -592,6 +600,40 @@ int guard(F fn) {
 
 }  // namespace
 
+// Device scratch of vr_world_diagnostics_static: each workgroup's row of partial sums and maxima, then the 14 results,
+// in place of the buffers vr_world_diagnostics makes on its first call. 1 MiB, one copy on each GPU the library runs
+// on, held until the process ends. In host code `scratch` is only a stand-in for the symbol: its device address on
+// each GPU comes from hipGetSymbolAddress (static_scratch), and only that address is handed to kernels and copies.
+constexpr int64_t STATIC_DIAG_DOUBLES = 131072;
+// The most bodies that fit: (ceil(n / 256) + 1) rows of 14 doubles, and the 14 results.
+constexpr int STATIC_DIAG_BODIES = (int)((STATIC_DIAG_DOUBLES / 14 - 2) * 256);
+static __device__ __align__(256) double scratch[STATIC_DIAG_DOUBLES];
+
+namespace {
+
+constexpr int MOST_DEVICES = 64;
+std::mutex g_scratchLock;
+double *g_scratchAt[MOST_DEVICES] = {};
+
+// The scratch array's address on `device`, looked up once per device with that device current.
+double *static_scratch(int device) {
+    if (device < 0 || device >= MOST_DEVICES) throw DeviceError("no scratch for device " + std::to_string(device));
+    std::lock_guard<std::mutex> hold(g_scratchLock);
+    if (!g_scratchAt[device]) {
+        int current = 0;
+        HIP_OK(hipGetDevice(&current));
+        if (current != device) HIP_OK(hipSetDevice(device));
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (current != device) HIP_OK(hipSetDevice(current));
+        if (e != hipSuccess) throw DeviceError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratchAt[device] = static_cast<double *>(p);
+    }
+    return g_scratchAt[device];
+}
+
+}  // namespace
+
 extern "C" {
 
 int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out) {
@@ -609,6 +651,7 @@ int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out) {
         } else {
             w->build(*desc);
         }
+        HIP_OK(hipGetDevice(&w->device));
     });
     if (rc != VR_OK) {
         delete w;
@@ -628,6 +671,7 @@ int vr_batch_create(const vr_step_desc *desc, int32_t scenes, const int32_t *bod
         } catch (const std::invalid_argument &e) {
             throw ArgError(e.what());
         }
+        HIP_OK(hipGetDevice(&w->device));
     });
     if (rc != VR_OK) {
         delete w;
@@ -859,6 +903,22 @@ int vr_world_diagnostics(vr_world *w, double *out, int32_t n) {
     if (!w || !out || n < vrdiag::D_WORDS) return fail(VR_ERR_ARGUMENT, "diagnostics need 14 doubles");
     if (w->box) return guard([&] { vrbox::box_diagnostics(w->box, out); });
     return guard([&] { w->diagnostics(out); });
+}
+
+int vr_world_diagnostics_static(vr_world *w, double *out, int32_t n) {
+    if (!w || !out || n < vrdiag::D_WORDS) return fail(VR_ERR_ARGUMENT, "diagnostics need 14 doubles");
+    return guard([&] {
+        int bodies = w->box ? vrbox::box_bodies(w->box) : w->n;
+        int64_t rows = vrdiag::partial_words(bodies);
+        if (rows + vrdiag::D_WORDS > STATIC_DIAG_DOUBLES)
+            throw ArgError("diagnostics without allocation take at most " + std::to_string(STATIC_DIAG_BODIES) +
+                           " bodies, got " + std::to_string(bodies));
+        double *at = static_scratch(w->device);
+        if (w->box)
+            vrbox::box_diagnostics_into(w->box, at, at + rows, out);
+        else
+            w->diagnostics_into(at, at + rows, out);
+    });
 }
 
 int vr_world_profile(vr_world *w, int32_t on) {
