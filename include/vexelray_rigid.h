@@ -267,6 +267,47 @@ int vr_mlp_last_error(char *buf, int32_t len);
  * (VR_ERR_STATE) a network that a world still holds; vr_box_learned(w, NULL) or vr_world_destroy lets it go.
  */
 int vr_box_log(vr_world *w, int32_t on, int64_t capacity);
+/*
+ * Sweeps stopped on a residual (no counterpart in the reference, whose BoxStep runs `sweeps` sweeps of the solve and
+ * of the relax every substep, BoxStep.java:199-214). A point's residual in a solve or relax sweep is the size of the
+ * impulse change that sweep made there (normal and tangent together) times the inverse mass along its normal: how
+ * far the sweep still moved the point's relative velocity, in m/s. With a tolerance of zero or more, each scene's
+ * solve (and relax) sweeps of a substep end after the first sweep whose points all stay at or below it, or at the
+ * step's `sweeps`; a scene's sweeps depend on its own residuals alone. Below zero (the default): every sweep runs, as
+ * the reference runs them. With `measure`, each step keeps what its phases did (vr_box_solve_read) and each sweep's
+ * largest residual (vr_box_solve_curve); a measured step computes every residual and runs slower, with the same
+ * result. A NaN tolerance is VR_ERR_ARGUMENT.
+ */
+int vr_box_tolerance(vr_world *w, float tolerance, int32_t measure);
+#define VR_SOLVE_SUBSTEPS 64
+/*
+ * What the last step's phases did (the first VR_SOLVE_SUBSTEPS substeps; index 0 the solve sweeps, 1 the relax).
+ * launched: sweeps the step launched, pass by pass (the most any scene needed; 0 when a workgroup steps each scene,
+ * scene_step = 1). Measured steps only: scene_sweeps, the sweeps a scene ran on average; residual, the largest
+ * residual a scene ended the phase with. fell_back: touching pairs whose network prediction raised their start
+ * residual, which then started from the carry (vr_box_learned).
+ */
+typedef struct vr_box_solve {
+    float tolerance;
+    int32_t measured;
+    int32_t substeps;
+    int32_t sweeps;
+    int32_t scenes;
+    int32_t scene_step;
+    int32_t fell_back;
+    int32_t unused;
+    int32_t launched[2][VR_SOLVE_SUBSTEPS];
+    float scene_sweeps[2][VR_SOLVE_SUBSTEPS];
+    float residual[2][VR_SOLVE_SUBSTEPS];
+} vr_box_solve;
+int vr_box_solve_read(vr_world *w, vr_box_solve *out);
+/*
+ * The last measured step's largest residual over every point of each sweep, laid out [substep][phase][sweep] with
+ * VR_SOLVE_SUBSTEPS substeps, 2 phases and VR_SOLVE_SWEEPS sweeps (zeros past what ran); up to n floats into out,
+ * *count how many.
+ */
+#define VR_SOLVE_SWEEPS 64
+int vr_box_solve_curve(vr_world *w, float *out, int32_t n, int32_t *count);
 int vr_box_samples(vr_world *w, float *x, float *t, int64_t most, int64_t *count);
 int vr_box_learned(vr_world *w, vr_mlp *m);
 

@@ -239,9 +239,20 @@ float mass_along(const Moving *first, const float *rA, const Moving &second, con
     return first ? k + first->inverseMassAlong(rA, n) : k;
 }
 
-// Boxes.contact, Boxes.java:462-542: one point, one sweep.
+// A point's residual in a solve or relax sweep (no counterpart in the reference, whose sweeps run a fixed count):
+// the size of the impulse change the sweep made there, normal `dn` and tangent `rubbed` together, times the inverse
+// mass along the normal, in m/s.
+float point_residual(float k, float dn, const float *rubbed) { return k * std::sqrt(dn * dn + dot(rubbed, rubbed)); }
+
+// res = the larger of res and r; NaN, once met, stays.
+void keep_largest(float &res, float r) {
+    if (res == res && !(r <= res)) res = r;
+}
+
+// Boxes.contact, Boxes.java:462-542: one point, one sweep. With `res`, the point's residual kept in it when larger.
 void contact(Sweep sweep, float gap, const float *n, Moving *first, const float *rA, Moving &second, const float *rB,
-             float *normals, size_t normalAt, float *tangents, size_t tangentAt, const Soft &soft) {
+             float *normals, size_t normalAt, float *tangents, size_t tangentAt, const Soft &soft,
+             float *res = nullptr) {
     float normal = normals[normalAt];
     float tangent[3] = {tangents[tangentAt], tangents[tangentAt + 1], tangents[tangentAt + 2]};
     bool holding = normal > 0 || dot(tangent, tangent) > 0;
@@ -296,6 +307,7 @@ void contact(Sweep sweep, float gap, const float *n, Moving *first, const float 
         rubbed[a] = whole[a] - tangent[a];
     }
     apply_pair(first, rA, second, rB, rubbed);
+    if (res) keep_largest(*res, point_residual(k, total - normal, rubbed));
 }
 
 void apply_angular(Moving *first, Moving &second, const float *j) {  // Boxes.java:603-612
@@ -869,7 +881,7 @@ struct BoxWorld {
     }
 
     // Boxes.walls, Boxes.java:357-416: every body's corners (a sphere's nearest point) against the six walls.
-    void walls(Sweep sweep, bool reverse) {
+    void walls(Sweep sweep, bool reverse, float *res = nullptr) {
         Bufs B = bufs();
         const float *P = B.params;
         float extent[3] = {P[SX], P[SY], P[SZ]};
@@ -892,7 +904,7 @@ struct BoxWorld {
                     size_t tangentAt = at * 3;
                     float reach = box.at[axis] + r[axis];
                     float gap = high ? extent[axis] - reach : reach;
-                    contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, soft);
+                    contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, soft, res);
                     roll(sweep, nullptr, box, box.half[0], B.lambda, at, B.roll, tangentAt, P);
                 } else {
                     for (int k = 0; k < CORNERS; k++) {
@@ -907,7 +919,8 @@ struct BoxWorld {
                         size_t tangentAt = at * 3;
                         float reach = box.at[axis] + r[axis];
                         float gap = high ? extent[axis] - reach : reach;
-                        contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, soft);
+                        contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, soft,
+                                res);
                     }
                 }
             }
@@ -916,7 +929,7 @@ struct BoxWorld {
     }
 
     // BoxPairs.solve, BoxPairs.java:64-151: every pair of round r, one sweep over its points.
-    void solve_round(Sweep sweep, bool reverse, int r) {
+    void solve_round(Sweep sweep, bool reverse, int r, float *res = nullptr) {
         Bufs B = bufs();
         const float *P = B.params;
         int32_t *rounds = i("rounds"), *pairs = i("pairs");
@@ -950,7 +963,7 @@ struct BoxWorld {
                 gap = gap + ((second.at[2] + rB[2]) - (first.at[2] + rA[2])) * nrm[2];
                 size_t normalAt = held + POINT_IMPULSE * slot;
                 size_t tangentAt = normalAt + 1;
-                contact(sweep, gap, nrm, &first, rA, second, rB, impulses, normalAt, impulses, tangentAt, soft);
+                contact(sweep, gap, nrm, &first, rA, second, rB, impulses, normalAt, impulses, tangentAt, soft, res);
                 if (first.sphere() || second.sphere()) {
                     float radius = second.half[0];
                     if (first.sphere()) radius = first.half[0];
@@ -964,11 +977,42 @@ struct BoxWorld {
         }
     }
 
-    // BoxStep.sweep, BoxStep.java:217-231.
-    void sweep(Sweep kind, bool reverse, int rounds) {
-        if (reverse) walls(kind, reverse);
-        for (int order = 0; order < rounds; order++) solve_round(kind, reverse, reverse ? rounds - 1 - order : order);
-        if (!reverse) walls(kind, reverse);
+    // BoxStep.sweep, BoxStep.java:217-231; with `res`, the largest residual of its points.
+    void sweep(Sweep kind, bool reverse, int rounds, float *res = nullptr) {
+        if (reverse) walls(kind, reverse, res);
+        for (int order = 0; order < rounds; order++)
+            solve_round(kind, reverse, reverse ? rounds - 1 - order : order, res);
+        if (!reverse) walls(kind, reverse, res);
+    }
+
+    // Sweeps stopped on a residual (no counterpart in the reference): with tol at zero or more, a phase's sweeps end
+    // after the first whose points' residuals are all at most tol; measured, each phase's sweeps, last residual and
+    // every sweep's largest residual are kept.
+    float tol = -1.f;
+    bool measure = false;
+    vr_box_solve lastSolve{};
+    std::vector<float> curve = std::vector<float>((size_t)2 * VR_SOLVE_SUBSTEPS * VR_SOLVE_SWEEPS, 0.f);
+
+    // The solve (or relax) sweeps of substep `sub`; returns how many ran.
+    int phase(Sweep kind, int sub, int used) {
+        bool stop = tol >= 0, kept = measure && sub < VR_SOLVE_SUBSTEPS;
+        int at = 2 * sub + (kind == RELAX ? 1 : 0);
+        float last = 0.f;
+        int ran = 0;
+        for (int k = 0; k < d.sweeps; k++) {
+            float res = 0.f;
+            sweep(kind, d.symmetric && k % 2 == 1, used, (stop || kept) ? &res : nullptr);
+            ran = k + 1;
+            last = res;
+            if (kept && k < VR_SOLVE_SWEEPS) keep_largest(curve[(size_t)at * VR_SOLVE_SWEEPS + k], res);
+            if (stop && res <= tol) break;
+        }
+        if (kept) {
+            lastSolve.scene_sweeps[at % 2][sub] = (float)ran;
+            lastSolve.residual[at % 2][sub] = last;
+        }
+        if (sub < VR_SOLVE_SUBSTEPS) lastSolve.launched[at % 2][sub] = ran;
+        return ran;
     }
 
     // ---- the broad phase (BoxBroad.java) and the collide ----
@@ -1274,12 +1318,20 @@ struct BoxWorld {
     // BoxStepper.step, BoxStepper.java:56-100, with BoxStep.collide and BoxStep.step (BoxStep.java:187-214).
     void step(vr_step_report *rep) {
         int used = prepare(rep);
+        std::fill(curve.begin(), curve.end(), 0.f);
+        vr_box_solve fresh{};
+        fresh.tolerance = tol;
+        fresh.measured = measure ? 1 : 0;
+        fresh.substeps = d.substeps;
+        fresh.sweeps = d.sweeps;
+        fresh.scenes = 1;
+        lastSolve = fresh;
         for (int sub = 0; sub < d.substeps; sub++) {
             velocities();
             sweep(WARM, false, used);
-            for (int k = 0; k < d.sweeps; k++) sweep(SOLVE, d.symmetric && k % 2 == 1, used);
+            phase(SOLVE, sub, used);
             positions();
-            for (int k = 0; k < d.sweeps; k++) sweep(RELAX, d.symmetric && k % 2 == 1, used);
+            phase(RELAX, sub, used);
         }
         show();
         if (rep) rep->submissions = slots > 0 ? 2 : 1;  // BoxStepper.step: collide read back, then the step
@@ -1324,6 +1376,15 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n) {
 void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
 
 void box_collide_list(BoxWorld *w, vr_step_report *rep) { w->prepare(rep); }
+
+void box_tolerance(BoxWorld *w, float tol, bool measure) {
+    if (tol != tol) throw std::invalid_argument("a tolerance is a number (below zero for none)");
+    w->tol = tol;
+    w->measure = measure;
+}
+
+const vr_box_solve &box_solve(BoxWorld *w) { return w->lastSolve; }
+const std::vector<float> &box_curve(BoxWorld *w) { return w->curve; }
 
 void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold) {
     if (npairs < 0) throw std::invalid_argument("a count of pairs is at least 0, got " + std::to_string(npairs));

@@ -3,6 +3,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../include/vexelray_rigid.h"
 
@@ -18,6 +19,10 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n);
 void box_step(BoxWorld *w, vr_step_report *rep);
 // BoxStepper.step up to the rounds' write: the collide list and the colouring, no substep.
 void box_collide_list(BoxWorld *w, vr_step_report *rep);
+// Sweeps stopped on a residual, as vr_box_tolerance; what the last step's phases did, and its sweeps' residuals.
+void box_tolerance(BoxWorld *w, float tol, bool measure);
+const vr_box_solve &box_solve(BoxWorld *w);
+const std::vector<float> &box_curve(BoxWorld *w);
 // The narrow phase alone over pairs (first, second), MANIFOLD_WORDS floats a pair out.
 void box_collide(BoxWorld *w, const int32_t *pairs, int32_t npairs, float *manifold);
 

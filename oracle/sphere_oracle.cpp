@@ -1552,6 +1552,65 @@ int vro_world_diagnostics(vro_world *p, double *out, int32_t n) {
 int vro_world_diagnostics_static(vro_world *p, double *out, int32_t n) { return vro_world_diagnostics(p, out, n); }
 
 // The learned warm start runs on the device only: the oracle's box worlds keep the reference's carry.
+int vro_box_tolerance(vro_world *p, float tolerance, int32_t measure) {
+    if (p->kind != VR_BOXES) return fail(VR_ERR_STATE, "not a box world");
+    try {
+        if (p->batch.empty()) vro::box_tolerance(p->box, tolerance, measure != 0);
+        for (auto *b : p->batch) vro::box_tolerance(b, tolerance, measure != 0);
+    } catch (const std::invalid_argument &e) {
+        return fail(VR_ERR_ARGUMENT, e.what());
+    }
+    return VR_OK;
+}
+
+// A batch's report gathers its scenes' own: the most any scene launched, the mean of their sweeps, the largest of
+// their residuals.
+int vro_box_solve_read(vro_world *p, vr_box_solve *out) {
+    if (!out) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (p->kind != VR_BOXES) return fail(VR_ERR_STATE, "not a box world");
+    if (p->batch.empty()) {
+        *out = vro::box_solve(p->box);
+        return VR_OK;
+    }
+    vr_box_solve r = vro::box_solve(p->batch[0]);
+    r.scenes = (int32_t)p->batch.size();
+    for (int k = 0; k < 2; k++)
+        for (int sub = 0; sub < VR_SOLVE_SUBSTEPS; sub++) {
+            r.launched[k][sub] = 0;
+            r.scene_sweeps[k][sub] = 0;
+            r.residual[k][sub] = 0;
+        }
+    for (auto *b : p->batch) {
+        const vr_box_solve &one = vro::box_solve(b);
+        for (int k = 0; k < 2; k++)
+            for (int sub = 0; sub < VR_SOLVE_SUBSTEPS; sub++) {
+                r.launched[k][sub] = std::max(r.launched[k][sub], one.launched[k][sub]);
+                r.scene_sweeps[k][sub] += one.scene_sweeps[k][sub] / (float)p->batch.size();
+                float v = one.residual[k][sub];
+                if (r.residual[k][sub] == r.residual[k][sub] && !(v <= r.residual[k][sub])) r.residual[k][sub] = v;
+            }
+    }
+    *out = r;
+    return VR_OK;
+}
+
+int vro_box_solve_curve(vro_world *p, float *out, int32_t n, int32_t *count) {
+    if (!count || n < 0 || (n > 0 && !out)) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (p->kind != VR_BOXES) return fail(VR_ERR_STATE, "not a box world");
+    std::vector<float> c(vro::box_curve(p->batch.empty() ? p->box : p->batch[0]).size(), 0.f);
+    std::vector<vro::BoxWorld *> worlds = p->batch;
+    if (worlds.empty()) worlds.push_back(p->box);
+    for (auto *b : worlds) {
+        const std::vector<float> &one = vro::box_curve(b);
+        for (size_t k = 0; k < c.size(); k++)
+            if (c[k] == c[k] && !(one[k] <= c[k])) c[k] = one[k];
+    }
+    int32_t take = std::min<int32_t>(n, (int32_t)c.size());
+    std::memcpy(out, c.data(), sizeof(float) * (size_t)take);
+    *count = take;
+    return VR_OK;
+}
+
 int vro_box_log(vro_world *, int32_t, int64_t) { return fail(VR_ERR_STATE, "the oracle keeps no sample log"); }
 int vro_box_samples(vro_world *, float *, float *, int64_t, int64_t *) {
     return fail(VR_ERR_STATE, "the oracle keeps no sample log");

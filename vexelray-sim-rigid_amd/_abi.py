@@ -28,7 +28,7 @@ ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "wor
                 "world_profile_read", "box_collide", "batch_create", "mlp_create", "mlp_destroy", "mlp_read",
                 "mlp_write", "mlp_forward", "mlp_gradient", "mlp_train", "mlp_last_error", "box_log",
                 "box_samples", "box_learned", "world_diagnostics", "box_collide_list",
-                "world_diagnostics_static")
+                "world_diagnostics_static", "box_tolerance", "box_solve_read", "box_solve_curve")
 
 
 class StepDesc(ctypes.Structure):
@@ -45,6 +45,20 @@ class StepReport(ctypes.Structure):
                 ("unlisted", ctypes.c_int32), ("touching", ctypes.c_int32), ("points", ctypes.c_int32),
                 ("unplaced", ctypes.c_int32), ("reserved", ctypes.c_int32), ("wall_ns", ctypes.c_int64),
                 ("gpu_ns", ctypes.c_int64), ("submissions", ctypes.c_int32), ("unused", ctypes.c_int32)]
+
+
+SOLVE_SUBSTEPS = 64
+SOLVE_SWEEPS = 64
+
+
+class SolveReport(ctypes.Structure):
+    """vr_box_solve: what the last step's phases did (index 0 the solve sweeps, 1 the relax)."""
+    _fields_ = [("tolerance", ctypes.c_float), ("measured", ctypes.c_int32), ("substeps", ctypes.c_int32),
+                ("sweeps", ctypes.c_int32), ("scenes", ctypes.c_int32), ("scene_step", ctypes.c_int32),
+                ("fell_back", ctypes.c_int32), ("unused", ctypes.c_int32),
+                ("launched", (ctypes.c_int32 * SOLVE_SUBSTEPS) * 2),
+                ("scene_sweeps", (ctypes.c_float * SOLVE_SUBSTEPS) * 2),
+                ("residual", (ctypes.c_float * SOLVE_SUBSTEPS) * 2)]
 
 
 class Profile(ctypes.Structure):
@@ -82,6 +96,9 @@ class Abi:
             "world_profile_read": ([p, ctypes.POINTER(Profile)]),
             "box_collide": ([p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
             "box_collide_list": ([p, ctypes.POINTER(StepReport)]),
+            "box_tolerance": ([p, ctypes.c_float, ctypes.c_int32]),
+            "box_solve_read": ([p, ctypes.POINTER(SolveReport)]),
+            "box_solve_curve": ([p, ctypes.POINTER(ctypes.c_float), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
             "mlp_create": ([ctypes.c_uint64, ctypes.POINTER(p)]),
             "mlp_destroy": ([p]),
             "mlp_read": ([p, ctypes.POINTER(ctypes.c_float)]),
@@ -246,6 +263,21 @@ class World:
         rep = StepReport()
         self._check(self.abi.fn("box_collide_list")(self.handle, ctypes.byref(rep)))
         return rep
+
+    def tolerance(self, tol: float, measure: bool) -> None:
+        self._check(self.abi.fn("box_tolerance")(self.handle, ctypes.c_float(tol), 1 if measure else 0))
+
+    def solve_read(self) -> SolveReport:
+        out = SolveReport()
+        self._check(self.abi.fn("box_solve_read")(self.handle, ctypes.byref(out)))
+        return out
+
+    def solve_curve(self) -> np.ndarray:
+        out = np.zeros(2 * SOLVE_SUBSTEPS * SOLVE_SWEEPS, dtype=np.float32)
+        count = ctypes.c_int32()
+        self._check(self.abi.fn("box_solve_curve")(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                   out.size, ctypes.byref(count)))
+        return out[:count.value].reshape(-1, 2, SOLVE_SWEEPS) if count.value == out.size else out[:count.value]
 
     def destroy(self) -> None:
         if self.handle:

@@ -140,6 +140,30 @@ class BoxReport:
     unlisted: int = 0
 
 
+@dataclass
+class SolveReport:
+    """vr_box_solve over the step's substeps: per substep, sweeps launched and (measured) sweeps a scene ran on average
+    and the largest residual a scene ended with, for the solve and the relax; and the pairs the learned start left to
+    the carry."""
+
+    tolerance: float
+    measured: bool
+    scene_step: bool
+    launched: np.ndarray      # 2 x substeps
+    scene_sweeps: np.ndarray  # 2 x substeps
+    residual: np.ndarray      # 2 x substeps
+    fell_back: int
+
+    @staticmethod
+    def of(raw, substeps) -> "SolveReport":
+        m = min(substeps, _abi.SOLVE_SUBSTEPS)
+        grab = lambda field, dtype: np.array([[getattr(raw, field)[k][s] for s in range(m)] for k in range(2)],  # noqa
+                                             dtype=dtype)
+        return SolveReport(float(raw.tolerance), bool(raw.measured), bool(raw.scene_step),
+                           grab("launched", np.int32), grab("scene_sweeps", np.float32),
+                           grab("residual", np.float32), int(raw.fell_back))
+
+
 class BoxSimulation:
     """Boxes stepped on resident device buffers (``gui/.../BoxSimulation.java``)."""
 
@@ -215,6 +239,24 @@ class BoxSimulation:
     def collide(self, pairs) -> np.ndarray:
         """The narrow phase alone over ``pairs`` (first, second, first, second, ...)."""
         return self._world.collide(pairs)
+
+    # Sweeps stopped on a residual (no counterpart in the reference): vr_box_tolerance / vr_box_solve_read /
+    # vr_box_solve_curve.
+    def tolerance(self, tol: float, measure: bool = False) -> None:
+        """Each scene's solve (and relax) sweeps of a substep end after the first sweep whose points' residuals (the
+        impulse change times the inverse mass along the normal, m/s) are all at most `tol`; below zero, every sweep
+        runs (the reference). With `measure`, each step keeps its phases' sweeps and residuals."""
+        if tol != tol:
+            raise IllegalArgumentException("a tolerance is a number (below zero for none)")
+        self._world.tolerance(float(tol), measure)
+
+    def solveReport(self) -> "SolveReport":
+        """What the last step's phases did (vr_box_solve)."""
+        return SolveReport.of(self._world.solve_read(), self.step_.substeps)
+
+    def solveCurve(self) -> np.ndarray:
+        """The last measured step's largest residual of each sweep: [substep][solve, relax][sweep]."""
+        return self._world.solve_curve()
 
     # The learned warm start (no counterpart in the reference): vr_box_log / vr_box_samples / vr_box_learned.
     FEATURES, TARGETS = 16, 3

@@ -20,8 +20,40 @@ struct SceneDev {
     float cell, sx, sy, sz;
 };
 
+// The sweeps stopped on a residual (no counterpart in the reference, whose substeps run a fixed count of sweeps of the
+// solve and of the relax). Each scene's solve (or relax) sweeps of a substep end after the first sweep whose points'
+// residuals (box_math.h point_residual) are all at most the tolerance, or at the count the step allows; so a scene's
+// state depends on its own residuals alone, never on how many sweeps the batch's other scenes needed. Sweeps are
+// numbered over the world's life (`g`); a scene takes part in sweep g when g is the first of its phase or the scene
+// had a point past the tolerance in sweep g - 1.
+struct SweepCtl {
+    int32_t *sceneOpen;            // scenes: the last sweep in which the scene had a point past the tolerance
+    int32_t *lastOpen;             // 1: the last sweep in which any scene did
+    // Measured steps: per scene (last sweep it took part in << 32 | bits of its largest residual there), the most;
+    // per phase (substep * 2 + relax) the scenes' sweeps added up and their last residuals' largest, and per phase and
+    // sweep the largest residual of any point.
+    unsigned long long *sceneLast;
+    int32_t *phaseSweeps;
+    uint32_t *phaseResidual;
+    uint32_t *curve;
+};
+
+// What one sweep's launches are told: its number, whether it opens its phase, the tolerance, and where a measured
+// sweep keeps its largest residual (-1: not measured).
+struct SweepArgs {
+    int g, first;
+    float tol;
+    int curveAt;
+};
+
+// Sweep modes: plain (the reference's fixed count), stopped on the residual, measured, or both.
+enum { SWEEP_STOP = 1, SWEEP_MEASURE = 2 };
+constexpr int SOLVE_MOST_SUBSTEPS = 64;   // phases measured: two a substep
+constexpr int SOLVE_MOST_SWEEPS = 64;     // sweeps of a phase whose residuals a measured step keeps
+
 struct BoxDev {
     int n, slots, width, cells;
+    SweepCtl sw;
     int scenes;
     const int32_t *scene;       // n: each body's scene
     const SceneDev *sceneInfo;  // scenes
@@ -56,6 +88,7 @@ struct BoxDev {
     int32_t *packPair;     // the pair's slot
     int32_t *packCount;    // its manifold's point count
     int32_t *packBody;     // 2 fields: its first body, then its second
+    int32_t *packScene;    // the pair's scene
     float *packNormal;     // 3 fields
     float *packAnchor;     // 24 fields: field (side * MOST_POINTS + point) * 3 + axis, side 0 the first body's
     float *packImpulse;    // 28 fields: field point * POINT_IMPULSE + word
@@ -99,8 +132,9 @@ struct Learned {
 
 void launch_point_features(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs, hipStream_t s);
 void launch_point_targets(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs, hipStream_t s);
+// fellBack (1 word): pairs left to the carry because the prediction raised their start residual, added to.
 void launch_apply_prediction(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs,
-                             hipStream_t s);
+                             int32_t *fellBack, hipStream_t s);
 
 int64_t colour_tmp_words(int64_t slots);
 void launch_colour_prepare(const BoxDev &D, const Colouring &C, const vr::SortScratch &S, hipStream_t s);
@@ -114,17 +148,22 @@ void launch_velocities(const BoxDev &D, bool motion, hipStream_t s);
 void launch_scene_from_params(const BoxDev &D, SceneDev *scene, hipStream_t s);
 // With `frame`, also the moving bodies' frame records' position and turn.
 void launch_positions(const BoxDev &D, bool frame, hipStream_t s);
-void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s);
+void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s, int mode = 0,
+                  const SweepArgs &a = SweepArgs{0, 1, -1.f, -1});
+// At the end of a measured phase that began with sweep g0: each scene's sweeps and last residual added into `phase`.
+void launch_phase_measure(const BoxDev &D, int g0, int phase, hipStream_t s);
 // The wall sweep's places for the bodies' shapes into `place` (n words), as the sweep takes the bodies.
 void launch_wall_place(const BoxDev &D, int32_t *place, hipStream_t s);
 // Each body's wall impulse words moved from place `from[s]` to `to[s]` (both within the body's workgroup of bodies).
 void launch_wall_move(const BoxDev &D, const int32_t *from, const int32_t *to, hipStream_t s);
 // Every substep of a step (velocities, the warm, solve and relax sweeps, positions), a workgroup a scene, for a batch
 // of scenes (k_scene_step). starts: the colouring's round starts (device); used: the step's rounds.
+// mode and tol: the sweeps stopped on the residual and measured as the pass-by-pass step stops and measures them.
 void launch_scene_step(const BoxDev &D, const int32_t *sceneFirst, const int32_t *starts, int used, int substeps,
-                       int sweeps, bool symmetric, hipStream_t s);
+                       int sweeps, bool symmetric, hipStream_t s, int mode = 0, float tol = -1.f);
 // Round r's pairs, packed entries base .. base + pairs - 1.
-void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int base, int pairs, hipStream_t s);
+void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int base, int pairs, hipStream_t s, int mode = 0,
+                        const SweepArgs &a = SweepArgs{0, 1, -1.f, -1});
 // The touching pairs (entries 0 .. touching - 1 of the colouring's order) into the packed fields, and the packed
 // impulses back to the pairs' slots.
 void launch_pack(const BoxDev &D, const Colouring &C, int touching, hipStream_t s);

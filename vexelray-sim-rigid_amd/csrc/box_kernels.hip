@@ -147,7 +147,7 @@ __device__ __forceinline__ float pick(const float *v, int a) { return a == 0 ? v
 // +0 alone, which does not need its words. Returns the word as the sweep leaves it.
 template <int SWEEP, bool REVERSE>
 __device__ unsigned long long ball_walls(const Bufs &B, Moving &box, const float *extent, unsigned long long held,
-                                         size_t n, int place, const Soft &soft, const float *P) {
+                                         size_t n, int place, const Soft &soft, const float *P, float *res) {
     const Sweep sweep = (Sweep)SWEEP;
     bool still = !(P[MU_ROLLING] > 0);  // Boxes.roll's turning stop is taken only with rolling friction
     // Moving::turn by a zero change adds the zero change to the momentum and its inverse-inertia product to the spin;
@@ -180,7 +180,7 @@ __device__ unsigned long long ball_walls(const Bufs &B, Moving &box, const float
         float tangent[3] = {B.rub[tangentAt], B.rub[tangentAt + n], B.rub[tangentAt + 2 * n]};
         float was[4] = {normal, tangent[0], tangent[1], tangent[2]};
         bool wrote;
-        contact_held(sweep, gap, nrm, nullptr, nullptr, box, r, normal, tangent, wrote, soft);
+        contact_held(sweep, gap, nrm, nullptr, nullptr, box, r, normal, tangent, wrote, soft, res);
         if (wrote) {  // a word written with the bits it holds is left as it is
             if (bits(normal) != bits(was[0])) B.lambda[at] = normal;
             for (int a = 0; a < 3; a++)
@@ -257,7 +257,8 @@ __device__ unsigned long long corner_list(const float *at, const float *q, const
 // after it, as cornerHeld has them.
 template <int SWEEP, bool REVERSE>
 __device__ unsigned long long corner_sweep(const Bufs &B, Moving &box, const float *extent, unsigned long long held,
-                                           unsigned long long todo, size_t n, int place, const Soft &soft) {
+                                           unsigned long long todo, size_t n, int place, const Soft &soft,
+                                           float *res) {
     const Sweep sweep = (Sweep)SWEEP;
     unsigned long long nowHeld = 0;
     while (__any(todo != 0)) {
@@ -282,7 +283,7 @@ __device__ unsigned long long corner_sweep(const Bufs &B, Moving &box, const flo
         // A corner holding nothing and outside the margin returns at once from Boxes.contact.
         if (!holding && !(gap < soft.margin)) continue;
         size_t at = (size_t)(wall * CORNERS + c) * n + place, tangentAt = (size_t)(wall * CORNERS + c) * 3 * n + place;
-        if (contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft))
+        if (contact(sweep, gap, nrm, nullptr, nullptr, box, r, B.lambda, at, B.rub, tangentAt, n, soft, res))
             nowHeld |= 1ull << (wall * CORNERS + c);
     }
     return nowHeld;
@@ -291,7 +292,7 @@ __device__ unsigned long long corner_sweep(const Bufs &B, Moving &box, const flo
 // Body s, its wall impulse words at place `place`. Its cornerHeld word is rewritten by a solve or relax sweep (a
 // warm sweep changes no impulse).
 template <int SWEEP, bool REVERSE, bool MOTION = false>
-__device__ void walls_of(const BoxDev &D, int s, int place) {
+__device__ void walls_of(const BoxDev &D, int s, int place, float *res = nullptr) {
     if (!(D.im[s] > 0)) return;
     Bufs B = bufs(D);
     const float *P = D.params;
@@ -303,10 +304,10 @@ __device__ void walls_of(const BoxDev &D, int s, int place) {
     size_t n = (size_t)D.n;
     unsigned long long held = D.cornerHeld[s], nowHeld;
     if (box.sphere()) {
-        nowHeld = ball_walls<SWEEP, REVERSE>(B, box, extent, held, n, place, soft, P);
+        nowHeld = ball_walls<SWEEP, REVERSE>(B, box, extent, held, n, place, soft, P, res);
     } else {
         unsigned long long todo = corner_list<SWEEP, REVERSE>(box.at, box.q, box.half, extent, held, soft);
-        nowHeld = corner_sweep<SWEEP, REVERSE>(B, box, extent, held, todo, n, place, soft);
+        nowHeld = corner_sweep<SWEEP, REVERSE>(B, box, extent, held, todo, n, place, soft, res);
     }
     if (SWEEP != WARM && nowHeld != held) D.cornerHeld[s] = nowHeld;
     if (MOTION)
@@ -318,10 +319,41 @@ __device__ void walls_of(const BoxDev &D, int s, int place) {
 // The sweep over all bodies. A workgroup takes its THREADS consecutive bodies with the boxes first and then the balls
 // (shape_first), so its waves run the box path or the ball path, not one after the other, while the workgroup's
 // loads still fall in the same cache lines as in body order.
-template <int SWEEP, bool REVERSE>
-__global__ void __launch_bounds__(THREADS) k_walls(BoxDev D) {
+// Whether scene `scene` takes part in sweep a.g (the first of its phase, or the scene was open in the sweep before).
+__device__ __forceinline__ bool sweeping(const BoxDev &D, const SweepArgs &a, int scene) {
+    return a.first || D.sw.sceneOpen[scene] >= a.g - 1;
+}
+
+// What a lane of a stopped or measured sweep leaves of its residual `res` for scene `scene`: past the tolerance,
+// the scene (and the world) open in this sweep; measured, the scene's last sweep and residual, and the sweep's largest.
+// Residuals are at least zero (or NaN), so their bits order as they do.
+template <int MODE>
+__device__ __forceinline__ void record(const BoxDev &D, const SweepArgs &a, int scene, float res) {
+    if ((MODE & SWEEP_STOP) && !(res <= a.tol)) {
+        D.sw.sceneOpen[scene] = a.g;
+        *D.sw.lastOpen = a.g;
+    }
+    if (MODE & SWEEP_MEASURE) {
+        unsigned long long key = ((unsigned long long)(unsigned)a.g << 32) | (unsigned long long)__float_as_uint(res);
+        atomicMax(D.sw.sceneLast + scene, key);
+        if (a.curveAt >= 0) atomicMax(D.sw.curve + a.curveAt, __float_as_uint(res));
+    }
+}
+
+template <int SWEEP, bool REVERSE, int MODE>
+__global__ void __launch_bounds__(THREADS) k_walls(BoxDev D, SweepArgs a) {
     int s = shape_first(D);
-    if (s < D.n) walls_of<SWEEP, REVERSE, true>(D, s, blockIdx.x * THREADS + threadIdx.x);
+    if (s >= D.n) return;
+    int place = blockIdx.x * THREADS + threadIdx.x;
+    if (MODE == 0) {
+        walls_of<SWEEP, REVERSE, true>(D, s, place);
+        return;
+    }
+    int scene = D.scene[s];
+    if ((MODE & SWEEP_STOP) && !sweeping(D, a, scene)) return;
+    float res = 0.f;
+    walls_of<SWEEP, REVERSE, true>(D, s, place, &res);
+    record<MODE>(D, a, scene, res);
 }
 
 __global__ void __launch_bounds__(THREADS) k_wall_place(BoxDev D, int32_t *place) {
@@ -356,7 +388,7 @@ struct HeldPoint {
 };
 
 template <int SWEEP, bool REVERSE>
-__device__ void solve_pair(const BoxDev &D, int pair) {
+__device__ void solve_pair(const BoxDev &D, int pair, float *res = nullptr) {
     if (pair < 0) return;
     size_t out = (size_t)pair * MANIFOLD_WORDS;
     size_t held = (size_t)pair * IMPULSE_WORDS;
@@ -398,7 +430,7 @@ __device__ void solve_pair(const BoxDev &D, int pair) {
         float gap = ((second.at[0] + rB[0]) - (first.at[0] + rA[0])) * nrm[0] +
                     ((second.at[1] + rB[1]) - (first.at[1] + rA[1])) * nrm[1];
         gap = gap + ((second.at[2] + rB[2]) - (first.at[2] + rA[2])) * nrm[2];
-        contact_held(sweep, gap, nrm, &first, rA, second, rB, p.imp[0], &p.imp[1], p.wroteContact, soft);
+        contact_held(sweep, gap, nrm, &first, rA, second, rB, p.imp[0], &p.imp[1], p.wroteContact, soft, res);
         if (first.sphere() || second.sphere()) {
             float radius = second.half[0];
             if (first.sphere()) radius = first.half[0];
@@ -499,7 +531,7 @@ __device__ __forceinline__ void apply_angular_side(Side &S, const float *j) {
 
 // contact_held (Boxes.contact, Boxes.java:462-542) for the pair's point, with r this lane's body's arm.
 __device__ bool contact_side(Sweep sweep, float gap, const float *n, Side &S, const float *r, float &normalHeld,
-                             float *tangentHeld, bool &wrote, const Soft &soft) {
+                             float *tangentHeld, bool &wrote, const Soft &soft, float *res) {
     wrote = false;
     float normal = normalHeld;
     float tangent[3] = {tangentHeld[0], tangentHeld[1], tangentHeld[2]};
@@ -551,6 +583,7 @@ __device__ bool contact_side(Sweep sweep, float gap, const float *n, Side &S, co
     float rubbed[3];
     for (int a = 0; a < 3; a++) rubbed[a] = whole[a] - tangent[a];
     apply_side(S, r, rubbed);
+    if (res) keep_largest(*res, point_residual(k, total - normal, rubbed));
     normalHeld = total;
     for (int a = 0; a < 3; a++) tangentHeld[a] = whole[a];
     wrote = true;
@@ -605,7 +638,7 @@ __device__ void roll_side(Sweep sweep, Side &S, float radius, float normal, floa
 // of a pair call it together with the same entry. Each lane loads its own body's anchors and the impulses of the
 // points the pair has.
 template <int SWEEP, bool REVERSE>
-__device__ void solve_pair_side(const BoxDev &D, int j, bool second) {
+__device__ void solve_pair_side(const BoxDev &D, int j, bool second, float *res = nullptr) {
     size_t T = (size_t)D.packCap;
     int side = second ? 1 : 0;
     int count = D.packCount[j];
@@ -655,7 +688,7 @@ __device__ void solve_pair_side(const BoxDev &D, int j, bool second) {
         both(S, at[2], f2, s2);
         float gap = (s0 - f0) * nrm[0] + (s1 - f1) * nrm[1];
         gap = gap + (s2 - f2) * nrm[2];
-        contact_side(sweep, gap, nrm, S, r, p.imp[0], &p.imp[1], p.wroteContact, soft);
+        contact_side(sweep, gap, nrm, S, r, p.imp[0], &p.imp[1], p.wroteContact, soft, res);
         if (firstSphere || secondSphere) {
             float radius = halfSecond;
             if (firstSphere) radius = halfFirst;
@@ -691,11 +724,23 @@ __device__ void solve_pair_side(const BoxDev &D, int j, bool second) {
 }
 
 // A round's pairs, packed entries base .. base + pairs - 1, two neighbouring lanes an entry (solve_pair_side).
-template <int SWEEP, bool REVERSE>
-__global__ void __launch_bounds__(THREADS) k_solve_round(BoxDev D, int base, int pairs) {
+// Stopped or measured (MODE), a pair of a scene that has stopped is passed over by both its lanes, and the first
+// lane records the pair's residual (both lanes hold the same).
+template <int SWEEP, bool REVERSE, int MODE>
+__global__ void __launch_bounds__(THREADS) k_solve_round(BoxDev D, int base, int pairs, SweepArgs a) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int k = t >> 1;
-    if (k < pairs) solve_pair_side<SWEEP, REVERSE>(D, base + k, (t & 1) != 0);
+    if (k >= pairs) return;
+    bool second = (t & 1) != 0;
+    if (MODE == 0) {
+        solve_pair_side<SWEEP, REVERSE>(D, base + k, second);
+        return;
+    }
+    int scene = D.packScene[base + k];
+    if ((MODE & SWEEP_STOP) && !sweeping(D, a, scene)) return;
+    float res = 0.f;
+    solve_pair_side<SWEEP, REVERSE>(D, base + k, second, &res);
+    if (!second) record<MODE>(D, a, scene, res);
 }
 
 // Body s's frame record from the state, and with `motion` its motion record.
@@ -740,6 +785,7 @@ __global__ void k_pack_pairs(BoxDev D, Colouring C, int touching) {
     D.packCount[j] = count;
     D.packBody[j] = D.pairs[2 * (size_t)pair];
     D.packBody[T + j] = D.pairs[2 * (size_t)pair + 1];
+    D.packScene[j] = D.scene[D.packBody[j]];
     for (int a = 0; a < 3; a++) D.packNormal[a * T + j] = m[a];
     for (int k = 0; k < MOST_POINTS; k++) {
         if (!(k < count)) break;
@@ -791,21 +837,53 @@ __device__ int first_pair_of(const BoxDev &D, const int32_t *list, int count, in
 }
 
 template <int SWEEP, bool REVERSE>
-__device__ void scene_sweep(const BoxDev &D, int b0, int b1, const int *from, const int *to, int used) {
+__device__ void scene_sweep(const BoxDev &D, int b0, int b1, const int *from, const int *to, int used, float *res) {
     if (REVERSE) {
-        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) walls_of<SWEEP, REVERSE>(D, s, s);
+        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) walls_of<SWEEP, REVERSE>(D, s, s, res);
         __syncthreads();
     }
     for (int order = 0; order < used; order++) {  // BoxStep.sweep, BoxStep.java:217-231
         int r = REVERSE ? used - 1 - order : order;
         if (from[r] == to[r]) continue;  // no pair of this scene in round r: nothing to wait for
         const int32_t *list = D.rounds + (size_t)r * D.width;
-        for (int k = from[r] + (int)threadIdx.x; k < to[r]; k += SCENE_THREADS) solve_pair<SWEEP, REVERSE>(D, list[k]);
+        for (int k = from[r] + (int)threadIdx.x; k < to[r]; k += SCENE_THREADS)
+            solve_pair<SWEEP, REVERSE>(D, list[k], res);
         __syncthreads();
     }
     if (!REVERSE) {
-        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) walls_of<SWEEP, REVERSE>(D, s, s);
+        for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) walls_of<SWEEP, REVERSE>(D, s, s, res);
         __syncthreads();
+    }
+}
+
+// One phase of a substep in the scene step: up to `sweeps` sweeps of the solve or the relax. With `stop`, the sweeps
+// end after the first whose residuals are all at most `tol`, as the pass-by-pass step ends them; with `measure`
+// (phase index `phase`), the scene's sweeps and last residual, and each sweep's largest, go where that step puts them.
+template <int SWEEP>
+__device__ void scene_phase(const BoxDev &D, int b0, int b1, const int *from, const int *to, int used, int sweeps,
+                            int symmetric, bool stop, bool measure, float tol, int phase, unsigned *largest) {
+    bool residuals = stop || measure;
+    unsigned last = 0;
+    int ran = 0;
+    for (int k = 0; k < sweeps; k++) {
+        float res = 0.f;
+        if (threadIdx.x == 0) *largest = 0u;
+        __syncthreads();
+        if (symmetric && k % 2 == 1) scene_sweep<SWEEP, true>(D, b0, b1, from, to, used, residuals ? &res : nullptr);
+        else scene_sweep<SWEEP, false>(D, b0, b1, from, to, used, residuals ? &res : nullptr);
+        ran = k + 1;
+        if (!residuals) continue;
+        atomicMax(largest, __float_as_uint(res));
+        __syncthreads();
+        last = *largest;
+        if (measure && threadIdx.x == 0 && k < SOLVE_MOST_SWEEPS)
+            atomicMax(D.sw.curve + (size_t)phase * SOLVE_MOST_SWEEPS + k, last);
+        __syncthreads();  // every thread has read *largest before the next sweep clears it
+        if (stop && __uint_as_float(last) <= tol) break;
+    }
+    if (measure && threadIdx.x == 0) {
+        atomicAdd(D.sw.phaseSweeps + phase, ran);
+        atomicMax(D.sw.phaseResidual + phase, last);
     }
 }
 
@@ -813,8 +891,9 @@ __device__ void scene_sweep(const BoxDev &D, int b0, int b1, const int *from, co
 // starts (round r has starts[r + 1] - starts[r] pairs). used: the rounds the step has.
 __global__ void __launch_bounds__(SCENE_THREADS) k_scene_step(BoxDev D, const int32_t *sceneFirst,
                                                               const int32_t *starts, int used, int substeps,
-                                                              int sweeps, int symmetric) {
+                                                              int sweeps, int symmetric, int mode, float tol) {
     __shared__ int from[MOST_ROUNDS], to[MOST_ROUNDS];
+    __shared__ unsigned largest;
     int j = blockIdx.x;
     int b0 = sceneFirst[j], b1 = sceneFirst[j + 1];
     for (int r = threadIdx.x; r < used; r += SCENE_THREADS) {
@@ -824,21 +903,34 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_step(BoxDev D, const in
         to[r] = first_pair_of(D, list, count, j + 1);
     }
     __syncthreads();
+    bool stop = (mode & SWEEP_STOP) != 0, measure = (mode & SWEEP_MEASURE) != 0;
     for (int sub = 0; sub < substeps; sub++) {
+        bool kept = measure && sub < SOLVE_MOST_SUBSTEPS;
         for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) velocity_of(D, s);
         __syncthreads();
-        scene_sweep<WARM, false>(D, b0, b1, from, to, used);
-        for (int k = 0; k < sweeps; k++) {
-            if (symmetric && k % 2 == 1) scene_sweep<SOLVE, true>(D, b0, b1, from, to, used);
-            else scene_sweep<SOLVE, false>(D, b0, b1, from, to, used);
-        }
+        scene_sweep<WARM, false>(D, b0, b1, from, to, used, nullptr);
+        scene_phase<SOLVE>(D, b0, b1, from, to, used, sweeps, symmetric, stop, kept, tol, 2 * sub, &largest);
         for (int s = b0 + (int)threadIdx.x; s < b1; s += SCENE_THREADS) position_of(D, s);
         __syncthreads();
-        for (int k = 0; k < sweeps; k++) {
-            if (symmetric && k % 2 == 1) scene_sweep<RELAX, true>(D, b0, b1, from, to, used);
-            else scene_sweep<RELAX, false>(D, b0, b1, from, to, used);
-        }
+        scene_phase<RELAX>(D, b0, b1, from, to, used, sweeps, symmetric, stop, kept, tol, 2 * sub + 1, &largest);
     }
+}
+
+// The end of a measured phase of the pass-by-pass step that began with sweep g0: each scene's sweeps (from the last
+// sweep it recorded; a scene with no point at all ran the phase's first sweep only) and the residual of its last
+// sweep, added into the phase's words.
+__global__ void k_phase_measure(BoxDev D, int g0, int phase) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= D.scenes) return;
+    unsigned long long key = D.sw.sceneLast[j];
+    int g = (int)(key >> 32);
+    unsigned res = (unsigned)(key & 0xffffffffull);
+    if (g < g0) {
+        g = g0;
+        res = 0u;
+    }
+    atomicAdd(D.sw.phaseSweeps + phase, g - g0 + 1);
+    atomicMax(D.sw.phaseResidual + phase, res);
 }
 
 // BoxBroad.bin, BoxBroad.java:74-89, in the body's scene's grid.
@@ -1256,30 +1348,93 @@ __global__ void k_point_targets(BoxDev D, Colouring C, Learned L) {
     }
 }
 
-// The prediction as the step's starting impulse, where it is usable (finite, the normal one not negative, a pair
-// with mass to push); elsewhere the carried impulse stays. The rolling words are left as carried.
-__global__ void k_apply_prediction(BoxDev D, Colouring C, Learned L) {
+// The first solve sweep of touching pair p alone, from start impulses `start` (each point's normal and tangent
+// words, 4 a point): the two bodies as the step begins with the first substep's gravity added, each point's start
+// impulse applied as the warm sweep applies it, then each point through Boxes.contact as a solve sweep takes it, in
+// order. Returns the largest residual (point_residual): how far that sweep still moves the pair's points. Other pairs
+// and the walls are left out, so a start is judged by the pair's own error alone.
+__device__ float pair_start_residual(const BoxDev &D, const Bufs &B, int p, const float (*start)[4]) {
+    size_t out = (size_t)p * MANIFOLD_WORDS;
+    int count = (int)D.manifold[out + COUNT];
+    float nrm[3] = {D.manifold[out], D.manifold[out + 1], D.manifold[out + 2]};
+    int ia = p / SLOTS, ib = D.contacts[p];
+    Moving first = load_moving(B, ia), second = load_moving(B, ib);
+    const float *P = D.params;
+    float h = P[H];
+    float g[3] = {P[GX], P[GY], P[GZ]};
+    for (int a = 0; a < 3; a++) {  // Boxes.velocities
+        if (first.im > 0) first.vel[a] = first.vel[a] + h * g[a];
+        if (second.im > 0) second.vel[a] = second.vel[a] + h * g[a];
+    }
+    Soft soft(P, false);
+    if (first.im == 0 || second.im == 0) soft.stiffen(P);
+    float rA[MOST_POINTS][3], rB[MOST_POINTS][3];
+    for (int k = 0; k < MOST_POINTS && k < count; k++) {
+        size_t where = out + POINTS + k * POINT_WORDS;
+        rotate(first.q, D.manifold + where + ANCHOR_FIRST, rA[k]);
+        rotate(second.q, D.manifold + where + ANCHOR_SECOND, rB[k]);
+        if (first.sphere())
+            for (int a = 0; a < 3; a++) rA[k][a] = first.half[0] * nrm[a];
+        if (second.sphere())
+            for (int a = 0; a < 3; a++) rB[k][a] = -(second.half[0] * nrm[a]);
+    }
+    for (int k = 0; k < MOST_POINTS && k < count; k++) {  // the warm sweep
+        float pushed[3];
+        for (int a = 0; a < 3; a++) pushed[a] = start[k][0] * nrm[a] + start[k][1 + a];
+        if (start[k][0] > 0 || dot(&start[k][1], &start[k][1]) > 0) apply_pair(&first, rA[k], second, rB[k], pushed);
+    }
+    float res = 0.f;
+    for (int k = 0; k < MOST_POINTS && k < count; k++) {  // the solve sweep
+        float gap = ((second.at[0] + rB[k][0]) - (first.at[0] + rA[k][0])) * nrm[0] +
+                    ((second.at[1] + rB[k][1]) - (first.at[1] + rA[k][1])) * nrm[1];
+        gap = gap + ((second.at[2] + rB[k][2]) - (first.at[2] + rA[k][2])) * nrm[2];
+        float normal = start[k][0], tangent[3] = {start[k][1], start[k][2], start[k][3]};
+        bool wrote;
+        contact_held(SOLVE, gap, nrm, &first, rA[k], second, rB[k], normal, tangent, wrote, soft, &res);
+    }
+    return res;
+}
+
+// The prediction as the step's starting impulse for each touching pair where it is usable (finite, no normal part
+// below zero, a pair with mass to push) and does not raise the pair's start residual (pair_start_residual) above
+// what the carried impulse leaves; elsewhere the whole pair keeps its carried impulses (the reference's start), and
+// is counted in fellBack. The rolling words are left as carried.
+__global__ void k_apply_prediction(BoxDev D, Colouring C, Learned L, int32_t *fellBack) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= *C.touching) return;
     Bufs B = bufs(D);
     int p = C.touch[t];
     int count = (int)D.manifold[(size_t)p * MANIFOLD_WORDS + COUNT];
-    int taken = 0;
-    for (int k = 0; k < count; k++) {
+    if (count > MOST_POINTS) count = MOST_POINTS;
+    float carried[MOST_POINTS][4], predicted[MOST_POINTS][4];
+    bool usable = count > 0;
+    float *held = D.impulses + (size_t)p * IMPULSE_WORDS;
+    for (int k = 0; k < MOST_POINTS; k++) {
+        for (int w = 0; w < 4; w++) carried[k][w] = predicted[k][w] = 0.f;
+        if (!(k < count)) continue;
+        for (int w = 0; w < 4; w++) carried[k][w] = predicted[k][w] = held[POINT_IMPULSE * k + w];
         PointAt P = point_at(D, B, p, k);
         const float *y = L.y + (size_t)(L.pointBase[t] + k) * LEARN_OUTPUTS;
-        bool usable = isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2]) && y[0] >= 0 &&
-                      P.first.im + P.second.im > 0;
-        if (!usable) continue;
+        if (!(isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2]) && y[0] >= 0 && P.first.im + P.second.im > 0)) {
+            usable = false;
+            continue;
+        }
         float t1[3], t2[3];
         plane_basis(P.n, t1, t2);
-        float *held = D.impulses + (size_t)p * IMPULSE_WORDS + POINT_IMPULSE * k;
         float along1 = unsquash(y[1]), along2 = unsquash(y[2]);
-        held[0] = unsquash(y[0]) / P.wsum;
-        for (int a = 0; a < 3; a++) held[1 + a] = (along1 * t1[a] + along2 * t2[a]) / P.wsum;
-        taken++;
+        predicted[k][0] = unsquash(y[0]) / P.wsum;
+        for (int a = 0; a < 3; a++) predicted[k][1 + a] = (along1 * t1[a] + along2 * t2[a]) / P.wsum;
     }
-    if (taken) atomicAdd(L.used, taken);
+    if (!usable) return;
+    float fromCarry = pair_start_residual(D, B, p, carried);
+    float fromPrediction = pair_start_residual(D, B, p, predicted);
+    if (!(fromPrediction <= fromCarry)) {  // the prediction would leave the pair further from its solution
+        atomicAdd(fellBack, 1);
+        return;
+    }
+    for (int k = 0; k < count; k++)
+        for (int w = 0; w < 4; w++) held[POINT_IMPULSE * k + w] = predicted[k][w];
+    atomicAdd(L.used, count);
 }
 
 __global__ void k_scene_from_params(const float *P, SceneDev *scene) {
@@ -1304,9 +1459,9 @@ void launch_point_targets(const BoxDev &D, const Colouring &C, const Learned &L,
 }
 
 void launch_apply_prediction(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs,
-                             hipStream_t s) {
+                             int32_t *fellBack, hipStream_t s) {
     if (touchingPairs <= 0) return;
-    k_apply_prediction<<<blocks_for(touchingPairs), THREADS, 0, s>>>(D, C, L);
+    k_apply_prediction<<<blocks_for(touchingPairs), THREADS, 0, s>>>(D, C, L, fellBack);
     check_launch("apply prediction");
 }
 
@@ -1326,8 +1481,15 @@ void launch_positions(const BoxDev &D, bool frame, hipStream_t s) {
 }
 
 template <int SWEEP, bool REVERSE>
-static void walls_then(const BoxDev &D, hipStream_t s) {
-    k_walls<SWEEP, REVERSE><<<blocks_for(D.n), THREADS, 0, s>>>(D);
+static void walls_then(const BoxDev &D, hipStream_t s, int mode, const SweepArgs &a) {
+    switch (SWEEP == WARM ? 0 : mode) {
+        case 0: k_walls<SWEEP, REVERSE, 0><<<blocks_for(D.n), THREADS, 0, s>>>(D, a); break;
+        case SWEEP_STOP: k_walls<SWEEP, REVERSE, SWEEP_STOP><<<blocks_for(D.n), THREADS, 0, s>>>(D, a); break;
+        case SWEEP_MEASURE: k_walls<SWEEP, REVERSE, SWEEP_MEASURE><<<blocks_for(D.n), THREADS, 0, s>>>(D, a); break;
+        default:
+            k_walls<SWEEP, REVERSE, SWEEP_STOP | SWEEP_MEASURE><<<blocks_for(D.n), THREADS, 0, s>>>(D, a);
+            break;
+    }
 }
 
 void launch_wall_place(const BoxDev &D, int32_t *place, hipStream_t s) {
@@ -1341,38 +1503,61 @@ void launch_wall_move(const BoxDev &D, const int32_t *from, const int32_t *to, h
     check_launch("wall move");
 }
 
-void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s) {
+void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s, int mode, const SweepArgs &a) {
     switch (sweep * 2 + (reverse ? 1 : 0)) {
-        case 0: walls_then<WARM, false>(D, s); break;
-        case 1: walls_then<WARM, true>(D, s); break;
-        case 2: walls_then<SOLVE, false>(D, s); break;
-        case 3: walls_then<SOLVE, true>(D, s); break;
-        case 4: walls_then<RELAX, false>(D, s); break;
-        default: walls_then<RELAX, true>(D, s); break;
+        case 0: walls_then<WARM, false>(D, s, 0, a); break;
+        case 1: walls_then<WARM, true>(D, s, 0, a); break;
+        case 2: walls_then<SOLVE, false>(D, s, mode, a); break;
+        case 3: walls_then<SOLVE, true>(D, s, mode, a); break;
+        case 4: walls_then<RELAX, false>(D, s, mode, a); break;
+        default: walls_then<RELAX, true>(D, s, mode, a); break;
     }
     check_launch("walls");
 }
 
+void launch_phase_measure(const BoxDev &D, int g0, int phase, hipStream_t s) {
+    if (D.scenes <= 0) return;
+    k_phase_measure<<<blocks_for(D.scenes), THREADS, 0, s>>>(D, g0, phase);
+    check_launch("phase measure");
+}
+
 void launch_scene_step(const BoxDev &D, const int32_t *sceneFirst, const int32_t *starts, int used, int substeps,
-                       int sweeps, bool symmetric, hipStream_t s) {
+                       int sweeps, bool symmetric, hipStream_t s, int mode, float tol) {
     if (D.scenes <= 0) return;
     if (used < 0 || used > MOST_ROUNDS) throw std::runtime_error("scene step: rounds out of range");
-    k_scene_step<<<D.scenes, SCENE_THREADS, 0, s>>>(D, sceneFirst, starts, used, substeps, sweeps, symmetric ? 1 : 0);
+    k_scene_step<<<D.scenes, SCENE_THREADS, 0, s>>>(D, sceneFirst, starts, used, substeps, sweeps, symmetric ? 1 : 0,
+                                                    mode, tol);
     check_launch("scene step");
 }
 
-void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int base, int pairs, hipStream_t s) {
+template <int SWEEP, bool REVERSE>
+static void round_then(const BoxDev &D, int blocks, int base, int pairs, hipStream_t s, int mode,
+                       const SweepArgs &a) {
+    switch (SWEEP == WARM ? 0 : mode) {
+        case 0: k_solve_round<SWEEP, REVERSE, 0><<<blocks, THREADS, 0, s>>>(D, base, pairs, a); break;
+        case SWEEP_STOP: k_solve_round<SWEEP, REVERSE, SWEEP_STOP><<<blocks, THREADS, 0, s>>>(D, base, pairs, a); break;
+        case SWEEP_MEASURE:
+            k_solve_round<SWEEP, REVERSE, SWEEP_MEASURE><<<blocks, THREADS, 0, s>>>(D, base, pairs, a);
+            break;
+        default:
+            k_solve_round<SWEEP, REVERSE, SWEEP_STOP | SWEEP_MEASURE><<<blocks, THREADS, 0, s>>>(D, base, pairs, a);
+            break;
+    }
+}
+
+void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int base, int pairs, hipStream_t s, int mode,
+                        const SweepArgs &a) {
     if (pairs <= 0) return;
     if (base < 0 || (int64_t)base + pairs > D.packCap)
         throw std::runtime_error("solve round: entries past the packed pairs");
     int blocks = blocks_for(2 * (int64_t)pairs);  // two lanes a pair
     switch (sweep * 2 + (reverse ? 1 : 0)) {
-        case 0: k_solve_round<WARM, false><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
-        case 1: k_solve_round<WARM, true><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
-        case 2: k_solve_round<SOLVE, false><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
-        case 3: k_solve_round<SOLVE, true><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
-        case 4: k_solve_round<RELAX, false><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
-        default: k_solve_round<RELAX, true><<<blocks, THREADS, 0, s>>>(D, base, pairs); break;
+        case 0: round_then<WARM, false>(D, blocks, base, pairs, s, 0, a); break;
+        case 1: round_then<WARM, true>(D, blocks, base, pairs, s, 0, a); break;
+        case 2: round_then<SOLVE, false>(D, blocks, base, pairs, s, mode, a); break;
+        case 3: round_then<SOLVE, true>(D, blocks, base, pairs, s, mode, a); break;
+        case 4: round_then<RELAX, false>(D, blocks, base, pairs, s, mode, a); break;
+        default: round_then<RELAX, true>(D, blocks, base, pairs, s, mode, a); break;
     }
     check_launch("solve round");
 }

@@ -295,11 +295,26 @@ __device__ float mass_along(const Moving *first, const float *rA, const Moving &
     return first ? k + first->inverseMassAlong(rA, n) : k;
 }
 
+// The residual of one point's update in a solve or relax sweep (no counterpart in the reference, whose sweeps run a
+// fixed count): the size of the whole impulse change the update made, normal `dn` and tangent `rubbed` together,
+// times the inverse mass along the normal `k`, so a velocity in m/s: how far the sweep still moved the point. A
+// point the sweep passes over has none.
+__device__ __forceinline__ float point_residual(float k, float dn, const float *rubbed) {
+    return k * sqrtf(dn * dn + dot(rubbed, rubbed));
+}
+
+// res = the larger of res and r, where NaN, once met, stays (a residual that is not a number is never small).
+__device__ __forceinline__ void keep_largest(float &res, float r) {
+    if (res == res && !(r <= res)) res = r;
+}
+
 // Boxes.contact, Boxes.java:462-542: one point, one sweep, on the point's impulse held in registers: `normal` and
 // `tangent` come in as the point holds them and leave as Boxes.contact leaves them; `wrote` says whether it writes
 // them (not in a warm sweep, nor for a point it passes over). Returns whether the point holds an impulse afterwards.
+// With `res`, the point's residual (point_residual) is kept in it when larger; nothing of the contact depends on it.
 __device__ bool contact_held(Sweep sweep, float gap, const float *n, Moving *first, const float *rA, Moving &second,
-                             const float *rB, float &normalHeld, float *tangentHeld, bool &wrote, const Soft &soft) {
+                             const float *rB, float &normalHeld, float *tangentHeld, bool &wrote, const Soft &soft,
+                             float *res = nullptr) {
     wrote = false;
     float normal = normalHeld;
     float tangent[3] = {tangentHeld[0], tangentHeld[1], tangentHeld[2]};
@@ -351,6 +366,7 @@ __device__ bool contact_held(Sweep sweep, float gap, const float *n, Moving *fir
     float rubbed[3];
     for (int a = 0; a < 3; a++) rubbed[a] = whole[a] - tangent[a];
     apply_pair(first, rA, second, rB, rubbed);
+    if (res) keep_largest(*res, point_residual(k, total - normal, rubbed));
     normalHeld = total;
     for (int a = 0; a < 3; a++) tangentHeld[a] = whole[a];
     wrote = true;
@@ -360,12 +376,13 @@ __device__ bool contact_held(Sweep sweep, float gap, const float *n, Moving *fir
 // Boxes.contact on the point's words in memory. The tangent impulse's three words are `stride` apart (1 for a pair's
 // point; the body count for a wall's, whose impulses are laid out word by word over the bodies).
 __device__ bool contact(Sweep sweep, float gap, const float *n, Moving *first, const float *rA, Moving &second, const float *rB,
-             float *normals, size_t normalAt, float *tangents, size_t tangentAt, size_t stride, const Soft &soft) {
+             float *normals, size_t normalAt, float *tangents, size_t tangentAt, size_t stride, const Soft &soft,
+             float *res = nullptr) {
     float normal = normals[normalAt];
     float tangent[3] = {tangents[tangentAt], tangents[tangentAt + stride], tangents[tangentAt + 2 * stride]};
     float was[4] = {normal, tangent[0], tangent[1], tangent[2]};
     bool wrote;
-    bool holds = contact_held(sweep, gap, n, first, rA, second, rB, normal, tangent, wrote, soft);
+    bool holds = contact_held(sweep, gap, n, first, rA, second, rB, normal, tangent, wrote, soft, res);
     if (wrote) {  // a word written with the bits it holds is left as it is
         if (bits(normal) != bits(was[0])) normals[normalAt] = normal;
         for (int a = 0; a < 3; a++)

@@ -37,6 +37,10 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n);
 void box_step(BoxWorld *w, vr_step_report *rep);
 // BoxStepper.step up to the rounds' write: the collide list and the colouring, no substep (vr_box_collide_list).
 void box_collide_list(BoxWorld *w, vr_step_report *rep);
+// Sweeps stopped on a residual (vr_box_tolerance), and what the last step's phases did.
+void box_tolerance(BoxWorld *w, float tol, bool measure);
+void box_solve_read(BoxWorld *w, vr_box_solve *out);
+int32_t box_solve_curve(BoxWorld *w, float *out, int32_t n);
 void box_wall_impulses_written(BoxWorld *w);  // after a write of lambda or rub through the ABI
 void box_shapes_written(BoxWorld *w);         // after a write of shape: the wall impulses follow their bodies
 void box_pair_buffer_written(BoxWorld *w, const std::string &name);  // after a write of a manifold or impulse buffer

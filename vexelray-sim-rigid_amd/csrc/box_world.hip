@@ -24,6 +24,9 @@
 
 namespace vrbox {
 
+static_assert(VR_SOLVE_SUBSTEPS == SOLVE_MOST_SUBSTEPS && VR_SOLVE_SWEEPS == SOLVE_MOST_SWEEPS,
+              "the ABI's solve report and the device's phase words agree");
+
 namespace {
 
 #define HIP_OK(expr)                                                                                       \
@@ -77,6 +80,13 @@ struct BoxWorld {
     int colourSteps = 1;        // steps of the colouring the last step needed
     int roundPairs[MOST_ROUNDS] = {};  // pairs in each round this step
     int roundBase[MOST_ROUNDS] = {};   // its first entry in the colouring's order (the packed entries)
+    // The sweeps stopped on a residual (vr_box_tolerance): off (every sweep runs) while tol is below zero.
+    float tol = -1.f;
+    bool measure = false;           // keep each phase's sweeps and residuals (vr_box_solve_read)
+    int sweepNo = 0;                // sweeps numbered over the world's life (SweepCtl)
+    std::vector<int> predicted;     // substeps x 2: sweeps each phase needed last step, launched before a look
+    vr_box_solve lastSolve{};       // what the last step's phases did
+    int32_t *fellBack = nullptr;    // device: pairs whose prediction raised their start residual, left to the carry
 
     template <class T>
     T *alloc(int64_t count) {
@@ -164,6 +174,14 @@ struct BoxWorld {
         D.rub = alloc<float>((int64_t)RUB_WORDS * n);
         D.roll = alloc<float>((int64_t)RUB_WORDS * n);
         D.shown = alloc<float>((int64_t)SHOWN_STRIDE * n);
+        D.sw.sceneOpen = alloc<int32_t>((int64_t)scenes.size());
+        D.sw.lastOpen = alloc<int32_t>(1);
+        D.sw.sceneLast = alloc<unsigned long long>((int64_t)scenes.size());
+        D.sw.phaseSweeps = alloc<int32_t>(2 * SOLVE_MOST_SUBSTEPS);
+        D.sw.phaseResidual = alloc<uint32_t>(2 * SOLVE_MOST_SUBSTEPS);
+        D.sw.curve = alloc<uint32_t>(2 * SOLVE_MOST_SUBSTEPS * SOLVE_MOST_SWEEPS);
+        fellBack = alloc<int32_t>(1);
+        predicted.assign((size_t)d.substeps * 2, d.sweeps);
         D.cornerHeld = alloc<uint64_t>(n);
         D.frame = alloc<float4>((int64_t)FRAME_VECTORS * n);
         D.broad = alloc<float4>(2 * (int64_t)n);
@@ -186,6 +204,7 @@ struct BoxWorld {
             reg("order", D.order, n, VR_I32);
             reg("starts", D.starts, (int64_t)D.cells + 1, VR_I32);
             int64_t slots = D.slots;
+            pack_make();
             D.pairs = alloc<int32_t>(2 * slots);
             D.collideList = alloc<int32_t>(slots);
             D.pairsWas = alloc<int32_t>(2 * slots);
@@ -264,6 +283,7 @@ struct BoxWorld {
 
     void clear() {  // PassRunner.clear(), then BoxStep.constants(): qw = 1 (BoxStep.java:103-105)
         tuneStep = 0;  // a new start: the step mode is timed again
+        sweep_control_clear();
         for (auto &kv : named) HIP_OK(hipMemsetAsync(kv.second.p, 0, (size_t)kv.second.count * 4, stream));
         if (grid) {
             size_t slots = D.slots;
@@ -281,6 +301,78 @@ struct BoxWorld {
         vr::launch_fill(D.q[3], 1.0f, n, stream);
         place_walls(false);  // every shape 0 (a box), every impulse 0
         HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    // No scene open, no sweep counted, every phase predicted at the full count.
+    void sweep_control_clear() {
+        HIP_OK(hipMemsetAsync(D.sw.sceneOpen, 0, sizeof(int32_t) * scenes.size(), stream));
+        HIP_OK(hipMemsetAsync(D.sw.lastOpen, 0, sizeof(int32_t), stream));
+        HIP_OK(hipMemsetAsync(D.sw.sceneLast, 0, sizeof(unsigned long long) * scenes.size(), stream));
+        HIP_OK(hipMemsetAsync(fellBack, 0, sizeof(int32_t), stream));
+        sweepNo = 0;
+        std::fill(predicted.begin(), predicted.end(), d.sweeps);
+    }
+
+    // vr_box_tolerance: the sweeps of each phase stop once a sweep's residuals are all at most `t` (below 0: the
+    // reference's fixed count); with `m`, each step's phases are measured.
+    void tolerance(float t, bool m) {
+        if (t != t) throw std::invalid_argument("a tolerance is a number (below zero for none)");
+        tol = t;
+        measure = m;
+        HIP_OK(hipStreamSynchronize(stream));
+        std::fill(predicted.begin(), predicted.end(), d.sweeps);
+    }
+
+    int sweep_mode() const { return (tol >= 0 ? SWEEP_STOP : 0) | (measure ? SWEEP_MEASURE : 0); }
+
+    // A measured step's phase words start from zero.
+    void measure_clear() {
+        vr::launch_fill_i32(D.sw.phaseSweeps, 0, 2 * SOLVE_MOST_SUBSTEPS, stream);
+        vr::launch_fill_i32(reinterpret_cast<int32_t *>(D.sw.phaseResidual), 0, 2 * SOLVE_MOST_SUBSTEPS, stream);
+        vr::launch_fill_i32(reinterpret_cast<int32_t *>(D.sw.curve), 0, 2 * SOLVE_MOST_SUBSTEPS * SOLVE_MOST_SWEEPS,
+                            stream);
+    }
+
+    // lastSolve from the step just run (the stream is idle).
+    void solve_report() {
+        vr_box_solve &r = lastSolve;
+        r.tolerance = tol;
+        r.measured = measure ? 1 : 0;
+        r.substeps = d.substeps;
+        r.sweeps = d.sweeps;
+        r.scenes = (int32_t)scenes.size();
+        r.scene_step = sceneStep ? 1 : 0;
+        for (int k = 0; k < 2 * VR_SOLVE_SUBSTEPS; k++) {
+            r.scene_sweeps[k / VR_SOLVE_SUBSTEPS][k % VR_SOLVE_SUBSTEPS] = 0.f;
+            r.residual[k / VR_SOLVE_SUBSTEPS][k % VR_SOLVE_SUBSTEPS] = 0.f;
+        }
+        if (!measure) return;
+        std::vector<int32_t> sweeps(2 * SOLVE_MOST_SUBSTEPS);
+        std::vector<uint32_t> res(2 * SOLVE_MOST_SUBSTEPS);
+        HIP_OK(hipMemcpyAsync(sweeps.data(), D.sw.phaseSweeps, sizeof(int32_t) * sweeps.size(), hipMemcpyDeviceToHost,
+                              stream));
+        HIP_OK(hipMemcpyAsync(res.data(), D.sw.phaseResidual, sizeof(uint32_t) * res.size(), hipMemcpyDeviceToHost,
+                              stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for (int sub = 0; sub < std::min(d.substeps, VR_SOLVE_SUBSTEPS); sub++)
+            for (int k = 0; k < 2; k++) {
+                r.scene_sweeps[k][sub] = (float)sweeps[2 * sub + k] / (float)std::max<size_t>(scenes.size(), 1);
+                uint32_t bitsOf = res[2 * sub + k];
+                float v;
+                std::memcpy(&v, &bitsOf, sizeof v);
+                r.residual[k][sub] = v;
+            }
+    }
+
+    // The last measured step's largest residual of each sweep, [substep][solve, relax][sweep], into out.
+    int32_t solve_curve(float *out, int32_t n) {
+        int32_t words = 2 * SOLVE_MOST_SUBSTEPS * SOLVE_MOST_SWEEPS;
+        std::vector<uint32_t> c((size_t)words);
+        HIP_OK(hipMemcpyAsync(c.data(), D.sw.curve, sizeof(uint32_t) * words, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        int32_t take = std::min(n, words);
+        std::memcpy(out, c.data(), sizeof(float) * (size_t)take);
+        return take;
     }
 
     // The wall sweep's places for the shapes as they are; with `move`, the wall impulse words follow their bodies.
@@ -331,42 +423,42 @@ struct BoxWorld {
         HIP_OK(hipStreamSynchronize(stream));
     }
 
-    // Room for this step's points (at most 4 a touching pair), grown as needed.
-    // Packed fields for `touching` pairs, grown by half again when short (the stream is idle: the colouring has just
-    // read its counts back).
-    void pack_room(int touching) {
-        if (touching <= D.packCap) return;
-        int64_t cap = std::max<int64_t>((int64_t)touching + touching / 2, 1024);
-        HIP_OK(hipStreamSynchronize(stream));
-        for (void *p : {(void *)D.packPair, (void *)D.packCount, (void *)D.packBody, (void *)D.packNormal,
-                        (void *)D.packAnchor, (void *)D.packImpulse})
-            if (p) HIP_OK(hipFree(p));
-        HIP_OK(hipMalloc((void **)&D.packPair, sizeof(int32_t) * cap));
-        HIP_OK(hipMalloc((void **)&D.packCount, sizeof(int32_t) * cap));
-        HIP_OK(hipMalloc((void **)&D.packBody, sizeof(int32_t) * 2 * cap));
-        HIP_OK(hipMalloc((void **)&D.packNormal, sizeof(float) * 3 * cap));
-        HIP_OK(hipMalloc((void **)&D.packAnchor, sizeof(float) * 2 * MOST_POINTS * 3 * cap));
-        HIP_OK(hipMalloc((void **)&D.packImpulse, sizeof(float) * MOST_POINTS * POINT_IMPULSE * cap));
+    // The packed fields, made once at build for the most touching pairs there can be (one a slot), so no step
+    // allocates.
+    void pack_make() {
+        int64_t cap = std::max<int64_t>(D.slots, 1);
+        D.packPair = alloc<int32_t>(cap);
+        D.packScene = alloc<int32_t>(cap);
+        D.packCount = alloc<int32_t>(cap);
+        D.packBody = alloc<int32_t>(2 * cap);
+        D.packNormal = alloc<float>(3 * cap);
+        D.packAnchor = alloc<float>(2 * MOST_POINTS * 3 * cap);
+        D.packImpulse = alloc<float>(MOST_POINTS * POINT_IMPULSE * cap);
         D.packCap = cap;
     }
 
-    void points_room(int64_t points) {
-        if (points <= pointCap) return;
-        int64_t cap = std::max<int64_t>(points + points / 4, 1024);
-        for (void *p : {(void *)L.pointBase, (void *)L.x, (void *)L.y, (void *)L.t})
-            if (p) HIP_OK(hipFree(p));
-        HIP_OK(hipMalloc((void **)&L.pointBase, sizeof(int32_t) * ((size_t)D.slots + 1)));
-        HIP_OK(hipMalloc((void **)&L.x, sizeof(float) * cap * LEARN_FEATURES));
-        HIP_OK(hipMalloc((void **)&L.y, sizeof(float) * cap * LEARN_OUTPUTS));
-        HIP_OK(hipMalloc((void **)&L.t, sizeof(float) * cap * LEARN_OUTPUTS));
-        if (!L.used) {
-            HIP_OK(hipMalloc((void **)&L.used, sizeof(int32_t)));
-            HIP_OK(hipMemset(L.used, 0, sizeof(int32_t)));
-        }
+    void pack_room(int touching) {
+        if (touching > D.packCap)
+            throw std::runtime_error("more touching pairs (" + std::to_string(touching) + ") than slots");
+    }
+
+    // The learned start's per-point data, made when a network is set or a log started (never inside a step), for
+    // the most points there can be: four a slot.
+    void points_make() {
+        if (pointCap > 0 || !grid) return;
+        int64_t cap = std::max<int64_t>((int64_t)MOST_POINTS * D.slots, 1);
+        L.pointBase = alloc<int32_t>((int64_t)D.slots + 1);
+        L.x = alloc<float>(cap * LEARN_FEATURES);
+        L.y = alloc<float>(cap * LEARN_OUTPUTS);
+        L.t = alloc<float>(cap * LEARN_OUTPUTS);
+        L.used = alloc<int32_t>(1);
+        HIP_OK(hipMemsetAsync(L.used, 0, sizeof(int32_t), stream));
+        HIP_OK(hipStreamSynchronize(stream));
         pointCap = cap;
     }
 
     void log(bool on, int64_t capacity) {
+        if (on) points_make();
         logging = on;
         logCount = 0;
         if (on && capacity > logCap) {
@@ -448,13 +540,52 @@ struct BoxWorld {
     }
 
     // BoxStep.sweep, BoxStep.java:217-231.
-    void sweep(int kind, bool reverse, int rounds) {
-        if (reverse) launch_walls(D, kind, reverse, stream);
+    void sweep(int kind, bool reverse, int rounds, int mode = 0, const SweepArgs &a = SweepArgs{0, 1, -1.f, -1}) {
+        if (reverse) launch_walls(D, kind, reverse, stream, mode, a);
         for (int order = 0; order < rounds; order++) {
             int r = reverse ? rounds - 1 - order : order;
-            launch_solve_round(D, kind, reverse, roundBase[r], roundPairs[r], stream);
+            launch_solve_round(D, kind, reverse, roundBase[r], roundPairs[r], stream, mode, a);
         }
-        if (!reverse) launch_walls(D, kind, reverse, stream);
+        if (!reverse) launch_walls(D, kind, reverse, stream, mode, a);
+    }
+
+    // The solve (or relax) sweeps of substep `sub` (BoxStep.java:204-210): the configured count, or, stopped on the
+    // residual, until every scene has stopped (each scene's own sweeps decided on the device, SweepCtl). The host
+    // launches what the same phase needed last step, then looks at the device's last open sweep and launches two at
+    // a time until none is open; a scene that has stopped passes over any sweep launched after it. Returns the
+    // sweeps launched.
+    int phase(int kind, int sub, int rounds) {
+        int mode = sweep_mode();
+        bool relax = kind == RELAX;
+        int at = 2 * sub + (relax ? 1 : 0);
+        bool kept = (mode & SWEEP_MEASURE) && sub < SOLVE_MOST_SUBSTEPS;
+        int runMode = kept ? mode : (mode & SWEEP_STOP);
+        int g0 = sweepNo + 1;
+        auto one = [&](int k) {
+            SweepArgs a{++sweepNo, k == 0 ? 1 : 0, tol,
+                        kept && k < SOLVE_MOST_SWEEPS ? at * SOLVE_MOST_SWEEPS + k : -1};
+            sweep(kind, d.symmetric && k % 2 == 1, rounds, runMode, a);
+        };
+        int launched = 0;
+        if (!(mode & SWEEP_STOP)) {
+            for (; launched < d.sweeps; launched++) one(launched);
+        } else {
+            int want = std::min(std::max(predicted[at], 1), d.sweeps);
+            int needed = d.sweeps;
+            while (true) {
+                for (; launched < want; launched++) one(launched);
+                int open = read_word(D.sw.lastOpen);
+                if (open < sweepNo) {  // no scene was open after the last sweep launched
+                    needed = open < g0 ? 1 : std::min(open - g0 + 2, d.sweeps);
+                    break;
+                }
+                if (launched >= d.sweeps) break;
+                want = std::min(d.sweeps, launched + 2);
+            }
+            predicted[at] = needed;
+        }
+        if (kept) launch_phase_measure(D, g0, at, stream);
+        return launched;
     }
 
     // The counts BoxStepper.step reports for the step's collide (BoxStepper.java:40, 62-90).
@@ -483,11 +614,10 @@ struct BoxWorld {
             // BoxStepper's colouring (first fit in slot order), on the device; the host reads back the counts.
             colour(&p.used, &p.touching, &p.points, &p.unplaced, &p.unlisted);
             if ((net || logging) && p.touching > 0) {
-                points_room(p.points);
                 launch_point_features(D, C, L, p.touching, stream);
                 if (net) {
                     net->forward(L.x, p.points, L.y, stream);
-                    launch_apply_prediction(D, C, L, p.touching, stream);
+                    launch_apply_prediction(D, C, L, p.touching, fellBack, stream);
                 }
             }
         }
@@ -525,8 +655,12 @@ struct BoxWorld {
         HIP_OK(hipEventRecord(t0, stream));
         Prepared p = prepare();
         int used = p.used, touching = p.touching, points = p.points;
-        if (sceneStep) {
-            launch_scene_step(D, sceneFirst, C.starts, used, d.substeps, d.sweeps, d.symmetric != 0, stream);
+        if (sweep_mode() & SWEEP_MEASURE) measure_clear();
+        std::fill(lastSolve.launched[0], lastSolve.launched[0] + 2 * VR_SOLVE_SUBSTEPS, 0);
+        bool logged = logging && grid && touching > 0;
+        if (sceneStep && !logged) {  // a logged step takes its targets between phases: pass by pass
+            launch_scene_step(D, sceneFirst, C.starts, used, d.substeps, d.sweeps, d.symmetric != 0, stream,
+                              sweep_mode(), tol);
         } else {
             if (used > 0) {  // the sweeps' rounds read and write the touching pairs packed in round order
                 pack_room(touching);
@@ -537,16 +671,25 @@ struct BoxWorld {
                 launch_velocities(D, sub > 0, stream);
                 if (sub == 0) launch_frame_in(D, true, stream);
                 sweep(WARM, false, used);
-                for (int k = 0; k < d.sweeps; k++) sweep(SOLVE, d.symmetric && k % 2 == 1, used);
+                int solved = phase(SOLVE, sub, used);
                 launch_motion_out(D, stream);
+                if (sub == 0 && logged) {
+                    // The targets: what each point holds once the first substep's solve sweeps are done, the
+                    // impulse a start would best begin that phase from.
+                    if (used > 0) launch_unpack(D, touching, stream);
+                    launch_point_targets(D, C, L, touching, stream);
+                }
                 launch_positions(D, true, stream);
-                for (int k = 0; k < d.sweeps; k++) sweep(RELAX, d.symmetric && k % 2 == 1, used);
+                int relaxed = phase(RELAX, sub, used);
                 launch_motion_out(D, stream);
+                if (sub < VR_SOLVE_SUBSTEPS) {
+                    lastSolve.launched[0][sub] = solved;
+                    lastSolve.launched[1][sub] = relaxed;
+                }
             }
             if (used > 0) launch_unpack(D, touching, stream);
         }
-        if (logging && grid && touching > 0) {
-            launch_point_targets(D, C, L, touching, stream);  // what the next step's carry would start from
+        if (logged) {
             int64_t take = std::min<int64_t>(points, logCap - logCount);
             if (take > 0) {
                 HIP_OK(hipMemcpyAsync(logX + logCount * LEARN_FEATURES, L.x, (size_t)take * LEARN_FEATURES * 4,
@@ -567,12 +710,17 @@ struct BoxWorld {
             tuneStep++;
         }
         rep->reserved = 0;
+        lastSolve.fell_back = 0;
         if (net && grid && touching > 0) {
             HIP_OK(hipMemcpyAsync(pinned + 48, L.used, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipMemcpyAsync(pinned + 49, fellBack, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
             HIP_OK(hipMemsetAsync(L.used, 0, sizeof(int32_t), stream));
+            HIP_OK(hipMemsetAsync(fellBack, 0, sizeof(int32_t), stream));
             HIP_OK(hipStreamSynchronize(stream));
             rep->reserved = pinned[48];
+            lastSolve.fell_back = pinned[49];
         }
+        solve_report();
         report_counts(p, rep);
         rep->submissions = grid ? 2 : 1;  // BoxStepper.step: the collide list read back, then the step's list
         rep->gpu_ns = (int64_t)(ms * 1e6);
@@ -603,9 +751,7 @@ struct BoxWorld {
         if (net) net->holders--;  // the network may be destroyed once no world starts from it
         if (stream) (void)hipStreamSynchronize(stream);
         if (pinned) (void)hipHostFree(pinned);
-        for (void *p : {(void *)L.pointBase, (void *)L.x, (void *)L.y, (void *)L.t, (void *)L.used, (void *)logX,
-                        (void *)logT, (void *)D.packPair, (void *)D.packCount, (void *)D.packBody,
-                        (void *)D.packNormal, (void *)D.packAnchor, (void *)D.packImpulse})
+        for (void *p : {(void *)logX, (void *)logT})
             if (p) (void)hipFree(p);
         for (void *p : allocs) (void)hipFree(p);
         if (t0) (void)hipEventDestroy(t0);
@@ -671,6 +817,9 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n) {
 
 void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
 void box_collide_list(BoxWorld *w, vr_step_report *rep) { w->collide_list(rep); }
+void box_tolerance(BoxWorld *w, float tol, bool measure) { w->tolerance(tol, measure); }
+void box_solve_read(BoxWorld *w, vr_box_solve *out) { *out = w->lastSolve; }
+int32_t box_solve_curve(BoxWorld *w, float *out, int32_t n) { return w->solve_curve(out, n); }
 void box_wall_impulses_written(BoxWorld *w) { w->wall_impulses_written(); }
 void box_shapes_written(BoxWorld *w) {
     w->place_walls(true);
@@ -687,6 +836,7 @@ void box_diagnostics_into(BoxWorld *w, double *partials, double *devOut, double 
 int box_bodies(BoxWorld *w) { return w->n; }
 int64_t box_samples(BoxWorld *w, float *x, float *t, int64_t most) { return w->samples(x, t, most); }
 void box_learned(BoxWorld *w, vrmlp::Net *net) {
+    if (net) w->points_make();
     if (net) net->holders++;
     if (w->net) w->net->holders--;
     w->net = net;

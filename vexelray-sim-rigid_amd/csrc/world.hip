@@ -881,6 +881,30 @@ int vr_box_collide_list(vr_world *w, vr_step_report *report) {
     });
 }
 
+int vr_box_tolerance(vr_world *w, float tolerance, int32_t measure) {
+    if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    if (!w->box) return fail(VR_ERR_STATE, "not a box world");
+    return guard([&] {
+        try {
+            vrbox::box_tolerance(w->box, tolerance, measure != 0);
+        } catch (const std::invalid_argument &e) {
+            throw ArgError(e.what());
+        }
+    });
+}
+
+int vr_box_solve_read(vr_world *w, vr_box_solve *out) {
+    if (!w || !out) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (!w->box) return fail(VR_ERR_STATE, "not a box world");
+    return guard([&] { vrbox::box_solve_read(w->box, out); });
+}
+
+int vr_box_solve_curve(vr_world *w, float *out, int32_t n, int32_t *count) {
+    if (!w || !count || n < 0 || (n > 0 && !out)) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (!w->box) return fail(VR_ERR_STATE, "not a box world");
+    return guard([&] { *count = vrbox::box_solve_curve(w->box, out, n); });
+}
+
 int vr_box_log(vr_world *w, int32_t on, int64_t capacity) {
     if (!w || (on && capacity < 1)) return fail(VR_ERR_ARGUMENT, "a log needs room for a sample");
     if (!w->box) return fail(VR_ERR_STATE, "not a box world");
