@@ -135,6 +135,16 @@ typedef struct vr_profile {
     int64_t listed_sum;
     int64_t steps;
     int64_t enqueue_ns;
+    /*
+     * Box worlds: events bracket the colour rounds (k_solve_round, back to back) and the walls sweep (k_walls) of
+     * every solve and relax sweep. round_ns / round_launches are the rounds', walls_ns / walls_launches the walls';
+     * passes counts those sweeps; listed_sum and points_sum add up the touching pairs and their manifold points the
+     * sweeps took (P and P k-bar of the algorithmic bytes), bodies_sum the bodies.
+     */
+    int64_t walls_ns;
+    int64_t walls_launches;
+    int64_t points_sum;
+    int64_t bodies_sum;
 } vr_profile;
 
 int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out);

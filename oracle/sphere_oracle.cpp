@@ -29,6 +29,9 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "../include/vexelray_rigid.h"
 #include "box_oracle.h"
@@ -1365,9 +1368,22 @@ int vro_world_step(vro_world *p, int32_t wait, vr_step_report *rep) {
             vr_step_report local{};
             vr_step_report *r = rep ? rep : &local;
             std::memset(r, 0, sizeof(*r));
-            for (auto *b : p->batch) {
-                vr_step_report one{};
-                vro::box_step(b, &one);
+            // The scenes share nothing, so they step on as many threads as OpenMP gives (vro_threads); each scene's
+            // result is the same whichever thread takes it.
+            std::vector<vr_step_report> ones(p->batch.size());
+            std::vector<std::string> errors(p->batch.size());
+#pragma omp parallel for schedule(dynamic, 1)
+            for (long k = 0; k < (long)p->batch.size(); k++) {
+                try {
+                    std::memset(&ones[k], 0, sizeof(ones[k]));
+                    vro::box_step(p->batch[k], &ones[k]);
+                } catch (const std::exception &e) {
+                    errors[k] = e.what();
+                }
+            }
+            for (const std::string &e : errors)
+                if (!e.empty()) throw std::runtime_error(e);
+            for (const vr_step_report &one : ones) {
                 r->touching += one.touching;
                 r->points += one.points;
                 r->unplaced += one.unplaced;
@@ -1619,5 +1635,16 @@ int vro_box_learned(vro_world *, struct vro_mlp *) { return fail(VR_ERR_STATE, "
 
 // Opening batch sizes the stepper holds, for the tests of its adaptation (SphereStepper.opening, :115-117).
 int vro_world_opening(vro_world *p, int32_t iteration) { return p->w.opening.at(iteration); }
+
+// Threads a batch's scenes step on (0: OpenMP's own choice); the oracle's CPU baseline runs with 1 and with all.
+int vro_threads(int32_t n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 }  // extern "C"

@@ -63,7 +63,9 @@ class SolveReport(ctypes.Structure):
 
 class Profile(ctypes.Structure):
     _fields_ = [("round_ns", ctypes.c_int64), ("round_launches", ctypes.c_int64), ("passes", ctypes.c_int64),
-                ("listed_sum", ctypes.c_int64), ("steps", ctypes.c_int64), ("enqueue_ns", ctypes.c_int64)]
+                ("listed_sum", ctypes.c_int64), ("steps", ctypes.c_int64), ("enqueue_ns", ctypes.c_int64),
+                ("walls_ns", ctypes.c_int64), ("walls_launches", ctypes.c_int64), ("points_sum", ctypes.c_int64),
+                ("bodies_sum", ctypes.c_int64)]
 
 
 class Abi:

@@ -150,6 +150,8 @@ void launch_scene_from_params(const BoxDev &D, SceneDev *scene, hipStream_t s);
 void launch_positions(const BoxDev &D, bool frame, hipStream_t s);
 void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s, int mode = 0,
                   const SweepArgs &a = SweepArgs{0, 1, -1.f, -1});
+// lastOpen = g if any scene is open in sweep g (what the host then reads).
+void launch_any_open(const BoxDev &D, int g, hipStream_t s);
 // At the end of a measured phase that began with sweep g0: each scene's sweeps and last residual added into `phase`.
 void launch_phase_measure(const BoxDev &D, int g0, int phase, hipStream_t s);
 // The wall sweep's places for the bodies' shapes into `place` (n words), as the sweep takes the bodies.

@@ -324,36 +324,60 @@ __device__ __forceinline__ bool sweeping(const BoxDev &D, const SweepArgs &a, in
     return a.first || D.sw.sceneOpen[scene] >= a.g - 1;
 }
 
-// What a lane of a stopped or measured sweep leaves of its residual `res` for scene `scene`: past the tolerance,
-// the scene (and the world) open in this sweep; measured, the scene's last sweep and residual, and the sweep's largest.
-// Residuals are at least zero (or NaN), so their bits order as they do.
+// One scene's largest residual in a sweep, recorded: past the tolerance, the scene is open in this sweep; measured,
+// the scene's last sweep and residual, and the sweep's largest. Residuals are at least zero (or NaN), so their bits
+// order as they do, NaN above every number.
 template <int MODE>
-__device__ __forceinline__ void record(const BoxDev &D, const SweepArgs &a, int scene, float res) {
-    if ((MODE & SWEEP_STOP) && !(res <= a.tol)) {
-        D.sw.sceneOpen[scene] = a.g;
-        *D.sw.lastOpen = a.g;
-    }
+__device__ __forceinline__ void record_one(const BoxDev &D, const SweepArgs &a, int scene, unsigned largest) {
+    if ((MODE & SWEEP_STOP) && !(__uint_as_float(largest) <= a.tol)) D.sw.sceneOpen[scene] = a.g;
     if (MODE & SWEEP_MEASURE) {
-        unsigned long long key = ((unsigned long long)(unsigned)a.g << 32) | (unsigned long long)__float_as_uint(res);
+        unsigned long long key = ((unsigned long long)(unsigned)a.g << 32) | (unsigned long long)largest;
         atomicMax(D.sw.sceneLast + scene, key);
-        if (a.curveAt >= 0) atomicMax(D.sw.curve + a.curveAt, __float_as_uint(res));
+        if (a.curveAt >= 0) atomicMax(D.sw.curve + a.curveAt, largest);
     }
+}
+
+// What a wave of a stopped or measured sweep leaves of its lanes' residuals, every lane of the wave calling it: a
+// wave whose live lanes all hold one scene (the usual case: bodies and rounds run in scene order) records the wave's
+// largest once, through one lane; a wave across scenes records lane by lane. So a sweep stores a word a wave, not a
+// lane, and no word is stored to by every lane of the launch.
+template <int MODE>
+__device__ __forceinline__ void record(const BoxDev &D, const SweepArgs &a, bool live, int scene, float res) {
+    unsigned long long lives = __ballot(live);
+    if (lives == 0ull) return;
+    int lane = threadIdx.x & 63;
+    int leader = __ffsll((long long)lives) - 1;
+    int first = __shfl(scene, leader);
+    unsigned mine = live ? __float_as_uint(res) : 0u;
+    if (__ballot(live && scene != first) == 0ull) {
+        unsigned largest = mine;
+        for (int step = 32; step > 0; step >>= 1) largest = max(largest, (unsigned)__shfl_xor((int)largest, step));
+        if (lane == leader) record_one<MODE>(D, a, first, largest);
+    } else if (live) {
+        record_one<MODE>(D, a, scene, mine);
+    }
+}
+
+// Sets lastOpen to g if any scene is open in sweep g: what the host reads to see whether a phase is done.
+__global__ void k_any_open(BoxDev D, int g) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    bool open = j < D.scenes && D.sw.sceneOpen[j] == g;
+    if (__ballot(open) != 0ull && (threadIdx.x & 63) == 0) *D.sw.lastOpen = g;
 }
 
 template <int SWEEP, bool REVERSE, int MODE>
 __global__ void __launch_bounds__(THREADS) k_walls(BoxDev D, SweepArgs a) {
     int s = shape_first(D);
-    if (s >= D.n) return;
     int place = blockIdx.x * THREADS + threadIdx.x;
     if (MODE == 0) {
-        walls_of<SWEEP, REVERSE, true>(D, s, place);
+        if (s < D.n) walls_of<SWEEP, REVERSE, true>(D, s, place);
         return;
     }
-    int scene = D.scene[s];
-    if ((MODE & SWEEP_STOP) && !sweeping(D, a, scene)) return;
+    int scene = s < D.n ? D.scene[s] : -1;
+    bool live = s < D.n && (!(MODE & SWEEP_STOP) || sweeping(D, a, scene));
     float res = 0.f;
-    walls_of<SWEEP, REVERSE, true>(D, s, place, &res);
-    record<MODE>(D, a, scene, res);
+    if (live) walls_of<SWEEP, REVERSE, true>(D, s, place, &res);
+    record<MODE>(D, a, live, scene, res);
 }
 
 __global__ void __launch_bounds__(THREADS) k_wall_place(BoxDev D, int32_t *place) {
@@ -730,17 +754,16 @@ template <int SWEEP, bool REVERSE, int MODE>
 __global__ void __launch_bounds__(THREADS) k_solve_round(BoxDev D, int base, int pairs, SweepArgs a) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int k = t >> 1;
-    if (k >= pairs) return;
     bool second = (t & 1) != 0;
     if (MODE == 0) {
-        solve_pair_side<SWEEP, REVERSE>(D, base + k, second);
+        if (k < pairs) solve_pair_side<SWEEP, REVERSE>(D, base + k, second);
         return;
     }
-    int scene = D.packScene[base + k];
-    if ((MODE & SWEEP_STOP) && !sweeping(D, a, scene)) return;
+    int scene = k < pairs ? D.packScene[base + k] : -1;
+    bool live = k < pairs && (!(MODE & SWEEP_STOP) || sweeping(D, a, scene));
     float res = 0.f;
-    solve_pair_side<SWEEP, REVERSE>(D, base + k, second, &res);
-    if (!second) record<MODE>(D, a, scene, res);
+    if (live) solve_pair_side<SWEEP, REVERSE>(D, base + k, second, &res);
+    record<MODE>(D, a, live && !second, scene, res);
 }
 
 // Body s's frame record from the state, and with `motion` its motion record.
@@ -1513,6 +1536,12 @@ void launch_walls(const BoxDev &D, int sweep, bool reverse, hipStream_t s, int m
         default: walls_then<RELAX, true>(D, s, mode, a); break;
     }
     check_launch("walls");
+}
+
+void launch_any_open(const BoxDev &D, int g, hipStream_t s) {
+    if (D.scenes <= 0) return;
+    k_any_open<<<blocks_for(D.scenes), THREADS, 0, s>>>(D, g);
+    check_launch("any open");
 }
 
 void launch_phase_measure(const BoxDev &D, int g0, int phase, hipStream_t s) {

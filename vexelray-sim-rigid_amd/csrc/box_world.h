@@ -39,6 +39,9 @@ void box_step(BoxWorld *w, vr_step_report *rep);
 void box_collide_list(BoxWorld *w, vr_step_report *rep);
 // Sweeps stopped on a residual (vr_box_tolerance), and what the last step's phases did.
 void box_tolerance(BoxWorld *w, float tol, bool measure);
+// Sweep timing (vr_world_profile on a box world): on zeroes the counters.
+void box_profile(BoxWorld *w, bool on);
+void box_profile_read(BoxWorld *w, vr_profile *out);
 void box_solve_read(BoxWorld *w, vr_box_solve *out);
 int32_t box_solve_curve(BoxWorld *w, float *out, int32_t n);
 void box_wall_impulses_written(BoxWorld *w);  // after a write of lambda or rub through the ABI

@@ -86,6 +86,13 @@ struct BoxWorld {
     int sweepNo = 0;                // sweeps numbered over the world's life (SweepCtl)
     std::vector<int> predicted;     // substeps x 2: sweeps each phase needed last step, launched before a look
     vr_box_solve lastSolve{};       // what the last step's phases did
+    // Measurement (vr_world_profile): three events a solve or relax sweep, around its rounds and its walls.
+    bool profiling = false;
+    vr_profile prof{};
+    std::vector<hipEvent_t> sweepEvents;
+    std::vector<char> sweepReversed;
+    int sweepsTimed = 0;
+    int roundsThisStep = 0;
     int32_t *fellBack = nullptr;    // device: pairs whose prediction raised their start residual, left to the carry
 
     template <class T>
@@ -255,8 +262,8 @@ struct BoxWorld {
             C.sort.keysB = alloc<int32_t>(slots);
             C.sort.hist = alloc<int32_t>(256 * (int64_t)sortBlocks);
             C.sort.scanTmp = alloc<int32_t>(std::max(vr::scan_tmp_words(256 * sortBlocks), vr::scan_tmp_words((int)slots)) + 64);
-            HIP_OK(hipHostMalloc((void **)&pinned, 64 * sizeof(int32_t), hipHostMallocDefault));
         }
+        HIP_OK(hipHostMalloc((void **)&pinned, 64 * sizeof(int32_t), hipHostMallocDefault));
         clear();
     }
 
@@ -539,14 +546,56 @@ struct BoxWorld {
         *unlisted = pinned[40 + REPORT_UNLISTED];
     }
 
+    // Three events for the next timed sweep.
+    hipEvent_t *timed_sweep(bool reverse) {
+        size_t at = 3 * (size_t)sweepsTimed;
+        while (sweepEvents.size() < at + 3) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            sweepEvents.push_back(e);
+        }
+        if (sweepReversed.size() <= (size_t)sweepsTimed) sweepReversed.resize(sweepsTimed + 1);
+        sweepReversed[sweepsTimed] = reverse ? 1 : 0;
+        sweepsTimed++;
+        return &sweepEvents[at];
+    }
+
     // BoxStep.sweep, BoxStep.java:217-231.
     void sweep(int kind, bool reverse, int rounds, int mode = 0, const SweepArgs &a = SweepArgs{0, 1, -1.f, -1}) {
+        hipEvent_t *ev = profiling && kind != WARM ? timed_sweep(reverse) : nullptr;
+        if (ev) HIP_OK(hipEventRecord(ev[0], stream));
         if (reverse) launch_walls(D, kind, reverse, stream, mode, a);
+        if (ev && reverse) HIP_OK(hipEventRecord(ev[1], stream));
         for (int order = 0; order < rounds; order++) {
             int r = reverse ? rounds - 1 - order : order;
             launch_solve_round(D, kind, reverse, roundBase[r], roundPairs[r], stream, mode, a);
+            if (ev && roundPairs[r] > 0) roundsThisStep++;
         }
+        if (ev && !reverse) HIP_OK(hipEventRecord(ev[1], stream));
         if (!reverse) launch_walls(D, kind, reverse, stream, mode, a);
+        if (ev) HIP_OK(hipEventRecord(ev[2], stream));
+    }
+
+    // After a profiled step (the stream is idle): its sweeps' round and walls times into prof.
+    void profile_step(int touching, int points) {
+        for (int k = 0; k < sweepsTimed; k++) {
+            hipEvent_t *ev = &sweepEvents[3 * (size_t)k];
+            float first = 0, second = 0;
+            HIP_OK(hipEventElapsedTime(&first, ev[0], ev[1]));
+            HIP_OK(hipEventElapsedTime(&second, ev[1], ev[2]));
+            float rounds = sweepReversed[k] ? second : first, walls = sweepReversed[k] ? first : second;
+            prof.round_ns += (int64_t)((double)rounds * 1e6);
+            prof.walls_ns += (int64_t)((double)walls * 1e6);
+        }
+        prof.passes += sweepsTimed;
+        prof.walls_launches += sweepsTimed;
+        prof.round_launches += roundsThisStep;
+        prof.listed_sum += (int64_t)touching * sweepsTimed;
+        prof.points_sum += (int64_t)points * sweepsTimed;
+        prof.bodies_sum += (int64_t)n * sweepsTimed;
+        prof.steps += 1;
+        sweepsTimed = 0;
+        roundsThisStep = 0;
     }
 
     // The solve (or relax) sweeps of substep `sub` (BoxStep.java:204-210): the configured count, or, stopped on the
@@ -574,6 +623,7 @@ struct BoxWorld {
             int needed = d.sweeps;
             while (true) {
                 for (; launched < want; launched++) one(launched);
+                launch_any_open(D, sweepNo, stream);
                 int open = read_word(D.sw.lastOpen);
                 if (open < sweepNo) {  // no scene was open after the last sweep launched
                     needed = open < g0 ? 1 : std::min(open - g0 + 2, d.sweeps);
@@ -721,6 +771,7 @@ struct BoxWorld {
             lastSolve.fell_back = pinned[49];
         }
         solve_report();
+        if (profiling) profile_step(touching, points);
         report_counts(p, rep);
         rep->submissions = grid ? 2 : 1;  // BoxStepper.step: the collide list read back, then the step's list
         rep->gpu_ns = (int64_t)(ms * 1e6);
@@ -754,6 +805,7 @@ struct BoxWorld {
         for (void *p : {(void *)logX, (void *)logT})
             if (p) (void)hipFree(p);
         for (void *p : allocs) (void)hipFree(p);
+        for (hipEvent_t e : sweepEvents) (void)hipEventDestroy(e);
         if (t0) (void)hipEventDestroy(t0);
         if (t1) (void)hipEventDestroy(t1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -818,6 +870,11 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n) {
 void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
 void box_collide_list(BoxWorld *w, vr_step_report *rep) { w->collide_list(rep); }
 void box_tolerance(BoxWorld *w, float tol, bool measure) { w->tolerance(tol, measure); }
+void box_profile(BoxWorld *w, bool on) {
+    w->profiling = on;
+    w->prof = vr_profile{};
+}
+void box_profile_read(BoxWorld *w, vr_profile *out) { *out = w->prof; }
 void box_solve_read(BoxWorld *w, vr_box_solve *out) { *out = w->lastSolve; }
 int32_t box_solve_curve(BoxWorld *w, float *out, int32_t n) { return w->solve_curve(out, n); }
 void box_wall_impulses_written(BoxWorld *w) { w->wall_impulses_written(); }

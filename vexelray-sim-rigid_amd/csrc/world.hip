@@ -947,6 +947,7 @@ int vr_world_diagnostics_static(vr_world *w, double *out, int32_t n) {
 
 int vr_world_profile(vr_world *w, int32_t on) {
     if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    if (w->box) return guard([&] { vrbox::box_profile(w->box, on != 0); });
     w->profiling = on != 0;
     w->prof = vr_profile{};
     return VR_OK;
@@ -954,6 +955,7 @@ int vr_world_profile(vr_world *w, int32_t on) {
 
 int vr_world_profile_read(vr_world *w, vr_profile *out) {
     if (!w || !out) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (w->box) return guard([&] { vrbox::box_profile_read(w->box, out); });
     *out = w->prof;
     return VR_OK;
 }
