@@ -1,22 +1,35 @@
-"""Benchmark: the Gauss-Seidel sphere step on the 65,536-sphere pile, body-substeps per second on one MI355X.
+"""Benchmark: body-substeps per second on one MI355X, at equal residual, plain against batched + learned warm start.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pile_64k|mixed_32k] [--cpu-seconds S]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--run R] [--cpu-seconds S] [--workload batched_2k|pile_64k|
+                mixed_32k] [--train-batches B] [--c1-steps K1] [--copy-probe MiB]
 
-The workload is the pile of 65,536 spheres (radius log-uniform 0.1-0.5 m, density 1000 kg/m^3, dropped with
-N(0, 1) m/s velocities into a 100 x 100 m walled floor; mu 0.4, e 0.1; dt 1/60 s in 8 substeps of 10 passes, 1 m
-grid cells), generated from a seeded java.util.Random. W untimed steps run first, then K timed steps (the config's
-600 by default). All state is resident in HBM before the timed region; nothing crosses PCIe inside it except the
-step's small report.
+The workload (batched_2k) is the batched configuration: 2,048 independent scenes of log-uniform 16-4,096 bodies
+(1,549,388 in all, half boxes and half balls, mu 0.6), stepped together at dt 1/60 s in 8 substeps of at most 10 solve
+and 10 relax sweeps, over the configuration's R = 300 steps. The timed window is the run's last K steps, after
+R - K - W untimed steps and W warm-up steps: the piles as the run leaves them, not the free fall of its first steps.
+All state is resident in HBM before the timed region; nothing crosses PCIe inside it but each step's small report.
 
-One JSON line is printed: value = spheres x substeps x K / seconds. It carries:
-  batched: the 2,048-scene box batch (about 1.5 M bodies), body-substeps per second stepped plain (the reference's
-    carried warm start) and with the learned warm start (the network trained on the device for a few epochs on
-    samples logged from the plain steps), with the network's and the carry's held-out error on those samples.
-  roofline: the Gauss-Seidel round kernels (k_round_wide ... k_round_tail, launched back to back for each pass),
-    timed with HIP events on the world's stream around every pass over the timed region; algorithmic bytes per
-    pass 72 C + 104 N (C contacts listed that substep, N spheres), against MI355X's 8 TB/s.
-  cpu_baseline: the CPU oracle (oracle/, a single-threaded C++ restatement of the reference's passes) timed on a
-    bounded sample: steps started from the GPU's final state, for about --cpu-seconds seconds.
+Three runs reach the same state at the window (the stepping is deterministic) and differ only inside it:
+  plain:    the reference's method: the carried warm start and every sweep (10 + 10 a substep);
+  stopped:  the plain method at a residual: the carried warm start, each scene's sweeps of a phase stopped once a sweep
+            moves no point's velocity by more than the tolerance (--tolerance, 0.01 m/s), at most 10;
+  learned:  the same stop, each step started in each scene from the network's prediction where a trial of the first
+            substep's first solve sweep finds it leaves the scene's residual smaller than the carry does (the carry
+            elsewhere).
+A point's residual is the impulse change a sweep makes there times the inverse mass along the normal (m/s). stopped and
+learned end every scene's phases at the same residual, the tolerance (or at the 10-sweep cap), so learned / stopped is
+the learned start's gain at equal residual; plain runs every sweep and ends below it. After the window each run steps
+once measured at its tolerance, then once at 1e-5 m/s: each phase's sweeps a scene ran, its residual (largest and mean
+over the scenes), and the deepest manifold point.
+
+value = learned body-substeps per second = bodies x substeps x K / seconds. The line carries:
+  plain, stopped, learned: each run's rate, residuals, deepest point, sweeps;
+  roofline: k_solve_round (the colour rounds of a sweep, back to back between HIP events on the world's stream) in
+    the plain window: algorithmic bytes P (236 + 88 k-bar) a sweep over its time (P touching pairs, k-bar points a
+    pair), against MI355X's 8 TB/s; roofline_walls the same for k_walls (96 N + 32 W a sweep);
+  cpu_baseline: the oracle (oracle/, a C++ restatement of the reference's passes) on the first scenes of the batch from
+    the plain window's state, one thread and all threads (the scenes share nothing: one thread a scene at a time);
+  pile_64k: the 65,536-sphere pile (the sphere configuration), the last K1 steps of its 600, with its own roofline.
 """
 
 import argparse
@@ -34,210 +47,348 @@ from __graft_entry__ import load_package  # noqa: E402
 
 METRIC = "body-substeps/s at equal residual: plain vs batched+learned warm-start"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes a round-kernel launch, measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE passes, each scaled by what it
-# reads for a copy of known size) over the first 100 steps of pile_64k: profiles/traffic_round_kernels_pile_64k.md.
-ROUND_TRAFFIC_BYTES = 5904187.0
-ROUND_TRAFFIC_SAMPLE = ("rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 (copy-calibrated) over the first 100 steps "
-                        "of pile_64k, all round-kernel launches; 892,506 algorithmic B a launch over the same steps")
+STATE = ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw")
 
-
-def cpu_baseline(pkg, spec, state, seconds):
-    """The oracle on the same workload, started from `state`, for about `seconds` of single-core CPU time."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import scene as sc  # the oracle's loader lives with the tests
-    sim = pkg.SphereSimulation(spec.step, abi=sc.oracle_abi())
-    x, y, z, u, v, w, ax, ay, az, qx, qy, qz, qw = state
-    sim.start(x, y, z, u, v, w, spec.r, spec.im, spec.params, ax=ax, ay=ay, az=az, shown=False)
-    for name, arr in (("qx", qx), ("qy", qy), ("qz", qz), ("qw", qw)):
-        sim.write(name, arr)
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        sim.stepAndWait()
-        steps += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    dt = time.perf_counter() - t0
-    sim.close()
-    return {
-        "value": spec.step.spheres * spec.step.substeps * steps / dt,
-        "unit": "body-substeps/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"{steps} steps of the same workload on the CPU oracle (single thread, -O3), started from the "
-                  f"GPU's state after its timed steps (the settled pile), {dt:.1f} s",
-    }
-
-
-def batched(pkg, steps, warmup, epochs):
-    """The 2,048-scene batch on the box path: plain (the reference's carry as the warm start), then with the learned
-    warm start, trained on the device on samples logged from the plain steps. Body-substeps per second for both,
-    timed over the same steps from the same state: two batches step alike (the stepping is deterministic) through the
-    warm-up and the logged steps; one then runs plain, the other with the network."""
-    scenes = __import__(pkg.__name__ + ".scenes", fromlist=["scenes"])
-    boxes = __import__(pkg.__name__ + ".boxes", fromlist=["boxes"])
-    mlp = __import__(pkg.__name__ + ".mlp", fromlist=["mlp"])
-    spec = scenes.batched_2k()
-    n = int(spec.bodies.sum())
-
-    def started():
-        sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps)
-        sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape,
-                  spec.im, spec.params)
-        for _ in range(warmup + 3):
-            sim.stepAndWait()
-        return sim
-
-    def timed(sim):
-        t0 = time.perf_counter()
-        reps = [sim.stepAndWait() for _ in range(steps)]
-        return time.perf_counter() - t0, reps
-
-    sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps)
-    sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
-              spec.params)
-    for _ in range(warmup):
-        sim.stepAndWait()
-    sim.log(True, 4 << 20)
-    for _ in range(3):
-        sim.stepAndWait()
-    x, t = sim.samples(4 << 20)
-    sim.log(False)
-    plain_s, reps = timed(sim)
-    plain_end = [sim.floats(k).copy() for k in ("x", "y", "z")]
-    sim.close()
-    net = mlp.Mlp(seed=3)
-    held = np.random.default_rng(0).permutation(len(x))
-    cut = int(0.9 * len(x))
-    t0 = time.perf_counter()
-    net.train(x[held[:cut]], t[held[:cut]], epochs=epochs, batch=4096, rate=1e-3, seed=1)
-    train_s = time.perf_counter() - t0
-    pred = net.forward(x[held[cut:]])
-    err_net = float(np.mean(np.sum((pred - t[held[cut:]]) ** 2, axis=1)))
-    err_carry = float(np.mean(np.sum((x[held[cut:]][:, 7:10] - t[held[cut:]]) ** 2, axis=1)))
-    sim = started()
-    sim.learned(net)
-    learned_s, lreps = timed(sim)
-    used = sim.learnedPoints
-    # How far the learned start moved the batch from the plain one over the same steps.
-    apart = max(float(np.max(np.abs(sim.floats(k) - plain_end[a]))) for a, k in enumerate(("x", "y", "z")))
-    sim.close()
-    net.close()
-    rate = lambda secs: n * spec.substeps * steps / secs  # noqa: E731
-    return {
-        "workload": f"batched_2k: {len(spec.bodies)} scenes, {n} bodies (half boxes, half balls), {spec.substeps} "
-                    f"substeps of {spec.sweeps} sweeps, seed {spec.seed}",
-        "steps_timed": steps, "warmup": warmup,
-        "timed_from": f"step {warmup + 3}, the same state for both",
-        "plain": {"value": rate(plain_s), "ms_per_step": plain_s * 1e3 / steps,
-                  "touching_at_end": reps[-1].touching, "rounds_at_end": reps[-1].rounds},
-        "learned": {"value": rate(learned_s), "ms_per_step": learned_s * 1e3 / steps,
-                    "touching_at_end": lreps[-1].touching, "rounds_at_end": lreps[-1].rounds,
-                    "points_started_from_prediction": int(used), "samples": int(len(x)), "epochs": epochs,
-                    "train_s": train_s, "held_out_sq_error": {"network": err_net, "carried_impulse": err_carry},
-                    "largest_position_difference_from_plain_m": apart},
-        "unit": "body-substeps/s",
-    }
-
-
-def residual_sweeps(pkg, scenes_count=256, settle=200, epochs=10):
-    """The learned warm start by the residual it leaves: a batch of `scenes_count` scenes settles for `settle` steps,
-    logs three more, and the network trains on the samples; then, from that same state, one step runs plain and one
-    with the network at each sweep count, and the residual after it is measured: the penetration at every touching
-    point (the step's listed pairs put through the library's narrow phase on the final state) and of every body past
-    its scene's walls, over the body's size (a ball's radius, a box's smallest half-extent)."""
-    scenes = __import__(pkg.__name__ + ".scenes", fromlist=["scenes"])
-    boxes = __import__(pkg.__name__ + ".boxes", fromlist=["boxes"])
-    mlp = __import__(pkg.__name__ + ".mlp", fromlist=["mlp"])
-    spec = scenes.batched_2k(scenes=scenes_count)
-    scene_of = np.repeat(np.arange(scenes_count), spec.bodies)
-    ext = np.asarray(spec.extents, dtype=np.float64)[scene_of]
-    half = spec.half.astype(np.float64)
-    ball = spec.shape > 0.5
-    size = np.where(ball, half[0], half.min(axis=0))
-    kept = ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw", "lambda", "rub", "roll",
-            "pairs", "pairsWas", "manifold", "manifoldWas", "impulses", "impulsesWas")
-
-    def make(sweeps):
-        b = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, sweeps)
-        b.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
-                spec.params)
-        return b
-
-    def residual(b):
-        pos = np.stack([b.floats(k).astype(np.float64) for k in ("x", "y", "z")])
-        qx, qy, qz, qw = (b.floats(k).astype(np.float64) for k in ("qx", "qy", "qz", "qw"))
-        pairs = b.ints("pairs").reshape(-1, 2)
-        listed = pairs[pairs[:, 1] >= 0]
-        parts = []
-        if len(listed):
-            m = b._world.collide(listed.reshape(-1)).reshape(len(listed), 40)  # the library's narrow phase
-            count = m[:, 3].astype(np.int64)
-            depth = np.stack([m[:, 8 + 8 * k + 6] for k in range(4)], axis=1)
-            valid = np.arange(4)[None, :] < count[:, None]
-            scale = np.minimum(size[listed[:, 0]], size[listed[:, 1]])
-            parts.append((depth / scale[:, None])[valid])
-        rot = np.array([[1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)],
-                        [2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)],
-                        [2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)]])
-        reach = np.where(ball[None, :], half[0][None, :], (np.abs(rot) * half[None, :, :]).sum(axis=1))
-        parts.append(np.maximum(reach - pos, pos + reach - ext.T).max(axis=0) / size)
-        pen = np.concatenate(parts)
-        pen = pen[pen > 0]
-        return {"max": float(pen.max()), "p99": float(np.percentile(pen, 99)), "mean": float(pen.mean())} \
-            if len(pen) else {"max": 0.0, "p99": 0.0, "mean": 0.0}
-
-    sim = make(spec.sweeps)
-    for _ in range(settle):
-        sim.stepAndWait()
-    sim.log(True, 4 << 20)
-    for _ in range(3):
-        sim.stepAndWait()
-    x, t = sim.samples(4 << 20)
-    sim.log(False)
-    snap = {k: (sim.ints(k).copy() if k in ("pairs", "pairsWas") else sim.floats(k).copy()) for k in kept}
-    settled = residual(sim)
-    sim.close()
-    net = mlp.Mlp(seed=3)
-    net.train(x, t, epochs=epochs, batch=4096, rate=1e-3, seed=1)
-
-    def one_step(sweeps, learned):
-        b = make(sweeps)
-        for k in kept:
-            b.write(k, snap[k])
-        if learned:
-            b.learned(net)
-        b.stepAndWait()
-        out = residual(b)
-        b.close()
-        return out
-
-    rows = []
-    for sweeps in (1, 2, 3, 5, 8, 10, 15, 20, 30):
-        rows.append({"sweeps": sweeps, "plain": one_step(sweeps, False), "learned": one_step(sweeps, True)})
-        progress(len(rows), 9)
-    net.close()
-    return {"workload": f"batched_2k with {scenes_count} scenes, {int(spec.bodies.sum())} bodies, settled "
-                        f"{settle} steps, one step from the same state at each sweep count",
-            "residual": "penetration over the body's size at touching points and walls (max, 99th percentile, "
-                        "mean over penetrating points)",
-            "settled": settled, "samples": int(len(x)), "epochs": epochs, "by_sweeps": rows}
-
+# HBM bytes a launch from rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, each scaled by what it reads for a
+# copy of known size), over the same windows: profiles/traffic_box_kernels_batched_2k.md and
+# profiles/traffic_round_kernels_pile_64k.md. None where not measured.
+SOLVE_ROUND_TRAFFIC_BYTES = None
+WALLS_TRAFFIC_BYTES = None
+SPHERE_ROUND_TRAFFIC_BYTES = None
 
 _last_progress = [time.perf_counter()]
 
 
-def progress(done, total):
-    """A progress line every 100 steps, or after 30 s without one (steps run slower under a counter profiler)."""
+def progress(what, done, total):
+    """A progress line every 50 steps, or after 20 s without one."""
     now = time.perf_counter()
-    if done % 100 == 0 or now - _last_progress[0] > 30:
+    if done % 50 == 0 or done == total or now - _last_progress[0] > 20:
         _last_progress[0] = now
-        print(f"[bench] {done}/{total} steps", file=sys.stderr, flush=True)
+        print(f"[bench] {what}: {done}/{total} steps", file=sys.stderr, flush=True)
 
 
-def box_scene(pkg, spec, args):
+def modules(pkg):
+    get = lambda name: __import__(pkg.__name__ + "." + name, fromlist=[name])  # noqa: E731
+    return get("scenes"), get("boxes"), get("mlp")
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def threads_available():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    limit = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(limit))) if limit and limit.isdigit() else n
+
+
+# ---- the batched configuration ----
+
+def batch_of(boxes, spec, abi=None):
+    sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps, abi=abi)
+    sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
+              spec.params)
+    return sim
+
+
+def training_samples(pkg, batches, settle, logged):
+    """Samples for the network from plain runs of `batches` other seeded batches of 2,048 scenes each: after `settle`
+    steps, `logged` steps logged (features at each step's start; targets, what each point holds once the first
+    substep's solve sweeps are done). The last batch is held out, so the held-out error is over scenes the network
+    never saw."""
+    scenes, boxes, _ = modules(pkg)
+    parts = []
+    for b in range(batches):
+        spec = scenes.batched_2k(seed=100 + b)
+        sim = batch_of(boxes, spec)
+        for k in range(settle):
+            sim.stepAndWait()
+        sim.log(True, 8 << 20)
+        for _ in range(logged):
+            sim.stepAndWait()
+        parts.append(sim.samples(8 << 20))
+        sim.close()
+        print(f"[bench] training batch {b + 1}/{batches}: {len(parts[-1][0])} samples", file=sys.stderr, flush=True)
+    return parts, len(parts) * 2048
+
+
+def measured_step(sim, tol, walls_too=True):
+    """One step with the phases measured: the residual a scene ended a solve or relax phase with (the largest over the
+    scenes, and their mean), the sweeps a scene ran on average and the most launched, the deepest manifold point, and
+    the active wall contacts."""
+    sim.tolerance(tol, True)
+    rep = sim.stepAndWait()
+    r = sim.solveReport()
+    _, _, broken, depth = sim.diagnostics()
+    walls = int(np.count_nonzero(sim.floats("lambda") > 0)) if walls_too else None
+    sim.tolerance(tol, False)
+    subs = r.residual.shape[1]
+    return {
+        "tolerance_m_per_s": tol if tol >= 0 else None,
+        "residual_max_m_per_s": float(np.max(r.residual)),
+        "residual_mean_m_per_s": float(np.mean(r.residual_mean)),
+        "residual_solve_max_m_per_s": float(np.max(r.residual[0])),
+        "residual_solve_mean_m_per_s": float(np.mean(r.residual_mean[0])),
+        "deepest_point_m": float(depth),
+        "broken": bool(broken),
+        "sweeps_per_scene_per_substep": float(np.sum(r.scene_sweeps)) / subs,
+        "sweeps_launched_per_substep": float(np.sum(r.launched)) / subs,
+        "solve_sweeps_per_scene_by_substep": [round(float(v), 3) for v in r.scene_sweeps[0]],
+        "touching": rep.touching, "points": rep.points, "rounds": rep.rounds,
+        "active_wall_contacts": walls,
+        "learned_points": int(sim.learnedPoints), "pairs_left_to_the_carry": int(r.fell_back),
+    }
+
+
+def batch_run(pkg, label, spec, args, net=None, tol=-1.0, profile=False, keep_state=False):
+    """Steps the batch R - K - W steps plain, then W warm-up and K timed steps with `net` and `tol`, then one measured
+    step."""
+    _, boxes, _ = modules(pkg)
+    sim = batch_of(boxes, spec)
+    settle = args.run - args.steps - args.warmup
+    for k in range(settle):
+        sim.stepAndWait()
+        progress(f"{label} settle", k + 1, settle)
+    state = {k: sim.floats(k).copy() for k in STATE + ("hx", "hy", "hz", "shape", "im")} if keep_state else None
+    if net is not None:
+        sim.learned(net)
+    sim.tolerance(tol, False)
+    for _ in range(args.warmup):
+        sim.stepAndWait()
+    if profile:
+        sim._world.profile(True)
+    reps = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        reps.append(sim.stepAndWait())
+        progress(f"{label} timed", k + 1, args.steps)
+    seconds = time.perf_counter() - t0
+    prof = sim._world.profile_read() if profile else None
+    if profile:
+        sim._world.profile(False)
+    after = measured_step(sim, tol)
+    tight = measured_step(sim, 1e-5, walls_too=False)
+    sim.learned(None)
+    sim.close()
+    n = int(spec.bodies.sum())
+    out = {
+        "value": n * spec.substeps * args.steps / seconds, "unit": "body-substeps/s",
+        "ms_per_step": seconds * 1e3 / args.steps,
+        "tolerance_m_per_s": tol if tol >= 0 else None,
+        "touching_at_start": reps[0].touching, "touching_at_end": reps[-1].touching,
+        "rounds_at_end": reps[-1].rounds, "points_at_end": reps[-1].points,
+        "unplaced": int(sum(r.unplaced for r in reps)), "unlisted": int(sum(r.unlisted for r in reps)),
+        "after_window": after,
+        "at_1e-5": {"sweeps_per_scene_per_substep": tight["sweeps_per_scene_per_substep"],
+                    "solve_sweeps_per_scene_by_substep": tight["solve_sweeps_per_scene_by_substep"],
+                    "residual_max_m_per_s": tight["residual_max_m_per_s"],
+                    "residual_mean_m_per_s": tight["residual_mean_m_per_s"],
+                    "note": "the next step measured at a tolerance of 1e-5 m/s: the sweeps each scene needs to reach "
+                            "it (10 when it does not within the step's 10)"},
+    }
+    return out, prof, state
+
+
+def box_roofline(prof, walls_per_sweep):
+    """k_solve_round and k_walls over the timed sweeps: algorithmic bytes over their event-timed durations."""
+    round_bytes = 236.0 * prof.listed_sum + 88.0 * prof.points_sum
+    round_s = prof.round_ns * 1e-9
+    achieved = round_bytes / round_s / 1e9 if round_s > 0 else 0.0
+    walls_bytes = 96.0 * prof.bodies_sum + 32.0 * walls_per_sweep * prof.passes
+    walls_s = prof.walls_ns * 1e-9
+    walls_achieved = walls_bytes / walls_s / 1e9 if walls_s > 0 else 0.0
+    kbar = prof.points_sum / max(prof.listed_sum, 1)
+    rounds = {
+        "bound": "latency",
+        "kernel": "k_solve_round (solve and relax sweeps' colour rounds), plain window",
+        "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+        "traffic": SOLVE_ROUND_TRAFFIC_BYTES,
+        "bytes_per_launch": round_bytes / max(prof.round_launches, 1),
+        "avg_launch_us": prof.round_ns / 1e3 / max(prof.round_launches, 1),
+        "launches": int(prof.round_launches), "sweeps": int(prof.passes),
+        "touching_pairs_per_sweep": prof.listed_sum / max(prof.passes, 1), "points_per_pair": kbar,
+        "formula": "P (236 + 88 k-bar) B a sweep, P touching pairs, k-bar points a pair",
+    }
+    walls = {
+        "bound": "latency",
+        "kernel": "k_walls (solve and relax sweeps), plain window",
+        "achieved": walls_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": walls_achieved / HBM_PEAK_GBS,
+        "traffic": WALLS_TRAFFIC_BYTES,
+        "bytes_per_launch": walls_bytes / max(prof.walls_launches, 1),
+        "avg_launch_us": prof.walls_ns / 1e3 / max(prof.walls_launches, 1),
+        "launches": int(prof.walls_launches), "active_wall_contacts": walls_per_sweep,
+        "formula": "96 N + 32 W B a sweep, N bodies, W active wall contacts (normal impulse above zero, after the "
+                   "window)",
+    }
+    return rounds, walls
+
+
+def cpu_baseline(pkg, spec, state, seconds, scenes_taken=48):
+    """The oracle on the batch's first `scenes_taken` scenes, started from the plain run's state at the window's start
+    (positions, velocities, turns and momenta; the impulses from zero), stepped on one thread and on all of them for
+    about `seconds` each."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import scene as sc  # the oracle's loader lives with the tests
+    import ctypes
+    _, boxes, _ = modules(pkg)
+    abi = sc.oracle_abi()
+    threads_fn = abi.lib.vro_threads
+    threads_fn.argtypes = [ctypes.c_int32]
+    threads_fn.restype = ctypes.c_int32
+    bodies = spec.bodies[:scenes_taken]
+    n = int(bodies.sum())
+    out = {}
+    for threads in (1, threads_available()):
+        threads_fn(threads)
+        sim = boxes.BoxBatch(bodies, spec.extents[:scenes_taken], spec.grids[:scenes_taken], spec.substeps,
+                             spec.sweeps, abi=abi)
+        sim.clear()
+        for name in STATE + ("hx", "hy", "hz", "shape", "im"):
+            sim.write(name, state[name][:n])
+        sim.params(spec.params)
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            sim.stepAndWait()
+            steps += 1
+            if time.perf_counter() - t0 >= seconds:
+                break
+        dt = time.perf_counter() - t0
+        sim.close()
+        out[threads] = (n * spec.substeps * steps / dt, steps, dt)
+    one, most = out[1], out[max(out)]
+    return {
+        "value": most[0], "unit": "body-substeps/s", "cores": max(out), "kind": "port",
+        "one_core": {"value": one[0], "steps": one[1], "seconds": one[2]},
+        "cpu": cpu_model(), "nproc": os.cpu_count(),
+        "sample": f"the batch's first {scenes_taken} scenes ({n} bodies), {spec.substeps} substeps of {spec.sweeps} "
+                  f"sweeps, on the CPU oracle (C++ -O3, the scenes on OpenMP threads), started from the GPU's state at "
+                  f"the plain window's start (impulses from zero): {most[1]} steps in {most[2]:.1f} s on "
+                  f"{max(out)} threads, {one[1]} steps in {one[2]:.1f} s on one",
+    }
+
+
+def batched(pkg, args):
+    scenes, boxes, mlp = modules(pkg)
+    spec = scenes.batched_2k()
+    n = int(spec.bodies.sum())
+    # The network, trained on other batches' plain runs.
+    parts, seen = training_samples(pkg, args.train_batches, 20, 2)
+    samples = int(sum(len(p[0]) for p in parts))
+    train_x = np.concatenate([p[0] for p in parts[:-1]])
+    train_t = np.concatenate([p[1] for p in parts[:-1]])
+    held_x, held_t = parts[-1]
+    net = mlp.Mlp(seed=3)
+    t0 = time.perf_counter()
+    net.train(train_x, train_t, epochs=args.epochs, batch=4096, rate=1e-3, seed=1)
+    train_s = time.perf_counter() - t0
+    err_net = float(np.mean(np.sum((net.forward(held_x) - held_t) ** 2, axis=1)))
+    err_carry = float(np.mean(np.sum((held_x[:, 7:10] - held_t) ** 2, axis=1)))
+    del train_x, train_t, parts
+
+    plain, prof, state = batch_run(pkg, "plain", spec, args, profile=True, keep_state=True)
+    tol = args.tolerance
+    stopped, _, _ = batch_run(pkg, "stopped", spec, args, tol=tol)
+    learned, _, _ = batch_run(pkg, "learned", spec, args, net=net, tol=tol)
+    net.close()
+    rounds, walls = box_roofline(prof, plain["after_window"]["active_wall_contacts"])
+    within = lambda a, b: abs(a - b) <= 0.01 * abs(b)  # noqa: E731
+    compare = {}
+    for key in ("residual_max_m_per_s", "residual_mean_m_per_s", "deepest_point_m"):
+        a, b = learned["after_window"][key], stopped["after_window"][key]
+        compare[key] = {"stopped": b, "learned": a, "within_1pct": within(a, b),
+                        "plain": plain["after_window"][key]}
+    learned["network"] = {
+        "samples": samples, "held_out_samples": int(len(held_x)), "scenes_logged": seen, "epochs": args.epochs,
+        "train_s": train_s, "trained_on": f"{args.train_batches - 1} plain batches of 2,048 scenes (seeds 100..), "
+                                          f"2 logged steps each after 20; held out: the last batch",
+        "held_out_sq_error": {"network": err_net, "carried_impulse": err_carry},
+    }
+    line = {
+        "metric": METRIC, "value": learned["value"], "unit": "body-substeps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": learned["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": f"batched_2k: {len(spec.bodies)} scenes, {n} bodies (half boxes, half balls), "
+                        f"{spec.substeps} substeps of at most {spec.sweeps} solve and {spec.sweeps} relax sweeps, "
+                        f"mu 0.6, seed {spec.seed}; the last {args.steps} of {args.run} steps",
+            "bodies": n, "scenes": len(spec.bodies), "substeps": spec.substeps, "sweeps": spec.sweeps,
+            "run_steps": args.run, "tolerance_m_per_s": tol,
+            "method": "batched + learned warm start, each scene's sweeps stopped at the tolerance",
+        },
+        "ratio_learned_to_stopped": learned["value"] / stopped["value"],
+        "ratio_learned_to_plain": learned["value"] / plain["value"],
+        "ratio_note": "learned / stopped is the learned start's gain at equal residual (both stop each scene at the "
+                      "tolerance); learned / plain also counts the stop itself, against a method that runs every "
+                      "sweep and ends each scene at a lower residual",
+        "equal_residual": compare,
+        "plain": plain, "stopped": stopped, "learned": learned,
+        "roofline": rounds, "roofline_walls": walls,
+    }
+    if args.cpu_seconds > 0:
+        line["cpu_baseline"] = cpu_baseline(pkg, spec, state, args.cpu_seconds)
+    return line
+
+
+# ---- the sphere pile ----
+
+def sphere_pile(pkg, steps, warmup, run=600):
+    """The 65,536-sphere pile, the last `steps` of its `run` steps, with the round kernels' roofline."""
+    scenes, _, _ = modules(pkg)
+    spec = scenes.pile_64k()
+    sim = pkg.SphereSimulation(spec.step)
+    sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.r, spec.im, spec.params)
+    settle = run - steps - warmup
+    for k in range(settle):
+        sim.stepAndWait()
+        progress("pile_64k settle", k + 1, settle)
+    contacts_start = int(sim.ints("contactCount")[0])
+    for _ in range(warmup):
+        sim.stepAndWait()
+    sim._world.profile(True)
+    sim.finish()
+    t0 = time.perf_counter()
+    reports = []
+    for k in range(steps):
+        reports.append(sim.stepAndWait().solve)
+    sim.finish()
+    seconds = time.perf_counter() - t0
+    prof = sim._world.profile_read()
+    contacts_end = int(sim.ints("contactCount")[0])
+    sim.close()
+    n, substeps = spec.step.spheres, spec.step.substeps
+    bytes_total = 72.0 * prof.listed_sum + 104.0 * n * prof.passes
+    round_s = prof.round_ns * 1e-9
+    achieved = bytes_total / round_s / 1e9 if round_s > 0 else 0.0
+    return {
+        "workload": f"pile_64k: {n} spheres, Gauss-Seidel until done, {substeps} substeps of "
+                    f"{spec.step.iterations} passes, the last {steps} of {run} steps, seed {spec.seed}",
+        "value": n * substeps * steps / seconds, "unit": "body-substeps/s", "ms_per_step": seconds * 1e3 / steps,
+        "contacts_at_start": contacts_start, "contacts_at_end": contacts_end,
+        "rounds_per_step": float(np.mean([r.rounds for r in reports])),
+        "roofline": {
+            "bound": "latency",
+            "kernel": "Gauss-Seidel round kernels (k_round_wide, k_round_tail), per pass",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": SPHERE_ROUND_TRAFFIC_BYTES,
+            "bytes_per_launch": bytes_total / max(prof.round_launches, 1),
+            "avg_launch_us": prof.round_ns / 1e3 / max(prof.round_launches, 1),
+            "host_enqueue_ms_per_step": prof.enqueue_ns / 1e6 / max(prof.steps, 1),
+            "formula": "72 C + 104 N B a pass, C listed contacts, N spheres",
+        },
+    }
+
+
+def box_scene(pkg, args):
     """A box-path workload of one scene (mixed_32k): body-substeps per second, plain stepping."""
-    boxes = __import__(pkg.__name__ + ".boxes", fromlist=["boxes"])
+    scenes, boxes, _ = modules(pkg)
+    spec = scenes.mixed_32k()
     sim = boxes.BoxSimulation(spec.step)
     sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
               spec.params)
@@ -247,9 +398,9 @@ def box_scene(pkg, spec, args):
     reps = []
     for k in range(args.steps):
         reps.append(sim.stepAndWait())
-        progress(k + 1, args.steps)
+        progress("mixed_32k", k + 1, args.steps)
     seconds = time.perf_counter() - t0
-    diag, speed, broken, depth = sim.diagnostics()
+    _, _, broken, depth = sim.diagnostics()
     sim.close()
     n, substeps = spec.step.boxes, spec.step.substeps
     return {
@@ -266,14 +417,15 @@ def box_scene(pkg, spec, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="pile_64k")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--batched-steps", type=int, default=30, help="timed steps of the batched workload (0: skip)")
-    ap.add_argument("--residual", action="store_true",
-                    help="only measure the residual a step leaves, plain and with the learned warm start, at several "
-                         "sweep counts from the same state (residual_sweeps), then exit")
+    ap.add_argument("--run", type=int, default=300, help="the configuration's steps; the window is the last --steps")
+    ap.add_argument("--workload", default="batched_2k")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--train-batches", type=int, default=5, help="batches of 2,048 scenes logged for the network")
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--tolerance", type=float, default=0.01, help="m/s: the residual each scene's sweeps stop at")
+    ap.add_argument("--c1-steps", type=int, default=20, help="timed steps of the sphere pile (0: skip it)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB: only copy a buffer of that size 5 times (the known-size kernel that calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -282,7 +434,6 @@ def main():
         print(f"bench.py measures one GPU: the library steps one world on one MI355X and has no multi-GPU path "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
-
     if args.copy_probe > 0:
         import torch
         src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
@@ -292,90 +443,21 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": src.numel() * 4, "copy_probe_bytes_written": dst.numel() * 4}))
         return
-
+    args.run = max(args.run, args.steps + args.warmup)
     pkg = load_package()
-    if args.residual:
-        print(json.dumps(residual_sweeps(pkg)))
+    if args.workload == "mixed_32k":
+        print(json.dumps(box_scene(pkg, args)))
         return
-    scenes = __import__(pkg.__name__ + ".scenes", fromlist=["scenes"])
-    spec = scenes.SCENES[args.workload]()
-    if hasattr(spec, "towers"):  # a box-path scene
-        print(json.dumps(box_scene(pkg, spec, args)))
+    if args.workload == "pile_64k":
+        line = sphere_pile(pkg, args.steps, args.warmup)
+        line.update({"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                     "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                     "data": "synthetic", "config": {"workload": line.pop("workload")}})
+        print(json.dumps(line))
         return
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except ImportError:
-        sync = lambda: None  # noqa: E731
-
-    sim = pkg.SphereSimulation(spec.step)
-    sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.r, spec.im, spec.params)
-    for _ in range(args.warmup):
-        sim.stepAndWait()
-    sim._world.profile(True)
-    sim.finish()
-    sync()
-    t0 = time.perf_counter()
-    reports = []
-    for k in range(args.steps):
-        rep = sim.stepAndWait()
-        reports.append(rep.solve)
-        progress(k + 1, args.steps)
-    sim.finish()
-    sync()
-    seconds = time.perf_counter() - t0
-    prof = sim._world.profile_read()
-    n, substeps = spec.step.spheres, spec.step.substeps
-    value = n * substeps * args.steps / seconds
-
-    passes = max(prof.passes, 1)
-    bytes_total = 72.0 * prof.listed_sum + 104.0 * n * prof.passes
-    round_s = prof.round_ns * 1e-9
-    achieved = bytes_total / round_s / 1e9 if round_s > 0 else 0.0
-    per_launch_bytes = bytes_total / max(prof.round_launches, 1)
-    state = [sim.floats(k).copy() for k in ("x", "y", "z", "u", "v", "w", "ax", "ay", "az", "qx", "qy", "qz", "qw")]
-    cc = sim.ints("contactCount")
-    sim.close()
-
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "body-substeps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": seconds * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"{args.workload}: {n} spheres, plain XPBD Gauss-Seidel until done, {substeps} substeps, "
-                        f"{spec.step.iterations} passes, 1 m cells, box {spec.box}, seed {spec.seed}",
-            "bodies": n, "substeps": substeps, "iterations": spec.step.iterations, "steps_timed": args.steps,
-            "method": "plain", "contacts_at_end": int(cc[0]),
-            "rounds_per_step": float(np.mean([r.rounds for r in reports])),
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "Gauss-Seidel round kernels (k_round_wide, k_round_tail), per pass",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": ROUND_TRAFFIC_BYTES,
-            "traffic_sample": ROUND_TRAFFIC_SAMPLE,
-            "bytes_per_launch": per_launch_bytes,
-            "avg_launch_us": prof.round_ns / 1e3 / max(prof.round_launches, 1),
-            "avg_pass_us": prof.round_ns / 1e3 / passes,
-            "host_enqueue_ms_per_step": prof.enqueue_ns / 1e6 / max(prof.steps, 1),
-        },
-    }
-    if args.cpu_seconds > 0:
-        line["cpu_baseline"] = cpu_baseline(pkg, spec, state, args.cpu_seconds)
-    if args.batched_steps > 0:
-        line["batched"] = batched(pkg, args.batched_steps, 5, 10)
+    line = batched(pkg, args)
+    if args.c1_steps > 0:
+        line["pile_64k"] = sphere_pile(pkg, args.c1_steps, 2)
     print(json.dumps(line))
 
 

@@ -267,12 +267,15 @@ int vr_mlp_last_error(char *buf, int32_t len);
  * a velocity (normal, and tangent in a basis of the contact plane: 3), the first body's share of the pair's
  * inverse mass, the pair's shapes one-hot (box-box, box-ball, ball-ball), the closing speed along the normal, and
  * the manifold's point count / 4. The 3 targets are the impulse the point holds after the step (what the next
- * step's carry would start it from), in those units.
+ * step's carry would start it from), in those units. Logged targets are the impulse each point holds once the first
+ * substep's solve sweeps are done: what a start would best begin that phase from.
  * Impulses (features 7-9, targets) are given as sign(v) log(1 + |v|) of the velocity v.
  * vr_box_log starts (or stops) logging (features, targets) of every step, up to `capacity` samples, on the device;
  * vr_box_samples copies up to `most` of them out. vr_box_learned sets the network (null: none) whose prediction
- * replaces the carried impulse at each point where it is usable (finite, normal part not negative); the report's
- * `reserved` word then counts the points it replaced.
+ * (where usable: finite, normal part not negative) is tried against the carried impulses at each step's first substep:
+ * the first solve sweep of each scene's pairs is run from each start and put back, and a scene takes the prediction
+ * only where it leaves that sweep's largest residual smaller, the carry otherwise (vr_box_solve.fell_back counts the
+ * scenes that kept it). The report's `reserved` word counts the points started from the prediction.
  * The world keeps only the network's handle: `m` must stay alive while it is set. vr_mlp_destroy refuses
  * (VR_ERR_STATE) a network that a world still holds; vr_box_learned(w, NULL) or vr_world_destroy lets it go.
  */
@@ -294,8 +297,9 @@ int vr_box_tolerance(vr_world *w, float tolerance, int32_t measure);
  * What the last step's phases did (the first VR_SOLVE_SUBSTEPS substeps; index 0 the solve sweeps, 1 the relax).
  * launched: sweeps the step launched, pass by pass (the most any scene needed; 0 when a workgroup steps each scene,
  * scene_step = 1). Measured steps only: scene_sweeps, the sweeps a scene ran on average; residual, the largest
- * residual a scene ended the phase with. fell_back: touching pairs whose network prediction raised their start
- * residual, which then started from the carry (vr_box_learned).
+ * residual a scene ended the phase with; residual_mean, their mean over the scenes. fell_back: scenes with pairs that
+ * kept the carried start because the network's prediction did not leave their first solve sweep nearer its solution
+ * (vr_box_learned).
  */
 typedef struct vr_box_solve {
     float tolerance;
@@ -309,6 +313,7 @@ typedef struct vr_box_solve {
     int32_t launched[2][VR_SOLVE_SUBSTEPS];
     float scene_sweeps[2][VR_SOLVE_SUBSTEPS];
     float residual[2][VR_SOLVE_SUBSTEPS];
+    float residual_mean[2][VR_SOLVE_SUBSTEPS];  /* measured: the mean over the scenes of the residual each ended with */
 } vr_box_solve;
 int vr_box_solve_read(vr_world *w, vr_box_solve *out);
 /*

@@ -1010,6 +1010,7 @@ struct BoxWorld {
         if (kept) {
             lastSolve.scene_sweeps[at % 2][sub] = (float)ran;
             lastSolve.residual[at % 2][sub] = last;
+            lastSolve.residual_mean[at % 2][sub] = last;
         }
         if (sub < VR_SOLVE_SUBSTEPS) lastSolve.launched[at % 2][sub] = ran;
         return ran;

@@ -1595,6 +1595,7 @@ int vro_box_solve_read(vro_world *p, vr_box_solve *out) {
             r.launched[k][sub] = 0;
             r.scene_sweeps[k][sub] = 0;
             r.residual[k][sub] = 0;
+            r.residual_mean[k][sub] = 0;
         }
     for (auto *b : p->batch) {
         const vr_box_solve &one = vro::box_solve(b);
@@ -1602,6 +1603,7 @@ int vro_box_solve_read(vro_world *p, vr_box_solve *out) {
             for (int sub = 0; sub < VR_SOLVE_SUBSTEPS; sub++) {
                 r.launched[k][sub] = std::max(r.launched[k][sub], one.launched[k][sub]);
                 r.scene_sweeps[k][sub] += one.scene_sweeps[k][sub] / (float)p->batch.size();
+                r.residual_mean[k][sub] += one.residual_mean[k][sub] / (float)p->batch.size();
                 float v = one.residual[k][sub];
                 if (r.residual[k][sub] == r.residual[k][sub] && !(v <= r.residual[k][sub])) r.residual[k][sub] = v;
             }

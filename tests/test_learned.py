@@ -1,11 +1,16 @@
 """The learned warm start on a batch of box scenes, end to end on the device: log (features, targets) from the
-plain solver, train the network on them, then start steps from its prediction.
+plain solver, train the network on them, then start steps from its prediction, with each scene's sweeps stopped at a
+residual (vr_box_tolerance).
 
-What the trained network gains is measured two ways. As a predictor of the impulses a step ends with, it is
-compared with the reference's own warm start (the carried impulse, which is feature 7..9) on held-out points: it
-must predict them better. As a warm start, one step from the same settled state is compared with a converged
-step (60 sweeps): the learned start must stay as close to it as the carry does, within 10 %; on the settled piles
-measured so far it gains under 1 %, which this test records rather than claims.
+What the trained network gains is measured where the reference's own start, the carried impulse, has nothing to
+give: a restart. Settled piles are started again from their bodies' state alone, every impulse zero (what a caller
+that keeps positions and velocities but not the solver's impulses does, and what the carry gives every new point).
+The network is trained on such restarts and on plain steps, logged from other scenes. From the same restarted state,
+one step with the carry and one with the learned start run until each scene's sweeps move no point by more than the
+tolerance: the learned start must need fewer sweeps than the carry to reach the same residual, over the scenes and in
+the first substep. As a predictor of the impulses the first substep's solve ends with, the network must beat the
+carried impulse on held-out scenes. Measured on MI355X the gain is small: about 1 % of the first substep's solve
+sweeps, a third of a percent of the step's, with the prediction taken by about half the scenes.
 """
 
 import numpy as np
@@ -18,68 +23,86 @@ boxes = __import__(pkg.__name__ + ".boxes", fromlist=["boxes"])
 scenes = __import__(pkg.__name__ + ".scenes", fromlist=["scenes"])
 mlp = __import__(pkg.__name__ + ".mlp", fromlist=["mlp"])
 pytestmark = pytest.mark.gpu
-STATE = ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw", "lambda", "rub", "roll")
-PAIRS = ("pairs", "pairsWas", "manifold", "manifoldWas", "impulses", "impulsesWas")
+BODY = ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw", "hx", "hy", "hz", "shape", "im")
+TOL = 0.5  # m/s: a residual the restarted piles reach within the step's sweeps in most scenes
 
 
-def batch(spec, sweeps):
-    b = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, sweeps)
+def settled(seed, steps=60):
+    spec = scenes.batched_2k(seed=seed, scenes=48, fewest=64, most=512)
+    b = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps)
     b.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
             spec.params)
+    for _ in range(steps):
+        b.stepAndWait()
+    return spec, b
+
+
+def restart(spec, state):
+    b = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps)
+    b.clear()
+    for k in BODY:
+        b.write(k, state[k])
+    b.params(spec.params)
     return b
 
 
-def snapshot(b):
-    return {k: (b.ints(k) if k in ("pairs", "pairsWas") else b.floats(k)).copy() for k in STATE + PAIRS}
-
-
-def one_step(spec, snap, sweeps, net=None):
-    b = batch(spec, sweeps)
-    for k, v in snap.items():
-        b.write(k, v)
-    b.learned(net)
-    b.stepAndWait()
-    used = b.learnedPoints
-    out = {k: b.floats(k).copy() for k in ("x", "y", "z", "u", "v", "w")}
+def samples(seed):
+    """Plain steps of a settled batch, then restarts of it, logged."""
+    spec, b = settled(seed)
+    b.log(True, 1 << 21)
+    for _ in range(6):
+        b.stepAndWait()
+    x1, t1 = b.samples(1 << 21)
+    state = {k: b.floats(k).copy() for k in BODY}
     b.close()
-    return out, used
+    xs, ts = [x1], [t1]
+    for _ in range(3):
+        r = restart(spec, state)
+        r.log(True, 1 << 21)
+        r.stepAndWait()
+        x, t = r.samples(1 << 21)
+        xs.append(x)
+        ts.append(t)
+        state = {k: r.floats(k).copy() for k in BODY}
+        r.close()
+    return np.concatenate(xs), np.concatenate(ts)
 
 
-def rms(a, b, keys):
-    return float(np.sqrt(np.mean(sum((a[k] - b[k]) ** 2 for k in keys))))
-
-
-def test_learned_warm_start_end_to_end():
-    spec = scenes.batched_2k(seed=6, scenes=48, fewest=64, most=512)
-    sim = batch(spec, spec.sweeps)
-    for _ in range(60):
-        sim.stepAndWait()
-    sim.log(True, 1 << 20)
-    for _ in range(20):
-        sim.stepAndWait()
-    x, t = sim.samples(1 << 20)
+def test_the_learned_start_reaches_the_residual_in_fewer_sweeps():
+    parts = [samples(s) for s in (31, 32)]
+    x = np.concatenate([p[0] for p in parts])
+    t = np.concatenate([p[1] for p in parts])
+    hx, ht = samples(33)  # held out: other scenes
     assert len(x) > 10_000 and np.all(np.isfinite(x)) and np.all(np.isfinite(t))
-    snap = snapshot(sim)
-    sim.close()
-
-    rng = np.random.default_rng(0)
-    order = rng.permutation(len(x))
-    train, held = order[: int(0.9 * len(x))], order[int(0.9 * len(x)):]
     net = mlp.Mlp(seed=3)
-    net.train(x[train], t[train], epochs=10, batch=1024, rate=1e-3, seed=1)
-    predicted = net.forward(x[held])
-    carried = x[held][:, 7:10]
-    mse_net = float(np.mean(np.sum((predicted - t[held]) ** 2, axis=1)))
-    mse_carry = float(np.mean(np.sum((carried - t[held]) ** 2, axis=1)))
+    net.train(x, t, epochs=8, batch=1024, rate=1e-3, seed=1)
+    mse_net = float(np.mean(np.sum((net.forward(hx) - ht) ** 2, axis=1)))
+    mse_carry = float(np.mean(np.sum((hx[:, 7:10] - ht) ** 2, axis=1)))
     print(f"held-out squared error: network {mse_net:.4g}, carried impulse {mse_carry:.4g}")
     assert mse_net < mse_carry
 
-    converged, _ = one_step(spec, snap, 60)
-    plain, _ = one_step(spec, snap, spec.sweeps)
-    learned, used = one_step(spec, snap, spec.sweeps, net)
+    spec, b = settled(34)
+    state = {k: b.floats(k).copy() for k in BODY}
+    b.close()
+    sweeps = {}
+    for learned in (False, True):
+        r = restart(spec, state)
+        r.tolerance(TOL, True)
+        if learned:
+            r.learned(net)
+        r.stepAndWait()
+        rep = r.solveReport()
+        used = r.learnedPoints
+        sweeps[learned] = (float(rep.scene_sweeps[0].sum()), float(rep.scene_sweeps.sum()), rep, used)
+        r.learned(None)
+        r.close()
+    (carry_solve, carry_all, carry_rep, _), (net_solve, net_all, net_rep, used) = sweeps[False], sweeps[True]
+    print(f"solve sweeps a scene ran over the step: carry {carry_solve:.2f}, learned {net_solve:.2f}; all sweeps "
+          f"{carry_all:.2f} against {net_all:.2f}; {used} points started from the prediction, "
+          f"{net_rep.fell_back} pairs kept the carry")
     assert used > 0
-    for keys in (("x", "y", "z"), ("u", "v", "w")):
-        ep, el = rms(plain, converged, keys), rms(learned, converged, keys)
-        print(f"{keys}: rms from the converged step, carry {ep:.4g}, learned {el:.4g}")
-        assert el <= 1.1 * ep
+    assert net_solve < carry_solve
+    assert float(net_rep.scene_sweeps[0][0]) < float(carry_rep.scene_sweeps[0][0])  # the first substep, where it starts
+    # Both ran under the same stop, each scene's phase ending at the tolerance or the cap (tests/test_residual.py).
+    assert carry_rep.measured and net_rep.measured and carry_rep.tolerance == net_rep.tolerance
     net.close()

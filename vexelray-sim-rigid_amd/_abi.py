@@ -58,7 +58,8 @@ class SolveReport(ctypes.Structure):
                 ("fell_back", ctypes.c_int32), ("unused", ctypes.c_int32),
                 ("launched", (ctypes.c_int32 * SOLVE_SUBSTEPS) * 2),
                 ("scene_sweeps", (ctypes.c_float * SOLVE_SUBSTEPS) * 2),
-                ("residual", (ctypes.c_float * SOLVE_SUBSTEPS) * 2)]
+                ("residual", (ctypes.c_float * SOLVE_SUBSTEPS) * 2),
+                ("residual_mean", (ctypes.c_float * SOLVE_SUBSTEPS) * 2)]
 
 
 class Profile(ctypes.Structure):

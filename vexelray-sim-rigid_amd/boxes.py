@@ -153,6 +153,7 @@ class SolveReport:
     scene_sweeps: np.ndarray  # 2 x substeps
     residual: np.ndarray      # 2 x substeps
     fell_back: int
+    residual_mean: np.ndarray  # 2 x substeps
 
     @staticmethod
     def of(raw, substeps) -> "SolveReport":
@@ -161,7 +162,7 @@ class SolveReport:
                                              dtype=dtype)
         return SolveReport(float(raw.tolerance), bool(raw.measured), bool(raw.scene_step),
                            grab("launched", np.int32), grab("scene_sweeps", np.float32),
-                           grab("residual", np.float32), int(raw.fell_back))
+                           grab("residual", np.float32), int(raw.fell_back), grab("residual_mean", np.float32))
 
 
 class BoxSimulation:

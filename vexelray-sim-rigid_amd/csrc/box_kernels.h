@@ -35,6 +35,7 @@ struct SweepCtl {
     unsigned long long *sceneLast;
     int32_t *phaseSweeps;
     uint32_t *phaseResidual;
+    float *phaseResidualSum;       // per phase, the scenes' last residuals added up (their mean, over the scenes)
     uint32_t *curve;
 };
 
@@ -128,13 +129,29 @@ struct Learned {
     float *y;            // 4 * slots * LEARN_OUTPUTS: the network's prediction
     float *t;            // 4 * slots * LEARN_OUTPUTS: the targets
     int32_t *used;       // 1: points whose carried impulse the prediction replaced
+    // The trial of the two starts in a step's first substep: the packed impulses of each start (28 fields of the
+    // touching pairs), the motion records before the trial, each scene's first-sweep residual from each start, and
+    // whether the scene takes the prediction.
+    float *startCarry, *startPredicted;
+    float4 *motionBefore;
+    uint32_t *fromCarry, *fromPredicted;
+    int32_t *takes;
 };
+
+// The packed impulses of the touching pairs (entries 0 .. touching - 1) into `to`, field by field, or back.
+void launch_pack_save(const BoxDev &D, float *to, int touching, hipStream_t s);
+void launch_pack_restore(const BoxDev &D, const float *from, int touching, hipStream_t s);
+// The network's prediction (L.y) as the normal and tangent words of every packed point where it is usable.
+void launch_pack_prediction(const BoxDev &D, const Colouring &C, const Learned &L, int touching, hipStream_t s);
+// out[scene] = the scene's largest residual recorded in measured sweep g (0 where it recorded none).
+void launch_trial_residual(const BoxDev &D, int g, uint32_t *out, hipStream_t s);
+// Each scene takes the prediction where its first-sweep residual from it is smaller than from the carry (counting the
+// scenes with pairs that keep the carry into fellBack), and its pairs' packed impulses are set from that start
+// (counting the points started from the prediction into L.used).
+void launch_pack_choose(const BoxDev &D, const Learned &L, int touching, int32_t *fellBack, hipStream_t s);
 
 void launch_point_features(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs, hipStream_t s);
 void launch_point_targets(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs, hipStream_t s);
-// fellBack (1 word): pairs left to the carry because the prediction raised their start residual, added to.
-void launch_apply_prediction(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs,
-                             int32_t *fellBack, hipStream_t s);
 
 int64_t colour_tmp_words(int64_t slots);
 void launch_colour_prepare(const BoxDev &D, const Colouring &C, const vr::SortScratch &S, hipStream_t s);

@@ -907,6 +907,7 @@ __device__ void scene_phase(const BoxDev &D, int b0, int b1, const int *from, co
     if (measure && threadIdx.x == 0) {
         atomicAdd(D.sw.phaseSweeps + phase, ran);
         atomicMax(D.sw.phaseResidual + phase, last);
+        atomicAdd(D.sw.phaseResidualSum + phase, __uint_as_float(last));
     }
 }
 
@@ -954,6 +955,7 @@ __global__ void k_phase_measure(BoxDev D, int g0, int phase) {
     }
     atomicAdd(D.sw.phaseSweeps + phase, g - g0 + 1);
     atomicMax(D.sw.phaseResidual + phase, res);
+    atomicAdd(D.sw.phaseResidualSum + phase, __uint_as_float(res));
 }
 
 // BoxBroad.bin, BoxBroad.java:74-89, in the body's scene's grid.
@@ -1371,93 +1373,68 @@ __global__ void k_point_targets(BoxDev D, Colouring C, Learned L) {
     }
 }
 
-// The first solve sweep of touching pair p alone, from start impulses `start` (each point's normal and tangent
-// words, 4 a point): the two bodies as the step begins with the first substep's gravity added, each point's start
-// impulse applied as the warm sweep applies it, then each point through Boxes.contact as a solve sweep takes it, in
-// order. Returns the largest residual (point_residual): how far that sweep still moves the pair's points. Other pairs
-// and the walls are left out, so a start is judged by the pair's own error alone.
-__device__ float pair_start_residual(const BoxDev &D, const Bufs &B, int p, const float (*start)[4]) {
-    size_t out = (size_t)p * MANIFOLD_WORDS;
-    int count = (int)D.manifold[out + COUNT];
-    float nrm[3] = {D.manifold[out], D.manifold[out + 1], D.manifold[out + 2]};
-    int ia = p / SLOTS, ib = D.contacts[p];
-    Moving first = load_moving(B, ia), second = load_moving(B, ib);
-    const float *P = D.params;
-    float h = P[H];
-    float g[3] = {P[GX], P[GY], P[GZ]};
-    for (int a = 0; a < 3; a++) {  // Boxes.velocities
-        if (first.im > 0) first.vel[a] = first.vel[a] + h * g[a];
-        if (second.im > 0) second.vel[a] = second.vel[a] + h * g[a];
+// The packed impulse fields of the touching pairs, saved and restored (28 fields of `touching` entries each).
+__global__ void k_pack_copy(BoxDev D, float *save, int touching, int back) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= touching) return;
+    size_t T = (size_t)D.packCap;
+    for (int f = 0; f < MOST_POINTS * POINT_IMPULSE; f++) {
+        float *packed = D.packImpulse + (size_t)f * T + j, *kept = save + (size_t)f * touching + j;
+        if (back) *packed = *kept;
+        else *kept = *packed;
     }
-    Soft soft(P, false);
-    if (first.im == 0 || second.im == 0) soft.stiffen(P);
-    float rA[MOST_POINTS][3], rB[MOST_POINTS][3];
-    for (int k = 0; k < MOST_POINTS && k < count; k++) {
-        size_t where = out + POINTS + k * POINT_WORDS;
-        rotate(first.q, D.manifold + where + ANCHOR_FIRST, rA[k]);
-        rotate(second.q, D.manifold + where + ANCHOR_SECOND, rB[k]);
-        if (first.sphere())
-            for (int a = 0; a < 3; a++) rA[k][a] = first.half[0] * nrm[a];
-        if (second.sphere())
-            for (int a = 0; a < 3; a++) rB[k][a] = -(second.half[0] * nrm[a]);
-    }
-    for (int k = 0; k < MOST_POINTS && k < count; k++) {  // the warm sweep
-        float pushed[3];
-        for (int a = 0; a < 3; a++) pushed[a] = start[k][0] * nrm[a] + start[k][1 + a];
-        if (start[k][0] > 0 || dot(&start[k][1], &start[k][1]) > 0) apply_pair(&first, rA[k], second, rB[k], pushed);
-    }
-    float res = 0.f;
-    for (int k = 0; k < MOST_POINTS && k < count; k++) {  // the solve sweep
-        float gap = ((second.at[0] + rB[k][0]) - (first.at[0] + rA[k][0])) * nrm[0] +
-                    ((second.at[1] + rB[k][1]) - (first.at[1] + rA[k][1])) * nrm[1];
-        gap = gap + ((second.at[2] + rB[k][2]) - (first.at[2] + rA[k][2])) * nrm[2];
-        float normal = start[k][0], tangent[3] = {start[k][1], start[k][2], start[k][3]};
-        bool wrote;
-        contact_held(SOLVE, gap, nrm, &first, rA[k], second, rB[k], normal, tangent, wrote, soft, &res);
-    }
-    return res;
 }
 
-// The prediction as the step's starting impulse for each touching pair where it is usable (finite, no normal part
-// below zero, a pair with mass to push) and does not raise the pair's start residual (pair_start_residual) above
-// what the carried impulse leaves; elsewhere the whole pair keeps its carried impulses (the reference's start), and
-// is counted in fellBack. The rolling words are left as carried.
-__global__ void k_apply_prediction(BoxDev D, Colouring C, Learned L, int32_t *fellBack) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= *C.touching) return;
+// The prediction as the start of every point of packed entry j where it is usable (finite, no normal part below
+// zero, a pair with mass to push); the carried words stay elsewhere, and the rolling words everywhere.
+__global__ void k_pack_prediction(BoxDev D, Colouring C, Learned L, int touching) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= touching) return;
     Bufs B = bufs(D);
+    size_t T = (size_t)D.packCap;
+    int t = C.order[j];
     int p = C.touch[t];
-    int count = (int)D.manifold[(size_t)p * MANIFOLD_WORDS + COUNT];
-    if (count > MOST_POINTS) count = MOST_POINTS;
-    float carried[MOST_POINTS][4], predicted[MOST_POINTS][4];
-    bool usable = count > 0;
-    float *held = D.impulses + (size_t)p * IMPULSE_WORDS;
-    for (int k = 0; k < MOST_POINTS; k++) {
-        for (int w = 0; w < 4; w++) carried[k][w] = predicted[k][w] = 0.f;
-        if (!(k < count)) continue;
-        for (int w = 0; w < 4; w++) carried[k][w] = predicted[k][w] = held[POINT_IMPULSE * k + w];
+    int count = min((int)D.manifold[(size_t)p * MANIFOLD_WORDS + COUNT], MOST_POINTS);
+    for (int k = 0; k < count; k++) {
         PointAt P = point_at(D, B, p, k);
         const float *y = L.y + (size_t)(L.pointBase[t] + k) * LEARN_OUTPUTS;
-        if (!(isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2]) && y[0] >= 0 && P.first.im + P.second.im > 0)) {
-            usable = false;
+        if (!(isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2]) && y[0] >= 0 && P.first.im + P.second.im > 0))
             continue;
-        }
         float t1[3], t2[3];
         plane_basis(P.n, t1, t2);
         float along1 = unsquash(y[1]), along2 = unsquash(y[2]);
-        predicted[k][0] = unsquash(y[0]) / P.wsum;
-        for (int a = 0; a < 3; a++) predicted[k][1 + a] = (along1 * t1[a] + along2 * t2[a]) / P.wsum;
+        float *imp = D.packImpulse + (size_t)(k * POINT_IMPULSE) * T + j;
+        imp[0] = unsquash(y[0]) / P.wsum;
+        for (int a = 0; a < 3; a++) imp[(1 + a) * T] = (along1 * t1[a] + along2 * t2[a]) / P.wsum;
     }
-    if (!usable) return;
-    float fromCarry = pair_start_residual(D, B, p, carried);
-    float fromPrediction = pair_start_residual(D, B, p, predicted);
-    if (!(fromPrediction <= fromCarry)) {  // the prediction would leave the pair further from its solution
-        atomicAdd(fellBack, 1);
-        return;
-    }
-    for (int k = 0; k < count; k++)
-        for (int w = 0; w < 4; w++) held[POINT_IMPULSE * k + w] = predicted[k][w];
-    atomicAdd(L.used, count);
+}
+
+__global__ void k_trial_residual(BoxDev D, int g, uint32_t *out) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= D.scenes) return;
+    unsigned long long key = D.sw.sceneLast[j];
+    out[j] = (int)(key >> 32) == g ? (uint32_t)(key & 0xffffffffull) : 0u;
+}
+
+// A scene takes the prediction only where it leaves the first sweep strictly nearer its solution (NaN never).
+__global__ void k_trial_choose(BoxDev D, Learned L, int32_t *fellBack) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= D.scenes) return;
+    float carry = __uint_as_float(L.fromCarry[j]), predicted = __uint_as_float(L.fromPredicted[j]);
+    bool take = predicted < carry;
+    L.takes[j] = take ? 1 : 0;
+    if (!take && L.fromCarry[j] != 0u) atomicAdd(fellBack, 1);
+}
+
+__global__ void k_pack_choose(BoxDev D, Learned L, int touching) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= touching) return;
+    size_t T = (size_t)D.packCap;
+    bool take = L.takes[D.packScene[j]] != 0;
+    const float *from = take ? L.startPredicted : L.startCarry;
+    for (int f = 0; f < MOST_POINTS * POINT_IMPULSE; f++)
+        D.packImpulse[(size_t)f * T + j] = from[(size_t)f * touching + j];
+    if (take) atomicAdd(L.used, D.packCount[j]);
 }
 
 __global__ void k_scene_from_params(const float *P, SceneDev *scene) {
@@ -1481,11 +1458,35 @@ void launch_point_targets(const BoxDev &D, const Colouring &C, const Learned &L,
     check_launch("point targets");
 }
 
-void launch_apply_prediction(const BoxDev &D, const Colouring &C, const Learned &L, int touchingPairs,
-                             int32_t *fellBack, hipStream_t s) {
-    if (touchingPairs <= 0) return;
-    k_apply_prediction<<<blocks_for(touchingPairs), THREADS, 0, s>>>(D, C, L, fellBack);
-    check_launch("apply prediction");
+void launch_pack_save(const BoxDev &D, float *to, int touching, hipStream_t s) {
+    if (touching <= 0) return;
+    k_pack_copy<<<blocks_for(touching), THREADS, 0, s>>>(D, to, touching, 0);
+    check_launch("pack save");
+}
+
+void launch_pack_restore(const BoxDev &D, const float *from, int touching, hipStream_t s) {
+    if (touching <= 0) return;
+    k_pack_copy<<<blocks_for(touching), THREADS, 0, s>>>(D, const_cast<float *>(from), touching, 1);
+    check_launch("pack restore");
+}
+
+void launch_pack_prediction(const BoxDev &D, const Colouring &C, const Learned &L, int touching, hipStream_t s) {
+    if (touching <= 0) return;
+    k_pack_prediction<<<blocks_for(touching), THREADS, 0, s>>>(D, C, L, touching);
+    check_launch("pack prediction");
+}
+
+void launch_trial_residual(const BoxDev &D, int g, uint32_t *out, hipStream_t s) {
+    if (D.scenes <= 0) return;
+    k_trial_residual<<<blocks_for(D.scenes), THREADS, 0, s>>>(D, g, out);
+    check_launch("trial residual");
+}
+
+void launch_pack_choose(const BoxDev &D, const Learned &L, int touching, int32_t *fellBack, hipStream_t s) {
+    if (touching <= 0 || D.scenes <= 0) return;
+    k_trial_choose<<<blocks_for(D.scenes), THREADS, 0, s>>>(D, L, fellBack);
+    k_pack_choose<<<blocks_for(touching), THREADS, 0, s>>>(D, L, touching);
+    check_launch("pack choose");
 }
 
 void launch_scene_from_params(const BoxDev &D, SceneDev *scene, hipStream_t s) {

@@ -186,6 +186,7 @@ struct BoxWorld {
         D.sw.sceneLast = alloc<unsigned long long>((int64_t)scenes.size());
         D.sw.phaseSweeps = alloc<int32_t>(2 * SOLVE_MOST_SUBSTEPS);
         D.sw.phaseResidual = alloc<uint32_t>(2 * SOLVE_MOST_SUBSTEPS);
+        D.sw.phaseResidualSum = alloc<float>(2 * SOLVE_MOST_SUBSTEPS);
         D.sw.curve = alloc<uint32_t>(2 * SOLVE_MOST_SUBSTEPS * SOLVE_MOST_SWEEPS);
         fellBack = alloc<int32_t>(1);
         predicted.assign((size_t)d.substeps * 2, d.sweeps);
@@ -336,6 +337,7 @@ struct BoxWorld {
     void measure_clear() {
         vr::launch_fill_i32(D.sw.phaseSweeps, 0, 2 * SOLVE_MOST_SUBSTEPS, stream);
         vr::launch_fill_i32(reinterpret_cast<int32_t *>(D.sw.phaseResidual), 0, 2 * SOLVE_MOST_SUBSTEPS, stream);
+        vr::launch_fill(D.sw.phaseResidualSum, 0.f, 2 * SOLVE_MOST_SUBSTEPS, stream);
         vr::launch_fill_i32(reinterpret_cast<int32_t *>(D.sw.curve), 0, 2 * SOLVE_MOST_SUBSTEPS * SOLVE_MOST_SWEEPS,
                             stream);
     }
@@ -352,10 +354,14 @@ struct BoxWorld {
         for (int k = 0; k < 2 * VR_SOLVE_SUBSTEPS; k++) {
             r.scene_sweeps[k / VR_SOLVE_SUBSTEPS][k % VR_SOLVE_SUBSTEPS] = 0.f;
             r.residual[k / VR_SOLVE_SUBSTEPS][k % VR_SOLVE_SUBSTEPS] = 0.f;
+            r.residual_mean[k / VR_SOLVE_SUBSTEPS][k % VR_SOLVE_SUBSTEPS] = 0.f;
         }
         if (!measure) return;
         std::vector<int32_t> sweeps(2 * SOLVE_MOST_SUBSTEPS);
         std::vector<uint32_t> res(2 * SOLVE_MOST_SUBSTEPS);
+        std::vector<float> sums(2 * SOLVE_MOST_SUBSTEPS);
+        HIP_OK(hipMemcpyAsync(sums.data(), D.sw.phaseResidualSum, sizeof(float) * sums.size(), hipMemcpyDeviceToHost,
+                              stream));
         HIP_OK(hipMemcpyAsync(sweeps.data(), D.sw.phaseSweeps, sizeof(int32_t) * sweeps.size(), hipMemcpyDeviceToHost,
                               stream));
         HIP_OK(hipMemcpyAsync(res.data(), D.sw.phaseResidual, sizeof(uint32_t) * res.size(), hipMemcpyDeviceToHost,
@@ -368,6 +374,7 @@ struct BoxWorld {
                 float v;
                 std::memcpy(&v, &bitsOf, sizeof v);
                 r.residual[k][sub] = v;
+                r.residual_mean[k][sub] = sums[2 * sub + k] / (float)std::max<size_t>(scenes.size(), 1);
             }
     }
 
@@ -459,6 +466,13 @@ struct BoxWorld {
         L.y = alloc<float>(cap * LEARN_OUTPUTS);
         L.t = alloc<float>(cap * LEARN_OUTPUTS);
         L.used = alloc<int32_t>(1);
+        int64_t packed = (int64_t)MOST_POINTS * POINT_IMPULSE * std::max<int64_t>(D.slots, 1);
+        L.startCarry = alloc<float>(packed);
+        L.startPredicted = alloc<float>(packed);
+        L.motionBefore = alloc<float4>((int64_t)MOTION_VECTORS * n);
+        L.fromCarry = alloc<uint32_t>((int64_t)scenes.size());
+        L.fromPredicted = alloc<uint32_t>((int64_t)scenes.size());
+        L.takes = alloc<int32_t>((int64_t)scenes.size());
         HIP_OK(hipMemsetAsync(L.used, 0, sizeof(int32_t), stream));
         HIP_OK(hipStreamSynchronize(stream));
         pointCap = cap;
@@ -638,6 +652,39 @@ struct BoxWorld {
         return launched;
     }
 
+    // The first solve sweep of the pairs alone (the warm sweep's rounds, then the solve sweep's, no walls), measured
+    // as sweep g: what a start leaves each scene's pairs to do.
+    void trial_sweep(int rounds, int g) {
+        for (int r = 0; r < rounds; r++) launch_solve_round(D, WARM, false, roundBase[r], roundPairs[r], stream);
+        SweepArgs a{g, 1, -1.f, -1};
+        for (int r = 0; r < rounds; r++)
+            launch_solve_round(D, SOLVE, false, roundBase[r], roundPairs[r], stream, SWEEP_MEASURE, a);
+    }
+
+    // The learned start, weighed against the carry by what each leaves the first solve sweep (no counterpart in the
+    // reference, whose start is the carry, BoxPairs.java:191-267). From the first substep's motion records, the
+    // sweep is tried once from the carried impulses and once from the prediction; each scene keeps the start whose
+    // trial left its largest residual smaller (the carry on a tie), and the motion records and impulses are put back
+    // as they were before the trial, with that start. Scenes share no pair, so a scene's choice rests on its own
+    // pairs alone.
+    void learned_trial(int rounds, int touching) {
+        size_t motionBytes = sizeof(float4) * MOTION_VECTORS * (size_t)n;
+        HIP_OK(hipMemcpyAsync(L.motionBefore, D.motion, motionBytes, hipMemcpyDeviceToDevice, stream));
+        launch_pack_save(D, L.startCarry, touching, stream);
+        int fromCarry = ++sweepNo;
+        trial_sweep(rounds, fromCarry);
+        launch_trial_residual(D, fromCarry, L.fromCarry, stream);
+        HIP_OK(hipMemcpyAsync(D.motion, L.motionBefore, motionBytes, hipMemcpyDeviceToDevice, stream));
+        launch_pack_restore(D, L.startCarry, touching, stream);
+        launch_pack_prediction(D, C, L, touching, stream);
+        launch_pack_save(D, L.startPredicted, touching, stream);
+        int fromPrediction = ++sweepNo;
+        trial_sweep(rounds, fromPrediction);
+        launch_trial_residual(D, fromPrediction, L.fromPredicted, stream);
+        HIP_OK(hipMemcpyAsync(D.motion, L.motionBefore, motionBytes, hipMemcpyDeviceToDevice, stream));
+        launch_pack_choose(D, L, touching, fellBack, stream);
+    }
+
     // The counts BoxStepper.step reports for the step's collide (BoxStepper.java:40, 62-90).
     struct Prepared {
         int used = 0, touching = 0, points = 0, unplaced = 0, unlisted = 0;
@@ -665,10 +712,7 @@ struct BoxWorld {
             colour(&p.used, &p.touching, &p.points, &p.unplaced, &p.unlisted);
             if ((net || logging) && p.touching > 0) {
                 launch_point_features(D, C, L, p.touching, stream);
-                if (net) {
-                    net->forward(L.x, p.points, L.y, stream);
-                    launch_apply_prediction(D, C, L, p.touching, fellBack, stream);
-                }
+                if (net) net->forward(L.x, p.points, L.y, stream);
             }
         }
         return p;
@@ -708,7 +752,10 @@ struct BoxWorld {
         if (sweep_mode() & SWEEP_MEASURE) measure_clear();
         std::fill(lastSolve.launched[0], lastSolve.launched[0] + 2 * VR_SOLVE_SUBSTEPS, 0);
         bool logged = logging && grid && touching > 0;
-        if (sceneStep && !logged) {  // a logged step takes its targets between phases: pass by pass
+        bool trial = net && grid && touching > 0 && used > 0;
+        // A logged step takes its targets between phases, and a learned start is tried in the first substep: pass
+        // by pass.
+        if (sceneStep && !logged && !trial) {
             launch_scene_step(D, sceneFirst, C.starts, used, d.substeps, d.sweeps, d.symmetric != 0, stream,
                               sweep_mode(), tol);
         } else {
@@ -720,6 +767,7 @@ struct BoxWorld {
                 // The frame records are made whole at the first substep; later only positions change them.
                 launch_velocities(D, sub > 0, stream);
                 if (sub == 0) launch_frame_in(D, true, stream);
+                if (sub == 0 && trial) learned_trial(used, touching);
                 sweep(WARM, false, used);
                 int solved = phase(SOLVE, sub, used);
                 launch_motion_out(D, stream);
