@@ -60,12 +60,25 @@ struct BoxWorld {
     // one workgroup, which beats launches over the whole chip for batches of small scenes (2,048 scenes of 16-64
     // bodies: 19 against 32 ms a step) but not once a scene holds thousands (16-4,096 bodies: 207 against 97).
     static constexpr int SCENE_STEP_MOST = 256;
-    // A batch's step mode, tuned: both modes leave every scene as the reference does, so after each start the batch
-    // steps once pass by pass and once a workgroup a scene, each timed by the step's own events, and keeps the faster
-    // (tuneStep counts the steps since the start, up to 2). VR_BOX_STEP_MODE=scene or pass fixes the mode instead.
+    // How a step's substeps are launched. Every plan leaves every scene as the reference does; they differ in cost:
+    //   scene:  a workgroup a scene through all substeps (k_scene_step);
+    //   pass:   kernel by kernel over the whole batch, and with sweeps stopped on a residual the host looks at the
+    //           device's last open sweep between batches of sweeps (one wait a phase at least);
+    //   all:    the same launches, but every phase launches all of its sweeps and never waits (a stopped scene
+    //           passes over the rest);
+    //   graph:  `all`, captured as a HIP graph each step and replayed, the executable graph updated in place when
+    //           the step's launches have the shape of the last.
+    // A batch tunes it after each start and each change of tolerance: each candidate steps twice in turn, timed by
+    // the step's own events, and the one with the smaller sum is kept. VR_BOX_PLAN=scene|pass|all|graph (or the
+    // older VR_BOX_STEP_MODE=scene|pass) fixes it instead.
+    enum Plan { PLAN_PASS, PLAN_ALL, PLAN_GRAPH, PLAN_SCENE };
+    Plan plan = PLAN_PASS;
     bool tunable = false;
+    std::vector<Plan> candidates;
+    std::vector<float> candidateMs;
     int tuneStep = 0;
-    float passMs = 0, sceneMs = 0;
+    hipGraphExec_t graphExec = nullptr;
+    int graphUpdates = 0, graphInstantiations = 0;
     Colouring C{};
     // The learned warm start (no counterpart in the reference): per-point data, a log of (features, targets), and
     // the network whose prediction starts each step's impulses, when one is set.
@@ -93,7 +106,7 @@ struct BoxWorld {
     std::vector<char> sweepReversed;
     int sweepsTimed = 0;
     int roundsThisStep = 0;
-    int32_t *fellBack = nullptr;    // device: pairs whose prediction raised their start residual, left to the carry
+    int32_t *fellBack = nullptr;    // device: scenes with pairs that kept the carried start (learned_trial)
 
     template <class T>
     T *alloc(int64_t count) {
@@ -148,10 +161,17 @@ struct BoxWorld {
             most = std::max(most, sceneBodies[k]);
         }
         sceneStep = !wallsFromParams && sceneBodies.size() >= SCENE_STEP_LEAST && most <= SCENE_STEP_MOST;
+        plan = sceneStep ? PLAN_SCENE : PLAN_PASS;
         tunable = !wallsFromParams && grid && sceneBodies.size() > 1;
-        if (const char *mode = std::getenv("VR_BOX_STEP_MODE")) {
-            if (std::string(mode) == "scene" && !wallsFromParams) sceneStep = true;
-            if (std::string(mode) == "pass") sceneStep = false;
+        const char *fixed = std::getenv("VR_BOX_PLAN");
+        if (!fixed) fixed = std::getenv("VR_BOX_STEP_MODE");
+        if (fixed) {
+            std::string m(fixed);
+            if (m == "scene" && !wallsFromParams) plan = PLAN_SCENE;
+            if (m == "pass") plan = PLAN_PASS;
+            if (m == "all") plan = PLAN_ALL;
+            if (m == "graph" && grid) plan = PLAN_GRAPH;
+            sceneStep = plan == PLAN_SCENE;
             tunable = false;
         }
         sceneFirst = alloc<int32_t>((int64_t)first.size());
@@ -290,7 +310,7 @@ struct BoxWorld {
     }
 
     void clear() {  // PassRunner.clear(), then BoxStep.constants(): qw = 1 (BoxStep.java:103-105)
-        tuneStep = 0;  // a new start: the step mode is timed again
+        tuneStep = 0;  // a new start: the launch plan is timed again
         sweep_control_clear();
         for (auto &kv : named) HIP_OK(hipMemsetAsync(kv.second.p, 0, (size_t)kv.second.count * 4, stream));
         if (grid) {
@@ -325,6 +345,7 @@ struct BoxWorld {
     // reference's fixed count); with `m`, each step's phases are measured.
     void tolerance(float t, bool m) {
         if (t != t) throw std::invalid_argument("a tolerance is a number (below zero for none)");
+        if (t != tol) tuneStep = 0;  // the plans' costs change with the stop: timed again
         tol = t;
         measure = m;
         HIP_OK(hipStreamSynchronize(stream));
@@ -332,6 +353,64 @@ struct BoxWorld {
     }
 
     int sweep_mode() const { return (tol >= 0 ? SWEEP_STOP : 0) | (measure ? SWEEP_MEASURE : 0); }
+
+    // Whether the host looks between a phase's sweeps (plan `pass`, and any step a plan without looks cannot run).
+    bool looks() const { return plan == PLAN_PASS || plan == PLAN_SCENE; }
+
+    // The plan for the next step: the fixed one, or while tuning the next candidate in turn, then the fastest.
+    void choose_plan() {
+        if (!tunable) return;
+        if (tuneStep == 0) {
+            candidates.clear();
+            candidates.push_back(PLAN_PASS);
+            if (tol >= 0) candidates.push_back(PLAN_ALL);
+            candidates.push_back(PLAN_GRAPH);
+            candidates.push_back(PLAN_SCENE);
+            candidateMs.assign(candidates.size(), 0.f);
+        }
+        int rounds = 2, trials = rounds * (int)candidates.size();
+        Plan next;
+        if (tuneStep < trials) {
+            next = candidates[tuneStep % candidates.size()];
+        } else {
+            size_t best = 0;
+            for (size_t k = 1; k < candidates.size(); k++)
+                if (candidateMs[k] < candidateMs[best]) best = k;
+            next = candidates[best];
+        }
+        plan = next;
+        scene_step(next == PLAN_SCENE);
+    }
+
+    void plan_timed(float ms) {
+        if (!tunable) return;
+        int trials = 2 * (int)candidates.size();
+        if (tuneStep < trials) candidateMs[tuneStep % candidates.size()] += ms;
+        if (tuneStep <= trials) tuneStep++;
+    }
+
+    // The step's captured graph launched: the executable graph of the last step updated to it when it has the same
+    // shape (the same kernels in the same order, only their arguments and grids changed), else made anew.
+    void launch_graph(hipGraph_t graph) {
+        if (graphExec) {
+            hipGraphNode_t failed = nullptr;
+            hipGraphExecUpdateResult result = hipGraphExecUpdateError;
+            if (hipGraphExecUpdate(graphExec, graph, &failed, &result) == hipSuccess &&
+                result == hipGraphExecUpdateSuccess) {
+                graphUpdates++;
+            } else {
+                (void)hipGetLastError();
+                HIP_OK(hipGraphExecDestroy(graphExec));
+                graphExec = nullptr;
+            }
+        }
+        if (!graphExec) {
+            HIP_OK(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
+            graphInstantiations++;
+        }
+        HIP_OK(hipGraphLaunch(graphExec, stream));
+        HIP_OK(hipGraphDestroy(graph));
+    }
 
     // A measured step's phase words start from zero.
     void measure_clear() {
@@ -630,7 +709,7 @@ struct BoxWorld {
             sweep(kind, d.symmetric && k % 2 == 1, rounds, runMode, a);
         };
         int launched = 0;
-        if (!(mode & SWEEP_STOP)) {
+        if (!(mode & SWEEP_STOP) || !looks()) {
             for (; launched < d.sweeps; launched++) one(launched);
         } else {
             int want = std::min(std::max(predicted[at], 1), d.sweeps);
@@ -744,7 +823,7 @@ struct BoxWorld {
     }
 
     void step(vr_step_report *rep) {
-        if (tunable && tuneStep < 3) scene_step(tuneStep == 0 ? false : tuneStep == 1 ? true : sceneMs < passMs);
+        choose_plan();
         auto w0 = std::chrono::steady_clock::now();
         HIP_OK(hipEventRecord(t0, stream));
         Prepared p = prepare();
@@ -759,6 +838,10 @@ struct BoxWorld {
             launch_scene_step(D, sceneFirst, C.starts, used, d.substeps, d.sweeps, d.symmetric != 0, stream,
                               sweep_mode(), tol);
         } else {
+            // A step with no host look and no host read can be captured: graph plan, nothing logged, tried, measured
+            // or timed sweep by sweep.
+            bool captured = plan == PLAN_GRAPH && !logged && !trial && !measure && !profiling;
+            if (captured) HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             if (used > 0) {  // the sweeps' rounds read and write the touching pairs packed in round order
                 pack_room(touching);
                 launch_pack(D, C, touching, stream);
@@ -786,6 +869,11 @@ struct BoxWorld {
                 }
             }
             if (used > 0) launch_unpack(D, touching, stream);
+            if (captured) {
+                hipGraph_t graph = nullptr;
+                HIP_OK(hipStreamEndCapture(stream, &graph));
+                launch_graph(graph);
+            }
         }
         if (logged) {
             int64_t take = std::min<int64_t>(points, logCap - logCount);
@@ -802,11 +890,7 @@ struct BoxWorld {
         HIP_OK(hipStreamSynchronize(stream));
         float ms = 0;
         HIP_OK(hipEventElapsedTime(&ms, t0, t1));
-        if (tunable && tuneStep < 3) {
-            if (tuneStep == 0) passMs = ms;
-            if (tuneStep == 1) sceneMs = ms;
-            tuneStep++;
-        }
+        plan_timed(ms);
         rep->reserved = 0;
         lastSolve.fell_back = 0;
         if (net && grid && touching > 0) {
@@ -854,6 +938,7 @@ struct BoxWorld {
             if (p) (void)hipFree(p);
         for (void *p : allocs) (void)hipFree(p);
         for (hipEvent_t e : sweepEvents) (void)hipEventDestroy(e);
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
         if (t0) (void)hipEventDestroy(t0);
         if (t1) (void)hipEventDestroy(t1);
         if (stream) (void)hipStreamDestroy(stream);
