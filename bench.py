@@ -6,7 +6,8 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--run R] [--cpu-seconds S] 
 The workload (batched_2k) is the batched configuration: 2,048 independent scenes of log-uniform 16-4,096 bodies
 (1,549,388 in all, half boxes and half balls, mu 0.6), stepped together at dt 1/60 s in 8 substeps of at most 10 solve
 and 10 relax sweeps, over the configuration's R = 300 steps. The timed window is the run's last K steps, after
-R - K - W untimed steps and W warm-up steps: the piles as the run leaves them, not the free fall of its first steps.
+R - K - W - 10 untimed steps, 10 in which the batch times its launch plans afresh for the method (vr_box_retune) and
+W warm-up steps: the piles as the run leaves them, not the free fall of its first steps.
 All state is resident in HBM before the timed region; nothing crosses PCIe inside it but each step's small report.
 
 Three runs reach the same state at the window (the stepping is deterministic) and differ only inside it:
@@ -24,7 +25,8 @@ over the scenes), and the deepest manifold point.
 
 value = learned body-substeps per second = bodies x substeps x K / seconds. The line carries:
   plain, stopped, learned: each run's rate, residuals, deepest point, sweeps;
-  roofline: k_solve_round (the colour rounds of a sweep, back to back between HIP events on the world's stream) in
+  roofline: the colour rounds of a sweep (k_group_rounds, one launch a sweep, on the group plan the tuner picks for
+    the settled batch; k_solve_round, a launch a round, on the others), between HIP events on the world's stream, in
     the plain window: algorithmic bytes P (236 + 88 k-bar) a sweep over its time (P touching pairs, k-bar points a
     pair), against MI355X's 8 TB/s; roofline_walls the same for k_walls (96 N + 32 W a sweep);
   cpu_baseline: the oracle (oracle/, a C++ restatement of the reference's passes) on the first scenes of the batch from
@@ -53,6 +55,7 @@ STATE = ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw")
 # copy of known size), over the same windows: profiles/traffic_box_kernels_batched_2k.md and
 # profiles/traffic_round_kernels_pile_64k.md. None where not measured.
 SOLVE_ROUND_TRAFFIC_BYTES = None
+GROUP_ROUNDS_TRAFFIC_BYTES = None
 WALLS_TRAFFIC_BYTES = None
 SPHERE_ROUND_TRAFFIC_BYTES = None
 
@@ -146,12 +149,16 @@ def measured_step(sim, tol, walls_too=True):
     }
 
 
+TUNE_STEPS = 10  # untimed steps in which a batch times its launch plans (two each) after vr_box_retune
+
+
 def batch_run(pkg, label, spec, args, net=None, tol=-1.0, profile=False, keep_state=False):
-    """Steps the batch R - K - W steps plain, then W warm-up and K timed steps with `net` and `tol`, then one measured
-    step."""
+    """Steps the batch R - K - W - 10 steps plain, then with `net` and `tol` 10 steps in which it times its launch
+    plans afresh (vr_box_retune: the plans' costs in the settled piles are not those of the free fall the batch timed
+    them in), W warm-up and K timed steps, then one measured step."""
     _, boxes, _ = modules(pkg)
     sim = batch_of(boxes, spec)
-    settle = args.run - args.steps - args.warmup
+    settle = args.run - args.steps - args.warmup - TUNE_STEPS
     for k in range(settle):
         sim.stepAndWait()
         progress(f"{label} settle", k + 1, settle)
@@ -159,6 +166,12 @@ def batch_run(pkg, label, spec, args, net=None, tol=-1.0, profile=False, keep_st
     if net is not None:
         sim.learned(net)
     sim.tolerance(tol, False)
+    sim.retune()
+    for _ in range(TUNE_STEPS):
+        sim.stepAndWait()
+    plan, tuning = sim.plan()
+    if tuning:
+        raise RuntimeError(f"{label}: the batch was still timing its launch plans after {TUNE_STEPS} steps")
     for _ in range(args.warmup):
         sim.stepAndWait()
     if profile:
@@ -181,6 +194,7 @@ def batch_run(pkg, label, spec, args, net=None, tol=-1.0, profile=False, keep_st
         "value": n * spec.substeps * args.steps / seconds, "unit": "body-substeps/s",
         "ms_per_step": seconds * 1e3 / args.steps,
         "tolerance_m_per_s": tol if tol >= 0 else None,
+        "launch_plan": plan,
         "touching_at_start": reps[0].touching, "touching_at_end": reps[-1].touching,
         "rounds_at_end": reps[-1].rounds, "points_at_end": reps[-1].points,
         "unplaced": int(sum(r.unplaced for r in reps)), "unlisted": int(sum(r.unlisted for r in reps)),
@@ -196,27 +210,33 @@ def batch_run(pkg, label, spec, args, net=None, tol=-1.0, profile=False, keep_st
 
 
 def box_roofline(prof, walls_per_sweep):
-    """k_solve_round and k_walls over the timed sweeps: algorithmic bytes over their event-timed durations."""
+    """The colour rounds (k_solve_round, or k_group_rounds on the group plan) and k_walls over the timed sweeps:
+    algorithmic bytes over their event-timed durations."""
+    grouped = prof.sweep_launches > 0
     round_bytes = 236.0 * prof.listed_sum + 88.0 * prof.points_sum
-    round_s = prof.round_ns * 1e-9
+    round_ns = prof.sweep_ns if grouped else prof.round_ns
+    launches = prof.sweep_launches if grouped else prof.round_launches
+    round_s = round_ns * 1e-9
     achieved = round_bytes / round_s / 1e9 if round_s > 0 else 0.0
     walls_bytes = 96.0 * prof.bodies_sum + 32.0 * walls_per_sweep * prof.passes
     walls_s = prof.walls_ns * 1e-9
     walls_achieved = walls_bytes / walls_s / 1e9 if walls_s > 0 else 0.0
     kbar = prof.points_sum / max(prof.listed_sum, 1)
+    kernel = ("k_group_rounds (a solve or relax sweep's colour rounds in one launch, a workgroup a scene group)"
+              if grouped else "k_solve_round (solve and relax sweeps' colour rounds, a launch a round)")
     rounds = {
-        "bound": "latency",
-        "kernel": "k_solve_round (solve and relax sweeps' colour rounds), plain window",
+        "bound": "hbm",
+        "kernel": kernel + ", plain window",
         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-        "traffic": SOLVE_ROUND_TRAFFIC_BYTES,
-        "bytes_per_launch": round_bytes / max(prof.round_launches, 1),
-        "avg_launch_us": prof.round_ns / 1e3 / max(prof.round_launches, 1),
-        "launches": int(prof.round_launches), "sweeps": int(prof.passes),
+        "traffic": (GROUP_ROUNDS_TRAFFIC_BYTES if grouped else SOLVE_ROUND_TRAFFIC_BYTES),
+        "bytes_per_launch": round_bytes / max(launches, 1),
+        "avg_launch_us": round_ns / 1e3 / max(launches, 1),
+        "launches": int(launches), "sweeps": int(prof.passes),
         "touching_pairs_per_sweep": prof.listed_sum / max(prof.passes, 1), "points_per_pair": kbar,
         "formula": "P (236 + 88 k-bar) B a sweep, P touching pairs, k-bar points a pair",
     }
     walls = {
-        "bound": "latency",
+        "bound": "hbm",
         "kernel": "k_walls (solve and relax sweeps), plain window",
         "achieved": walls_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": walls_achieved / HBM_PEAK_GBS,
         "traffic": WALLS_TRAFFIC_BYTES,
@@ -272,6 +292,88 @@ def cpu_baseline(pkg, spec, state, seconds, scenes_taken=48):
                   f"the plain window's start (impulses from zero): {most[1]} steps in {most[2]:.1f} s on "
                   f"{max(out)} threads, {one[1]} steps in {one[2]:.1f} s on one",
     }
+
+
+def plain_only(pkg, args):
+    """The plain run alone, for the profilers: its rate and roofline inputs, one JSON line."""
+    scenes, _, _ = modules(pkg)
+    spec = scenes.batched_2k()
+    plain, prof, _ = batch_run(pkg, "plain", spec, args, profile=True)
+    rounds, walls = box_roofline(prof, plain["after_window"]["active_wall_contacts"])
+    return {"workload": f"batched_2k plain, the last {args.steps} of {args.run} steps", "plain": plain,
+            "roofline": rounds, "roofline_walls": walls}
+
+
+# ---- the profilers' output over the timed window ----
+
+WINDOW_KERNELS = {
+    "k_solve_round": ("k_solve_round<1, false, 0>", "k_solve_round<2, false, 0>", "k_solve_round<1, true, 0>",
+                      "k_solve_round<2, true, 0>"),
+    "k_group_rounds": ("k_group_rounds<1, false, 0>", "k_group_rounds<2, false, 0>", "k_group_rounds<1, true, 0>",
+                       "k_group_rounds<2, true, 0>"),
+    "k_walls": ("k_walls<1, false, 0>", "k_walls<2, false, 0>", "k_walls<1, true, 0>", "k_walls<2, true, 0>"),
+}
+
+
+def window_kind(name):
+    for kind, names in WINDOW_KERNELS.items():
+        if any(n in name for n in names):
+            return kind
+    return None
+
+
+def summarize(trace, counters, line, out):
+    """Markdown of a `plain_only` run's timed window from rocprofv3's output: the solve and relax sweeps' round and
+    walls dispatches of the plain method (their template's last argument 0) are the window's last `launches` of each
+    kind (the measured steps after it run other instantiations). From a kernel trace, their mean duration; from a
+    counter pass (FETCH_SIZE or WRITE_SIZE, KiB a dispatch), their mean reading."""
+    import csv
+    d = json.loads(line)
+    rounds_kind = "k_group_rounds" if d["roofline"]["kernel"].startswith("k_group_rounds") else "k_solve_round"
+    want = {rounds_kind: d["roofline"]["launches"], "k_walls": d["roofline_walls"]["launches"]}
+    rows = {k: [] for k in want}
+    if trace:
+        for r in csv.DictReader(open(trace)):
+            kind = window_kind(r["Kernel_Name"])
+            if kind:
+                rows[kind].append((int(r["Dispatch_Id"]), int(r["End_Timestamp"]) - int(r["Start_Timestamp"]),
+                                   int(r["Grid_Size_X"]), r["VGPR_Count"], r["SGPR_Count"]))
+    if counters:
+        for r in csv.DictReader(open(counters)):
+            kind = window_kind(r["Kernel_Name"])
+            if kind:
+                rows[kind].append((int(r["Dispatch_Id"]), float(r["Counter_Value"]), int(r["Grid_Size"]),
+                                   r["VGPR_Count"], r["SGPR_Count"]))
+    lines = ["| Kernel | Dispatches | Mean | Of | VGPRs, SGPRs |", "| --- | --- | --- | --- | --- |"]
+    buckets = []
+    for kind, n in want.items():
+        got = sorted(rows[kind])[-n:] if n > 0 else []
+        if not got:
+            lines.append(f"| `{kind}` | 0 | - | the window's dispatches not found | - |")
+            continue
+        mean = sum(g[1] for g in got) / len(got)
+        what = "duration, us" if trace else "counter reading, KiB a dispatch"
+        value = mean / 1e3 if trace else mean
+        regs = sorted({f"{g[3]}, {g[4]}" for g in got})
+        lines.append(f"| `{kind}` (plain solve and relax sweeps) | {len(got)} | {value:.3f} | {what} | "
+                     f"{'; '.join(regs)} |")
+        if kind == "k_solve_round":  # launch by launch, each a round of its own size
+            by = {}
+            for g in got:
+                by.setdefault(max(g[2], 1).bit_length() - 1, []).append(g)
+            total = sum(g[1] for g in got)
+            buckets.append("")
+            buckets.append(f"`k_solve_round` by launch size (lanes, two a pair), mean {what}:")
+            buckets.append("")
+            buckets.append("| Lanes | Dispatches | Mean lanes | Mean | Share of the kernel's total |")
+            buckets.append("| --- | --- | --- | --- | --- |")
+            for b in sorted(by):
+                v = by[b]
+                m = sum(g[1] for g in v) / len(v)
+                buckets.append(f"| 2^{b} .. 2^{b + 1} | {len(v)} | {sum(g[2] for g in v) / len(v):.0f} | "
+                               f"{m / 1e3 if trace else m:.3f} | {sum(g[1] for g in v) / total:.3f} |")
+    with open(out, "w") as f:
+        f.write("\n".join(lines + buckets) + "\n")
 
 
 def batched(pkg, args):
@@ -426,6 +528,11 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--tolerance", type=float, default=0.01, help="m/s: the residual each scene's sweeps stop at")
     ap.add_argument("--c1-steps", type=int, default=20, help="timed steps of the sphere pile (0: skip it)")
+    ap.add_argument("--only", default="", help="plain: the plain run alone (for the profilers), one JSON line")
+    ap.add_argument("--summarize", nargs=3, metavar=("TRACE_OR_-", "COUNTERS_OR_-", "LINE_JSON"),
+                    help="write the timed window's kernels from rocprofv3 CSVs (kernel trace or counter collection) "
+                         "and a --only plain line to --summary-out, then exit")
+    ap.add_argument("--summary-out", default="window.md")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB: only copy a buffer of that size 5 times (the known-size kernel that calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -443,8 +550,16 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": src.numel() * 4, "copy_probe_bytes_written": dst.numel() * 4}))
         return
+    if args.summarize:
+        trace, counters, line = args.summarize
+        summarize(None if trace == "-" else trace, None if counters == "-" else counters,
+                  open(line).read().strip().splitlines()[-1], args.summary_out)
+        return
     args.run = max(args.run, args.steps + args.warmup)
     pkg = load_package()
+    if args.only == "plain":
+        print(json.dumps(plain_only(pkg, args)))
+        return
     if args.workload == "mixed_32k":
         print(json.dumps(box_scene(pkg, args)))
         return

@@ -139,12 +139,16 @@ typedef struct vr_profile {
      * Box worlds: events bracket the colour rounds (k_solve_round, back to back) and the walls sweep (k_walls) of
      * every solve and relax sweep. round_ns / round_launches are the rounds', walls_ns / walls_launches the walls';
      * passes counts those sweeps; listed_sum and points_sum add up the touching pairs and their manifold points the
-     * sweeps took (P and P k-bar of the algorithmic bytes), bodies_sum the bodies.
+     * sweeps took (P and P k-bar of the algorithmic bytes), bodies_sum the bodies. A batch whose sweeps' rounds
+     * run a workgroup a scene group (k_group_rounds: all of a sweep's rounds in one launch) counts them in sweep_ns
+     * and sweep_launches (one a sweep) in place of round_ns and round_launches; its walls as above.
      */
     int64_t walls_ns;
     int64_t walls_launches;
     int64_t points_sum;
     int64_t bodies_sum;
+    int64_t sweep_ns;
+    int64_t sweep_launches;
 } vr_profile;
 
 int vr_world_create(int32_t kind, const vr_step_desc *desc, vr_world **out);
@@ -292,6 +296,27 @@ int vr_box_log(vr_world *w, int32_t on, int64_t capacity);
  * result. A NaN tolerance is VR_ERR_ARGUMENT.
  */
 int vr_box_tolerance(vr_world *w, float tolerance, int32_t measure);
+/*
+ * The launch plan of a batch's steps (no counterpart in the reference, whose BoxStepper launches a step one way). Every
+ * plan leaves every scene as the reference does; they differ in time only:
+ *   VR_PLAN_PASS   kernel by kernel over the batch, a launch for each colour round;
+ *   VR_PLAN_ALL    the same, every sweep of a stopped phase launched with no look at the residuals between them;
+ *   VR_PLAN_GRAPH  VR_PLAN_ALL captured as a HIP graph each step and replayed;
+ *   VR_PLAN_SCENE  a workgroup a scene through every substep, one launch a step;
+ *   VR_PLAN_GROUP  VR_PLAN_PASS, with each sweep's colour rounds in one launch, a workgroup a group of consecutive
+ *                  scenes.
+ * A batch times its plans over its first steps after a start, a change of tolerance or vr_box_retune, two steps each
+ * in turn, and keeps the fastest; VR_BOX_PLAN=pass|all|graph|scene|group fixes one instead. vr_box_plan reads the
+ * plan the next step takes and whether the batch is still timing them (*tuning = 1); a world of one scene steps
+ * VR_PLAN_PASS and never tunes.
+ */
+#define VR_PLAN_PASS 0
+#define VR_PLAN_ALL 1
+#define VR_PLAN_GRAPH 2
+#define VR_PLAN_SCENE 3
+#define VR_PLAN_GROUP 4
+int vr_box_retune(vr_world *w);
+int vr_box_plan(vr_world *w, int32_t *plan, int32_t *tuning);
 #define VR_SOLVE_SUBSTEPS 64
 /*
  * What the last step's phases did (the first VR_SOLVE_SUBSTEPS substeps; index 0 the solve sweeps, 1 the relax).

@@ -1579,6 +1579,20 @@ int vro_box_tolerance(vro_world *p, float tolerance, int32_t measure) {
     return VR_OK;
 }
 
+// The oracle steps a batch one way (scene after scene on its threads): nothing to tune, its plan the pass.
+int vro_box_retune(vro_world *p) {
+    if (p->kind != VR_BOXES) return fail(VR_ERR_STATE, "not a box world");
+    return VR_OK;
+}
+
+int vro_box_plan(vro_world *p, int32_t *plan, int32_t *tuning) {
+    if (!plan || !tuning) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (p->kind != VR_BOXES) return fail(VR_ERR_STATE, "not a box world");
+    *plan = VR_PLAN_PASS;
+    *tuning = 0;
+    return VR_OK;
+}
+
 // A batch's report gathers its scenes' own: the most any scene launched, the mean of their sweeps, the largest of
 // their residuals.
 int vro_box_solve_read(vro_world *p, vr_box_solve *out) {

@@ -28,7 +28,10 @@ ENTRY_POINTS = ("world_create", "world_clear", "world_write", "world_read", "wor
                 "world_profile_read", "box_collide", "batch_create", "mlp_create", "mlp_destroy", "mlp_read",
                 "mlp_write", "mlp_forward", "mlp_gradient", "mlp_train", "mlp_last_error", "box_log",
                 "box_samples", "box_learned", "world_diagnostics", "box_collide_list",
-                "world_diagnostics_static", "box_tolerance", "box_solve_read", "box_solve_curve")
+                "world_diagnostics_static", "box_tolerance", "box_solve_read", "box_solve_curve", "box_retune",
+                "box_plan")
+# vr_box_plan's plans.
+PLANS = ("pass", "all", "graph", "scene", "group")
 
 
 class StepDesc(ctypes.Structure):
@@ -66,7 +69,7 @@ class Profile(ctypes.Structure):
     _fields_ = [("round_ns", ctypes.c_int64), ("round_launches", ctypes.c_int64), ("passes", ctypes.c_int64),
                 ("listed_sum", ctypes.c_int64), ("steps", ctypes.c_int64), ("enqueue_ns", ctypes.c_int64),
                 ("walls_ns", ctypes.c_int64), ("walls_launches", ctypes.c_int64), ("points_sum", ctypes.c_int64),
-                ("bodies_sum", ctypes.c_int64)]
+                ("bodies_sum", ctypes.c_int64), ("sweep_ns", ctypes.c_int64), ("sweep_launches", ctypes.c_int64)]
 
 
 class Abi:
@@ -100,6 +103,8 @@ class Abi:
             "box_collide": ([p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
             "box_collide_list": ([p, ctypes.POINTER(StepReport)]),
             "box_tolerance": ([p, ctypes.c_float, ctypes.c_int32]),
+            "box_retune": ([p]),
+            "box_plan": ([p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
             "box_solve_read": ([p, ctypes.POINTER(SolveReport)]),
             "box_solve_curve": ([p, ctypes.POINTER(ctypes.c_float), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
             "mlp_create": ([ctypes.c_uint64, ctypes.POINTER(p)]),
@@ -269,6 +274,15 @@ class World:
 
     def tolerance(self, tol: float, measure: bool) -> None:
         self._check(self.abi.fn("box_tolerance")(self.handle, ctypes.c_float(tol), 1 if measure else 0))
+
+    def retune(self) -> None:
+        self._check(self.abi.fn("box_retune")(self.handle))
+
+    def plan(self):
+        """(the plan the next step takes, by name; whether the batch is still timing its plans)."""
+        plan, tuning = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.abi.fn("box_plan")(self.handle, ctypes.byref(plan), ctypes.byref(tuning)))
+        return PLANS[plan.value], bool(tuning.value)
 
     def solve_read(self) -> SolveReport:
         out = SolveReport()

@@ -251,6 +251,15 @@ class BoxSimulation:
             raise IllegalArgumentException("a tolerance is a number (below zero for none)")
         self._world.tolerance(float(tol), measure)
 
+    # The launch plan (no counterpart in the reference): vr_box_retune / vr_box_plan.
+    def retune(self) -> None:
+        """Times the batch's launch plans again over its next steps (two each) and keeps the fastest."""
+        self._world.retune()
+
+    def plan(self):
+        """(the plan the next step takes: pass, all, graph, scene or group; whether the plans are still being timed)."""
+        return self._world.plan()
+
     def solveReport(self) -> "SolveReport":
         """What the last step's phases did (vr_box_solve)."""
         return SolveReport.of(self._world.solve_read(), self.step_.substeps)

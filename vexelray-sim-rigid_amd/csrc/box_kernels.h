@@ -86,6 +86,11 @@ struct BoxDev {
     // the state by k_frame_in and the motion returned to it by k_motion_out.
     float4 *frame, *motion;
     int64_t packCap;
+    // Within each round the entries are in the order of a sort by (the pair's scene group when a step's rounds run a
+    // workgroup a group, then its point count and whether a ball takes part): a round's pairs share no body, so their
+    // order within the round changes nothing, and a wave's lanes then mostly run the same points (PACK_CLASSES).
+    int32_t *packFrom;     // the colouring's place (in C.order) whose pair the entry holds
+    const int32_t *sceneGroup;  // scenes: each scene's group (Groups), for that sort
     int32_t *packPair;     // the pair's slot
     int32_t *packCount;    // its manifold's point count
     int32_t *packBody;     // 2 fields: its first body, then its second
@@ -180,6 +185,28 @@ void launch_wall_move(const BoxDev &D, const int32_t *from, const int32_t *to, h
 // mode and tol: the sweeps stopped on the residual and measured as the pass-by-pass step stops and measures them.
 void launch_scene_step(const BoxDev &D, const int32_t *sceneFirst, const int32_t *starts, int used, int substeps,
                        int sweeps, bool symmetric, hipStream_t s, int mode = 0, float tol = -1.f);
+// The classes of a round's pairs in the packed order: (points - 1) * 2, plus 1 when a ball takes part.
+constexpr int PACK_CLASSES = 8;
+// The packed order (packFrom) for the step's colouring, its sort's starts (one a round, group and class, of
+// (MOST_ROUNDS + 1) * groups * PACK_CLASSES + 1) into `starts`; groups = 1 sorts by class alone within a round.
+void launch_pack_order(const BoxDev &D, const Colouring &C, int touching, int groups, int32_t *starts,
+                       hipStream_t s);
+
+// Scene groups for a sweep's rounds a workgroup a group (k_group_rounds): runs of consecutive scenes, each group's
+// scenes and, for the step's rounds, packed entries. Scenes share no body and no pair, so a group's rounds wait on
+// nothing outside the group: they follow one another inside one workgroup, with a barrier where the launches over
+// the whole batch have a kernel boundary.
+constexpr int GROUP_MOST_THREADS = 1024;
+struct Groups {
+    int count;
+    int32_t *ranges;  // count * MOST_ROUNDS * 2: group g's packed entries in round r, [from, to)
+};
+// Each group's packed entries in each of the step's rounds (starts: launch_pack_order's, sorted by group).
+void launch_group_ranges(const BoxDev &D, const Groups &G, const int32_t *starts, int used, hipStream_t s);
+// Rounds first .. last - 1 of a sweep (in the order of launch_solve_round's launches, so in reverse with `reverse`),
+// a workgroup of `threads` a group.
+void launch_group_rounds(const BoxDev &D, const Groups &G, int sweep, bool reverse, int first, int last, int threads,
+                         hipStream_t s, int mode = 0, const SweepArgs &a = SweepArgs{0, 1, -1.f, -1});
 // Round r's pairs, packed entries base .. base + pairs - 1.
 void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int base, int pairs, hipStream_t s, int mode = 0,
                         const SweepArgs &a = SweepArgs{0, 1, -1.f, -1});

@@ -800,7 +800,7 @@ __global__ void k_pack_pairs(BoxDev D, Colouring C, int touching) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= touching) return;
     size_t T = (size_t)D.packCap;
-    int pair = C.touch[C.order[j]];
+    int pair = C.touch[C.order[D.packFrom[j]]];
     const float *m = D.manifold + (size_t)pair * MANIFOLD_WORDS;
     const float *imp = D.impulses + (size_t)pair * IMPULSE_WORDS;
     int count = (int)m[COUNT];
@@ -956,6 +956,95 @@ __global__ void k_phase_measure(BoxDev D, int g0, int phase) {
     atomicAdd(D.sw.phaseSweeps + phase, g - g0 + 1);
     atomicMax(D.sw.phaseResidual + phase, res);
     atomicAdd(D.sw.phaseResidualSum + phase, __uint_as_float(res));
+}
+
+// ---- A sweep's rounds a workgroup a scene group ----
+//
+// A sweep's rounds launched one by one over the whole batch (k_solve_round) pay a launch and a chain of dependent
+// loads and arithmetic for every round, however few pairs it holds, and each round's loads of the bodies' records
+// land anywhere in the batch. Scenes share no body and no pair, so a scene needs no other scene's round to be done
+// before its own next one: a workgroup takes a group of consecutive scenes through all of the sweep's rounds, in
+// order (or the reverse), with a workgroup barrier where the launches have a kernel boundary. Each pair is solved by
+// the same code as in those launches (solve_pair_side), and each scene meets its rounds in the same order, so every
+// scene ends the rounds as they leave it. A group's bodies' records stay in its XCD's L2 from round to round. The
+// walls stay one launch over the batch (k_walls), before or after the rounds as in BoxStep.sweep.
+
+// The sort key of colouring place i: (round, group, class), the round the most significant, so that each round's
+// entries stay where the colouring put the round.
+__global__ void k_pack_keys(BoxDev D, Colouring C, int touching, int groups) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= touching) return;
+    int pair = C.touch[C.order[i]];
+    int r = min(max(C.colour[pair], 0), MOST_ROUNDS);
+    int count = min(max((int)D.manifold[(size_t)pair * MANIFOLD_WORDS + COUNT], 1), MOST_POINTS);
+    int a = D.pairs[2 * (size_t)pair], b = D.pairs[2 * (size_t)pair + 1];
+    bool ball = D.shape[a] > 0.5f || D.shape[b] > 0.5f;  // Moving::sphere
+    int g = groups > 1 ? D.sceneGroup[D.scene[a]] : 0;
+    C.keys[i] = (r * groups + g) * PACK_CLASSES + (count - 1) * 2 + (ball ? 1 : 0);
+}
+
+// Group g's entries in round r: its classes' run of the packed order.
+__global__ void k_group_ranges(BoxDev D, Groups G, const int32_t *starts, int used) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= G.count * MOST_ROUNDS) return;
+    int g = t / MOST_ROUNDS, r = t % MOST_ROUNDS;
+    int from = 0, to = 0;
+    if (r < used) {
+        size_t at = ((size_t)r * G.count + g) * PACK_CLASSES;
+        from = starts[at];
+        to = starts[at + PACK_CLASSES];
+    }
+    G.ranges[2 * t] = from;
+    G.ranges[2 * t + 1] = to;
+}
+
+// Round r of a group's sweep: k_solve_round's work for the group's packed entries from .. to - 1, two neighbouring
+// lanes a pair. Every thread runs the same number of iterations, so the waves' residual records (record) see whole
+// waves.
+//
+// The packed fields' stride is made opaque to the compiler at each iteration (an empty asm that may change it), so
+// the fifty-odd field addresses a pair's loads and stores use are worked out inside the iteration, as in a launch of
+// k_solve_round, rather than kept live across the loop in scalar registers, which would spill into vector ones.
+template <int SWEEP, bool REVERSE, int MODE>
+__device__ void group_round(const BoxDev &D0, int from, int to, const SweepArgs &a) {
+    int half = (int)blockDim.x >> 1;
+    bool second = (threadIdx.x & 1) != 0;
+    BoxDev D = D0;
+#pragma unroll 1
+    for (int base = from; base < to; base += half) {
+        asm volatile("" : "+s"(D.packCap));
+        int k = base + ((int)threadIdx.x >> 1);
+        if (MODE == 0) {
+            if (k < to) solve_pair_side<SWEEP, REVERSE>(D, k, second);
+            continue;
+        }
+        int scene = k < to ? D.packScene[k] : -1;
+        bool live = k < to && (!(MODE & SWEEP_STOP) || sweeping(D, a, scene));
+        float res = 0.f;
+        if (live) solve_pair_side<SWEEP, REVERSE>(D, k, second, &res);
+        record<MODE>(D, a, live && !second, scene, res);
+    }
+}
+
+// Rounds first .. last - 1 of a sweep (in reverse with REVERSE); blockIdx.x: the group (the groups are listed
+// largest first, so the longest start first).
+template <int SWEEP, bool REVERSE, int MODE>
+__global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_rounds(BoxDev D, Groups G, int first, int last,
+                                                                     SweepArgs a) {
+    __shared__ int from[MOST_ROUNDS], to[MOST_ROUNDS];
+    int g = blockIdx.x;
+    for (int r = first + (int)threadIdx.x; r < last; r += blockDim.x) {
+        from[r] = G.ranges[2 * (g * MOST_ROUNDS + r)];
+        to[r] = G.ranges[2 * (g * MOST_ROUNDS + r) + 1];
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int order = first; order < last; order++) {  // BoxStep.sweep, BoxStep.java:217-231
+        int r = REVERSE ? first + last - 1 - order : order;
+        if (from[r] == to[r]) continue;  // no pair of the group in round r: nothing to wait for
+        group_round<SWEEP, REVERSE, MODE>(D, from[r], to[r], a);
+        __syncthreads();
+    }
 }
 
 // BoxBroad.bin, BoxBroad.java:74-89, in the body's scene's grid.
@@ -1392,7 +1481,7 @@ __global__ void k_pack_prediction(BoxDev D, Colouring C, Learned L, int touching
     if (j >= touching) return;
     Bufs B = bufs(D);
     size_t T = (size_t)D.packCap;
-    int t = C.order[j];
+    int t = C.order[D.packFrom[j]];
     int p = C.touch[t];
     int count = min((int)D.manifold[(size_t)p * MANIFOLD_WORDS + COUNT], MOST_POINTS);
     for (int k = 0; k < count; k++) {
@@ -1590,6 +1679,61 @@ void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int base, int 
         default: round_then<RELAX, true>(D, blocks, base, pairs, s, mode, a); break;
     }
     check_launch("solve round");
+}
+
+void launch_pack_order(const BoxDev &D, const Colouring &C, int touching, int groups, int32_t *starts,
+                       hipStream_t s) {
+    if (touching <= 0) return;
+    if (touching > C.most || groups < 1) throw std::runtime_error("pack order: out of range");
+    k_pack_keys<<<blocks_for(touching), THREADS, 0, s>>>(D, C, touching, groups);
+    check_launch("pack keys");
+    // The colouring's keys are spent (its sort is done), so its key words and sort scratch serve this sort.
+    vr::SphereDev view{};
+    view.n = touching;
+    view.cells = (MOST_ROUNDS + 1) * groups * PACK_CLASSES;
+    view.keys = C.keys;
+    view.order = D.packFrom;
+    view.starts = starts;
+    vr::launch_sort(view, C.sort, s);
+}
+
+void launch_group_ranges(const BoxDev &D, const Groups &G, const int32_t *starts, int used, hipStream_t s) {
+    if (G.count <= 0) return;
+    if (used < 0 || used > MOST_ROUNDS) throw std::runtime_error("group ranges: rounds out of range");
+    k_group_ranges<<<blocks_for((int64_t)G.count * MOST_ROUNDS), THREADS, 0, s>>>(D, G, starts, used);
+    check_launch("group ranges");
+}
+
+template <int SWEEP, bool REVERSE>
+static void group_then(const BoxDev &D, const Groups &G, int first, int last, int threads, hipStream_t s, int mode,
+                       const SweepArgs &a) {
+    switch (SWEEP == WARM ? 0 : mode) {
+        case 0: k_group_rounds<SWEEP, REVERSE, 0><<<G.count, threads, 0, s>>>(D, G, first, last, a); break;
+        case SWEEP_STOP: k_group_rounds<SWEEP, REVERSE, SWEEP_STOP><<<G.count, threads, 0, s>>>(D, G, first, last, a); break;
+        case SWEEP_MEASURE:
+            k_group_rounds<SWEEP, REVERSE, SWEEP_MEASURE><<<G.count, threads, 0, s>>>(D, G, first, last, a);
+            break;
+        default:
+            k_group_rounds<SWEEP, REVERSE, SWEEP_STOP | SWEEP_MEASURE><<<G.count, threads, 0, s>>>(D, G, first, last, a);
+            break;
+    }
+}
+
+void launch_group_rounds(const BoxDev &D, const Groups &G, int sweep, bool reverse, int first, int last, int threads,
+                         hipStream_t s, int mode, const SweepArgs &a) {
+    if (G.count <= 0 || first >= last) return;
+    if (first < 0 || last > MOST_ROUNDS) throw std::runtime_error("group rounds: rounds out of range");
+    if (threads < 64 || threads > GROUP_MOST_THREADS || threads % 64 != 0)
+        throw std::runtime_error("group rounds: a workgroup of whole waves, at most GROUP_MOST_THREADS threads");
+    switch (sweep * 2 + (reverse ? 1 : 0)) {
+        case 0: group_then<WARM, false>(D, G, first, last, threads, s, 0, a); break;
+        case 1: group_then<WARM, true>(D, G, first, last, threads, s, 0, a); break;
+        case 2: group_then<SOLVE, false>(D, G, first, last, threads, s, mode, a); break;
+        case 3: group_then<SOLVE, true>(D, G, first, last, threads, s, mode, a); break;
+        case 4: group_then<RELAX, false>(D, G, first, last, threads, s, mode, a); break;
+        default: group_then<RELAX, true>(D, G, first, last, threads, s, mode, a); break;
+    }
+    check_launch("group rounds");
 }
 
 void launch_pack(const BoxDev &D, const Colouring &C, int touching, hipStream_t s) {

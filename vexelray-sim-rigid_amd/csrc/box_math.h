@@ -192,7 +192,9 @@ __device__ void store_moving_changed(const Bufs &B, const Moving &m, const Movin
 //
 // frame, 4 float4 a body: (at, im), q, (half, shape), (inverse, 0); fixed over a run of sweeps (positions rewrite
 // it). motion, 2 float4 a body: (vel, 0), (mom, 0); what the sweeps change. A round's bodies lie anywhere in the
-// world, so it gathers each from these records, two cache lines, not from the state's 18 fields.
+// world, so it gathers each from these records, two cache lines, not from the state's 18 fields. (One 128-byte record
+// a body, frame and motion in one line, measured slower on the 2,048-scene batch: 116 against 111 ms of colour
+// rounds a step, and 28 against 26 ms of walls sweeps, whose coalesced loads of the motion then stride.)
 constexpr int FRAME_VECTORS = 4, MOTION_VECTORS = 2;
 
 __device__ __forceinline__ Moving load_framed(const float4 *frame, const float4 *motion, int s) {

@@ -39,6 +39,9 @@ void box_step(BoxWorld *w, vr_step_report *rep);
 void box_collide_list(BoxWorld *w, vr_step_report *rep);
 // Sweeps stopped on a residual (vr_box_tolerance), and what the last step's phases did.
 void box_tolerance(BoxWorld *w, float tol, bool measure);
+// The launch plan timed again over the next steps (vr_box_retune); the plan the next step takes (vr_box_plan).
+void box_retune(BoxWorld *w);
+void box_plan(BoxWorld *w, int32_t *plan, int32_t *tuning);
 // Sweep timing (vr_world_profile on a box world): on zeroes the counters.
 void box_profile(BoxWorld *w, bool on);
 void box_profile_read(BoxWorld *w, vr_profile *out);

@@ -67,11 +67,16 @@ struct BoxWorld {
     //   all:    the same launches, but every phase launches all of its sweeps and never waits (a stopped scene
     //           passes over the rest);
     //   graph:  `all`, captured as a HIP graph each step and replayed, the executable graph updated in place when
-    //           the step's launches have the shape of the last.
+    //           the step's launches have the shape of the last;
+    //   group:  `pass`, but a sweep's rounds in one launch, a workgroup a group of consecutive scenes through all of
+    //           them (k_group_rounds), in place of a launch over the batch for each round.
     // A batch tunes it after each start and each change of tolerance: each candidate steps twice in turn, timed by
-    // the step's own events, and the one with the smaller sum is kept. VR_BOX_PLAN=scene|pass|all|graph (or the
-    // older VR_BOX_STEP_MODE=scene|pass) fixes it instead.
-    enum Plan { PLAN_PASS, PLAN_ALL, PLAN_GRAPH, PLAN_SCENE };
+    // the step's own events, and the one with the smaller sum is kept. VR_BOX_PLAN=scene|pass|all|graph|group (or
+    // the older VR_BOX_STEP_MODE=scene|pass) fixes it instead.
+    enum Plan { PLAN_PASS, PLAN_ALL, PLAN_GRAPH, PLAN_SCENE, PLAN_GROUP };
+    static_assert(PLAN_PASS == VR_PLAN_PASS && PLAN_ALL == VR_PLAN_ALL && PLAN_GRAPH == VR_PLAN_GRAPH &&
+                      PLAN_SCENE == VR_PLAN_SCENE && PLAN_GROUP == VR_PLAN_GROUP,
+                  "the ABI's plan numbers");
     Plan plan = PLAN_PASS;
     bool tunable = false;
     std::vector<Plan> candidates;
@@ -79,6 +84,14 @@ struct BoxWorld {
     int tuneStep = 0;
     hipGraphExec_t graphExec = nullptr;
     int graphUpdates = 0, graphInstantiations = 0;
+    // The scene groups of the group plan: consecutive scenes of at most groupBodies bodies together (a larger scene
+    // alone), listed largest first, each swept by a workgroup of groupThreads. VR_BOX_GROUP_BODIES and
+    // VR_BOX_GROUP_THREADS set them.
+    // Rounds of more than groupTail pairs, and every round before them, are launched one by one over the batch
+    // (VR_BOX_GROUP_TAIL); groupSplit: the first round of the step that goes to the groups.
+    Groups G{};
+    int groupBodies = 1024, groupThreads = 512, groupTail = INT_MAX, groupSplit = 0;
+    int32_t *packStarts = nullptr;  // the packed order's sort: where each (round, group, class) begins
     Colouring C{};
     // The learned warm start (no counterpart in the reference): per-point data, a log of (features, targets), and
     // the network whose prediction starts each step's impulses, when one is set.
@@ -118,6 +131,41 @@ struct BoxWorld {
 
     void reg(const char *name, void *p, int64_t count, int dtype, int bodyWords = 0) {
         named.push_back({name, BoxBuf{p, count, dtype, bodyWords}});
+    }
+
+    static int env_int(const char *name, int fallback, int least, int most) {
+        const char *v = std::getenv(name);
+        if (!v) return fallback;
+        int k = std::atoi(v);
+        return k < least || k > most ? fallback : k;
+    }
+
+    // The group plan's scene groups from the scenes' first bodies (first[k] .. first[k + 1] - 1 scene k's).
+    void make_groups(const std::vector<int32_t> &first) {
+        groupBodies = env_int("VR_BOX_GROUP_BODIES", groupBodies, 1, 1 << 30);
+        groupThreads = env_int("VR_BOX_GROUP_THREADS", groupThreads, 64, GROUP_MOST_THREADS) / 64 * 64;
+        groupTail = env_int("VR_BOX_GROUP_TAIL", groupTail, 0, INT_MAX);
+        struct Group {
+            int scene0, scene1, body0, body1;
+        };
+        std::vector<Group> groups;
+        int scenesTotal = (int)sceneBodies.size();
+        for (int k = 0; k < scenesTotal;) {
+            Group g{k, k + 1, first[k], first[k + 1]};
+            while (g.scene1 < scenesTotal && first[g.scene1 + 1] - g.body0 <= groupBodies) g.body1 = first[++g.scene1];
+            groups.push_back(g);
+            k = g.scene1;
+        }
+        std::stable_sort(groups.begin(), groups.end(),
+                         [](const Group &a, const Group &b) { return a.body1 - a.body0 > b.body1 - b.body0; });
+        std::vector<int32_t> groupOf((size_t)scenesTotal);
+        for (size_t g = 0; g < groups.size(); g++)
+            for (int k = groups[g].scene0; k < groups[g].scene1; k++) groupOf[(size_t)k] = (int32_t)g;
+        G.count = (int)groups.size();
+        G.ranges = alloc<int32_t>(2 * (int64_t)G.count * MOST_ROUNDS);
+        int32_t *sceneGroup = alloc<int32_t>(scenesTotal);
+        HIP_OK(hipMemcpy(sceneGroup, groupOf.data(), sizeof(int32_t) * groupOf.size(), hipMemcpyHostToDevice));
+        D.sceneGroup = sceneGroup;
     }
 
     void build(const vr_step_desc &desc, std::vector<int32_t> bodies, std::vector<SceneDev> sceneList,
@@ -171,6 +219,7 @@ struct BoxWorld {
             if (m == "pass") plan = PLAN_PASS;
             if (m == "all") plan = PLAN_ALL;
             if (m == "graph" && grid) plan = PLAN_GRAPH;
+            if (m == "group" && grid && !wallsFromParams) plan = PLAN_GROUP;
             sceneStep = plan == PLAN_SCENE;
             tunable = false;
         }
@@ -183,6 +232,8 @@ struct BoxWorld {
         D.scenes = (int)scenes.size();
         D.scene = sceneIdx;
         D.sceneInfo = sceneDev;
+        if (grid && !wallsFromParams && sceneBodies.size() > 1) make_groups(first);
+        if (plan == PLAN_GROUP && G.count == 0) plan = PLAN_PASS;
         D.width = std::max(1, n / 2);
         static const char *field[] = {"x",  "y",  "z",  "u",  "v",  "w",  "lx", "ly", "lz",
                                       "qx", "qy", "qz", "qw", "im", "hx", "hy", "hz", "shape"};
@@ -212,11 +263,11 @@ struct BoxWorld {
         predicted.assign((size_t)d.substeps * 2, d.sweeps);
         D.cornerHeld = alloc<uint64_t>(n);
         D.frame = alloc<float4>((int64_t)FRAME_VECTORS * n);
+        D.motion = alloc<float4>((int64_t)MOTION_VECTORS * n);
         D.broad = alloc<float4>(2 * (int64_t)n);
         D.collideFirst = alloc<int32_t>(n);
         D.collideTotal = alloc<int32_t>(1);
         D.collideScan = alloc<int32_t>(vr::scan_tmp_words(n) + 64);
-        D.motion = alloc<float4>((int64_t)MOTION_VECTORS * n);
         D.wallPlace = alloc<int32_t>(n);
         wallPlaceNext = alloc<int32_t>(n);
         reg("lambda", D.lambda, (int64_t)LAMBDA_WORDS * n, VR_F32, LAMBDA_WORDS);
@@ -355,7 +406,7 @@ struct BoxWorld {
     int sweep_mode() const { return (tol >= 0 ? SWEEP_STOP : 0) | (measure ? SWEEP_MEASURE : 0); }
 
     // Whether the host looks between a phase's sweeps (plan `pass`, and any step a plan without looks cannot run).
-    bool looks() const { return plan == PLAN_PASS || plan == PLAN_SCENE; }
+    bool looks() const { return plan == PLAN_PASS || plan == PLAN_SCENE || plan == PLAN_GROUP; }
 
     // The plan for the next step: the fixed one, or while tuning the next candidate in turn, then the fastest.
     void choose_plan() {
@@ -366,6 +417,7 @@ struct BoxWorld {
             if (tol >= 0) candidates.push_back(PLAN_ALL);
             candidates.push_back(PLAN_GRAPH);
             candidates.push_back(PLAN_SCENE);
+            if (G.count > 0) candidates.push_back(PLAN_GROUP);
             candidateMs.assign(candidates.size(), 0.f);
         }
         int rounds = 2, trials = rounds * (int)candidates.size();
@@ -381,6 +433,19 @@ struct BoxWorld {
         plan = next;
         scene_step(next == PLAN_SCENE);
     }
+
+    // The plan the next step takes, and whether the batch is still timing its plans.
+    Plan upcoming_plan() const {
+        if (!tunable) return plan;
+        if (tuneStep == 0) return PLAN_PASS;  // the first candidate of the timing about to begin
+        int trials = 2 * (int)candidates.size();
+        if (tuneStep < trials) return candidates[tuneStep % candidates.size()];
+        size_t best = 0;
+        for (size_t k = 1; k < candidates.size(); k++)
+            if (candidateMs[k] < candidateMs[best]) best = k;
+        return candidates[best];
+    }
+    bool tuning() const { return tunable && (tuneStep == 0 || tuneStep < 2 * (int)candidates.size()); }
 
     void plan_timed(float ms) {
         if (!tunable) return;
@@ -520,6 +585,8 @@ struct BoxWorld {
     // allocates.
     void pack_make() {
         int64_t cap = std::max<int64_t>(D.slots, 1);
+        D.packFrom = alloc<int32_t>(cap);
+        packStarts = alloc<int32_t>((int64_t)(MOST_ROUNDS + 1) * std::max(G.count, 1) * PACK_CLASSES + 1);
         D.packPair = alloc<int32_t>(cap);
         D.packScene = alloc<int32_t>(cap);
         D.packCount = alloc<int32_t>(cap);
@@ -639,8 +706,9 @@ struct BoxWorld {
         *unlisted = pinned[40 + REPORT_UNLISTED];
     }
 
-    // Three events for the next timed sweep.
-    hipEvent_t *timed_sweep(bool reverse) {
+    // Three events for the next timed sweep; layout 0: rounds, then walls; 1: walls, then rounds; 2 and 3: the same
+    // with the rounds in one launch (k_group_rounds).
+    hipEvent_t *timed_sweep(int layout) {
         size_t at = 3 * (size_t)sweepsTimed;
         while (sweepEvents.size() < at + 3) {
             hipEvent_t e;
@@ -648,14 +716,33 @@ struct BoxWorld {
             sweepEvents.push_back(e);
         }
         if (sweepReversed.size() <= (size_t)sweepsTimed) sweepReversed.resize(sweepsTimed + 1);
-        sweepReversed[sweepsTimed] = reverse ? 1 : 0;
+        sweepReversed[sweepsTimed] = (char)layout;
         sweepsTimed++;
         return &sweepEvents[at];
     }
 
     // BoxStep.sweep, BoxStep.java:217-231.
     void sweep(int kind, bool reverse, int rounds, int mode = 0, const SweepArgs &a = SweepArgs{0, 1, -1.f, -1}) {
-        hipEvent_t *ev = profiling && kind != WARM ? timed_sweep(reverse) : nullptr;
+        hipEvent_t *ev = profiling && kind != WARM ? timed_sweep((plan == PLAN_GROUP ? 2 : 0) + (reverse ? 1 : 0)) : nullptr;
+        if (plan == PLAN_GROUP) {
+            // Rounds before groupSplit launched one by one over the batch, the rest in one launch a workgroup a scene
+            // group; the walls as below.
+            int split = std::min(groupSplit, rounds);
+            if (ev) HIP_OK(hipEventRecord(ev[0], stream));
+            if (reverse) launch_walls(D, kind, reverse, stream, mode, a);
+            if (ev && reverse) HIP_OK(hipEventRecord(ev[1], stream));
+            for (int order = 0; order < split && !reverse; order++)
+                launch_solve_round(D, kind, reverse, roundBase[order], roundPairs[order], stream, mode, a);
+            launch_group_rounds(D, G, kind, reverse, split, rounds, groupThreads, stream, mode, a);
+            for (int order = 0; order < split && reverse; order++) {
+                int r = split - 1 - order;
+                launch_solve_round(D, kind, reverse, roundBase[r], roundPairs[r], stream, mode, a);
+            }
+            if (ev && !reverse) HIP_OK(hipEventRecord(ev[1], stream));
+            if (!reverse) launch_walls(D, kind, reverse, stream, mode, a);
+            if (ev) HIP_OK(hipEventRecord(ev[2], stream));
+            return;
+        }
         if (ev) HIP_OK(hipEventRecord(ev[0], stream));
         if (reverse) launch_walls(D, kind, reverse, stream, mode, a);
         if (ev && reverse) HIP_OK(hipEventRecord(ev[1], stream));
@@ -676,8 +763,14 @@ struct BoxWorld {
             float first = 0, second = 0;
             HIP_OK(hipEventElapsedTime(&first, ev[0], ev[1]));
             HIP_OK(hipEventElapsedTime(&second, ev[1], ev[2]));
-            float rounds = sweepReversed[k] ? second : first, walls = sweepReversed[k] ? first : second;
-            prof.round_ns += (int64_t)((double)rounds * 1e6);
+            bool reversed = (sweepReversed[k] & 1) != 0;
+            float rounds = reversed ? second : first, walls = reversed ? first : second;
+            if (sweepReversed[k] & 2) {
+                prof.sweep_ns += (int64_t)((double)rounds * 1e6);
+                prof.sweep_launches++;
+            } else {
+                prof.round_ns += (int64_t)((double)rounds * 1e6);
+            }
             prof.walls_ns += (int64_t)((double)walls * 1e6);
         }
         prof.passes += sweepsTimed;
@@ -844,7 +937,16 @@ struct BoxWorld {
             if (captured) HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             if (used > 0) {  // the sweeps' rounds read and write the touching pairs packed in round order
                 pack_room(touching);
+                bool grouped = plan == PLAN_GROUP;
+                launch_pack_order(D, C, touching, grouped ? G.count : 1, packStarts, stream);
                 launch_pack(D, C, touching, stream);
+                if (grouped) {
+                    // The rounds from the first of the tail (every round from it on of at most groupTail pairs) on
+                    // go to the groups.
+                    groupSplit = used;
+                    while (groupSplit > 0 && roundPairs[groupSplit - 1] <= groupTail) groupSplit--;
+                    launch_group_ranges(D, G, packStarts, used, stream);
+                }
             }
             for (int sub = 0; sub < d.substeps; sub++) {  // the sweeps work on the frame and motion records
                 // The frame records are made whole at the first substep; later only positions change them.
@@ -1003,6 +1105,11 @@ void box_params(BoxWorld *w, const int32_t *words, int32_t n) {
 void box_step(BoxWorld *w, vr_step_report *rep) { w->step(rep); }
 void box_collide_list(BoxWorld *w, vr_step_report *rep) { w->collide_list(rep); }
 void box_tolerance(BoxWorld *w, float tol, bool measure) { w->tolerance(tol, measure); }
+void box_retune(BoxWorld *w) { w->tuneStep = 0; }
+void box_plan(BoxWorld *w, int32_t *plan, int32_t *tuning) {
+    *plan = (int32_t)w->upcoming_plan();
+    *tuning = w->tuning() ? 1 : 0;
+}
 void box_profile(BoxWorld *w, bool on) {
     w->profiling = on;
     w->prof = vr_profile{};

@@ -893,6 +893,18 @@ int vr_box_tolerance(vr_world *w, float tolerance, int32_t measure) {
     });
 }
 
+int vr_box_retune(vr_world *w) {
+    if (!w) return fail(VR_ERR_ARGUMENT, "null world");
+    if (!w->box) return fail(VR_ERR_STATE, "not a box world");
+    return guard([&] { vrbox::box_retune(w->box); });
+}
+
+int vr_box_plan(vr_world *w, int32_t *plan, int32_t *tuning) {
+    if (!w || !plan || !tuning) return fail(VR_ERR_ARGUMENT, "null argument");
+    if (!w->box) return fail(VR_ERR_STATE, "not a box world");
+    return guard([&] { vrbox::box_plan(w->box, plan, tuning); });
+}
+
 int vr_box_solve_read(vr_world *w, vr_box_solve *out) {
     if (!w || !out) return fail(VR_ERR_ARGUMENT, "null argument");
     if (!w->box) return fail(VR_ERR_STATE, "not a box world");
