@@ -222,8 +222,11 @@ def box_roofline(prof, walls_per_sweep):
     walls_s = prof.walls_ns * 1e-9
     walls_achieved = walls_bytes / walls_s / 1e9 if walls_s > 0 else 0.0
     kbar = prof.points_sum / max(prof.listed_sum, 1)
-    kernel = ("k_group_rounds (a solve or relax sweep's colour rounds in one launch, a workgroup a scene group)"
-              if grouped else "k_solve_round (solve and relax sweeps' colour rounds, a launch a round)")
+    if grouped:
+        kernel = ("k_group_rounds_local (a solve or relax sweep's colour rounds in one launch, a workgroup a scene "
+                  "group, its motion records in LDS)")
+    else:
+        kernel = "k_solve_round (solve and relax sweeps' colour rounds, a launch a round)"
     rounds = {
         "bound": "hbm",
         "kernel": kernel + ", plain window",
@@ -310,7 +313,9 @@ WINDOW_KERNELS = {
     "k_solve_round": ("k_solve_round<1, false, 0>", "k_solve_round<2, false, 0>", "k_solve_round<1, true, 0>",
                       "k_solve_round<2, true, 0>"),
     "k_group_rounds": ("k_group_rounds<1, false, 0>", "k_group_rounds<2, false, 0>", "k_group_rounds<1, true, 0>",
-                       "k_group_rounds<2, true, 0>"),
+                       "k_group_rounds<2, true, 0>", "k_group_rounds_local<1, false, 0>",
+                       "k_group_rounds_local<2, false, 0>", "k_group_rounds_local<1, true, 0>",
+                       "k_group_rounds_local<2, true, 0>"),
     "k_walls": ("k_walls<1, false, 0>", "k_walls<2, false, 0>", "k_walls<1, true, 0>", "k_walls<2, true, 0>"),
 }
 
@@ -329,7 +334,8 @@ def summarize(trace, counters, line, out):
     counter pass (FETCH_SIZE or WRITE_SIZE, KiB a dispatch), their mean reading."""
     import csv
     d = json.loads(line)
-    rounds_kind = "k_group_rounds" if d["roofline"]["kernel"].startswith("k_group_rounds") else "k_solve_round"
+    name = d["roofline"]["kernel"]
+    rounds_kind = "k_group_rounds" if name.startswith("k_group_rounds") else "k_solve_round"
     want = {rounds_kind: d["roofline"]["launches"], "k_walls": d["roofline_walls"]["launches"]}
     rows = {k: [] for k in want}
     if trace:

@@ -140,7 +140,7 @@ typedef struct vr_profile {
      * every solve and relax sweep. round_ns / round_launches are the rounds', walls_ns / walls_launches the walls';
      * passes counts those sweeps; listed_sum and points_sum add up the touching pairs and their manifold points the
      * sweeps took (P and P k-bar of the algorithmic bytes), bodies_sum the bodies. A batch whose sweeps' rounds
-     * run a workgroup a scene group (k_group_rounds: all of a sweep's rounds in one launch) counts them in sweep_ns
+     * run a workgroup a scene group (VR_PLAN_GROUP: all of a sweep's rounds in one launch) counts them in sweep_ns
      * and sweep_launches (one a sweep) in place of round_ns and round_launches; its walls as above.
      */
     int64_t walls_ns;

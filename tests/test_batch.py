@@ -128,20 +128,21 @@ def test_a_fixed_step_mode_matches_the_oracle(mode, monkeypatch):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("tol", [-1.0, 2e-3])
-@pytest.mark.parametrize("plan", ["all", "graph", "group", "group:200:128"])
+@pytest.mark.parametrize("plan", ["all", "graph", "group", "group:200:128", "group:200:128:local", "group:1:512:local"])
 def test_each_launch_plan_matches_the_oracle(plan, tol, monkeypatch):
     """VR_BOX_PLAN fixes how a step is launched: `all` launches every sweep of a phase with no host look between them
     (a scene that has stopped passes over the rest), `graph` does the same captured as a HIP graph each step, the
     executable graph updated in place from step to step, `group` runs each sweep's rounds in one launch, a workgroup a
     group of consecutive scenes (here also in groups of at most 200 bodies, scenes of more alone, on workgroups of 128
-    threads, so that a group's round takes several passes of its workgroup). Six steps, each a replay of the step's
-    graph, with eager reads of the state between them, and the sweeps stopped at a tolerance or not: bit for bit
-    against the oracle."""
+    threads, so that a group's round takes several passes of its workgroup; and with the groups' motion records in
+    LDS over the rounds, in those groups and a scene a group). Six steps, each a replay of the step's graph, with eager
+    reads of the state between them, and the sweeps stopped at a tolerance or not: bit for bit against the oracle."""
     plan, *group = plan.split(":")
     monkeypatch.setenv("VR_BOX_PLAN", plan)
     if group:
         monkeypatch.setenv("VR_BOX_GROUP_BODIES", group[0])
         monkeypatch.setenv("VR_BOX_GROUP_THREADS", group[1])
+        monkeypatch.setenv("VR_BOX_GROUP_LOCAL", "1" if group[2:] == ["local"] else "0")
     spec = scenes.batched_2k(seed=11, scenes=40, fewest=4, most=300, substeps=3, sweeps=5)
     out = {}
     for backend in ("oracle", "gpu"):
@@ -158,3 +159,62 @@ def test_each_launch_plan_matches_the_oracle(plan, tol, monkeypatch):
     for t, (a, b) in enumerate(zip(out["oracle"], out["gpu"])):
         for name in STATE:
             assert np.array_equal(a[name].view(np.int32), b[name].view(np.int32)), (plan, tol, t, name)
+
+
+def test_the_oracle_reports_its_one_plan():
+    """vr_box_plan on the oracle: it steps a batch one way, so the plan is the pass and it never times plans; a
+    retune changes nothing."""
+    spec = small_batch()
+    sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps, abi=sc.oracle_abi())
+    start(sim, spec)
+    assert sim.plan() == ("pass", False)
+    sim.retune()
+    sim.stepAndWait()
+    assert sim.plan() == ("pass", False)
+    sim.close()
+
+
+@pytest.mark.gpu
+def test_a_batch_times_its_plans_after_a_start_and_a_retune(monkeypatch):
+    """A batch times its launch plans over its first steps (two each in turn), then keeps one; vr_box_retune, and a
+    change of tolerance, start the timing again; VR_BOX_PLAN fixes the plan, with nothing timed. Whatever the plan,
+    the state is the oracle's, bit for bit."""
+    monkeypatch.delenv("VR_BOX_PLAN", raising=False)
+    spec = scenes.batched_2k(seed=12, scenes=30, fewest=4, most=200, substeps=2, sweeps=3)
+    sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps)
+    ref = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps, abi=sc.oracle_abi())
+    for s in (sim, ref):
+        start(s, spec)
+    assert sim.plan()[1]
+    seen = set()
+    steps = 0
+    while sim.plan()[1]:
+        seen.add(sim.plan()[0])
+        sim.stepAndWait()
+        ref.stepAndWait()
+        steps += 1
+        assert steps <= 10, "still timing plans after ten steps"
+    assert {"pass", "graph", "scene", "group"} <= seen and steps == 2 * len(seen)
+    kept = sim.plan()[0]
+    sim.stepAndWait()
+    ref.stepAndWait()
+    assert sim.plan() == (kept, False)
+    sim.retune()
+    assert sim.plan() == ("pass", True)
+    sim.tolerance(1e-3)
+    assert sim.plan()[1]
+    ref.tolerance(1e-3)
+    for _ in range(3):
+        sim.stepAndWait()
+        ref.stepAndWait()
+    for name in STATE:
+        assert np.array_equal(sim.floats(name).view(np.int32), ref.floats(name).view(np.int32)), name
+    sim.close()
+    ref.close()
+    monkeypatch.setenv("VR_BOX_PLAN", "group")
+    fixed = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps)
+    start(fixed, spec)
+    assert fixed.plan() == ("group", False)
+    fixed.retune()
+    assert fixed.plan() == ("group", False)
+    fixed.close()

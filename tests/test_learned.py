@@ -106,3 +106,54 @@ def test_the_learned_start_reaches_the_residual_in_fewer_sweeps():
     # Both ran under the same stop, each scene's phase ending at the tolerance or the cap (tests/test_residual.py).
     assert carry_rep.measured and net_rep.measured and carry_rep.tolerance == net_rep.tolerance
     net.close()
+
+
+def test_the_learned_start_on_the_whole_batch():
+    """The learned start on the whole 2,048-scene batch (1,549,388 bodies), beside the carry, both stopped at the same
+    tolerance and measured: every body's state stays finite, no pair is left without a round, points start from the
+    prediction, and the learned start ends the step at the carry's residual (the largest any scene's phase ended
+    with, within 1 %, or both at or below the tolerance) with no more sweeps than the carry's, give or take 5 %. The
+    network is trained on two logged steps of the carried run itself, after 15 settling steps."""
+    spec = scenes.batched_2k()
+    runs = []
+    for _ in range(2):
+        b = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps)
+        b.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
+                spec.params)
+        runs.append(b)
+    carry, learned = runs
+    for _ in range(15):
+        for b in runs:
+            b.stepAndWait()
+    carry.log(True, 8 << 20)
+    reps = []
+    for _ in range(2):
+        reps.append(carry.stepAndWait())
+        learned.stepAndWait()
+    x, t = carry.samples(8 << 20)
+    carry.log(False)
+    assert len(x) > 100_000 and np.all(np.isfinite(x)) and np.all(np.isfinite(t))
+    net = mlp.Mlp(seed=3)
+    net.train(x, t, epochs=2, batch=4096, rate=1e-3, seed=1)
+    for b in runs:
+        b.tolerance(TOL, True)
+    learned.learned(net)
+    out = {}
+    for name, b in (("carry", carry), ("learned", learned)):
+        rep = b.stepAndWait()
+        reps.append(rep)
+        solve = b.solveReport()
+        state = [b.floats(k) for k in ("x", "y", "z", "u", "v", "w", "qx", "qy", "qz", "qw")]
+        assert all(np.all(np.isfinite(f)) for f in state), name
+        out[name] = (float(np.max(solve.residual)), float(np.sum(solve.scene_sweeps)), b.learnedPoints)
+    learned.learned(None)
+    print(f"carry: largest residual {out['carry'][0]:.4g} m/s, sweeps a scene {out['carry'][1]:.2f}; learned: "
+          f"{out['learned'][0]:.4g} m/s, {out['learned'][1]:.2f}, {out['learned'][2]} points from the prediction")
+    assert all(r.unplaced == 0 for r in reps)
+    assert out["learned"][2] > 0
+    (rc, sc_, _), (rl, sl, _) = out["carry"], out["learned"]
+    assert abs(rl - rc) <= 0.01 * rc or (rl <= TOL and rc <= TOL)
+    assert sl <= 1.05 * sc_
+    net.close()
+    for b in runs:
+        b.close()

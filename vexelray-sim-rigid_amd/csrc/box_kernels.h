@@ -199,14 +199,25 @@ void launch_pack_order(const BoxDev &D, const Colouring &C, int touching, int gr
 constexpr int GROUP_MOST_THREADS = 1024;
 struct Groups {
     int count;
+    int mostBodies;   // bodies in the largest group
+    int mostScenes;   // scenes in the group of most scenes
+    int32_t *scenes;  // 2 a group: its first scene, and one past its last
+    int32_t *bodies;  // 2 a group: its first body, and one past its last
     int32_t *ranges;  // count * MOST_ROUNDS * 2: group g's packed entries in round r, [from, to)
 };
 // Each group's packed entries in each of the step's rounds (starts: launch_pack_order's, sorted by group).
 void launch_group_ranges(const BoxDev &D, const Groups &G, const int32_t *starts, int used, hipStream_t s);
 // Rounds first .. last - 1 of a sweep (in the order of launch_solve_round's launches, so in reverse with `reverse`),
 // a workgroup of `threads` a group.
+// With `local`, each workgroup keeps its group's motion records in LDS over the rounds (MOTION_VECTORS float4 a body
+// of the largest group's, at most GROUP_LOCAL_BODIES), read in at the start and written back at the end, and with
+// the sweeps stopped (not measured) two words a scene of the group of most scenes, all within GROUP_LOCAL_BYTES.
+constexpr int GROUP_LOCAL_BODIES = 5000;
+constexpr size_t GROUP_LOCAL_BYTES = 160 * 1024 - 512;
+
 void launch_group_rounds(const BoxDev &D, const Groups &G, int sweep, bool reverse, int first, int last, int threads,
-                         hipStream_t s, int mode = 0, const SweepArgs &a = SweepArgs{0, 1, -1.f, -1});
+                         hipStream_t s, int mode = 0, const SweepArgs &a = SweepArgs{0, 1, -1.f, -1},
+                         bool local = false);
 // Round r's pairs, packed entries base .. base + pairs - 1.
 void launch_solve_round(const BoxDev &D, int sweep, bool reverse, int base, int pairs, hipStream_t s, int mode = 0,
                         const SweepArgs &a = SweepArgs{0, 1, -1.f, -1});

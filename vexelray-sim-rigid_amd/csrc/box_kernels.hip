@@ -661,15 +661,17 @@ __device__ void roll_side(Sweep sweep, Side &S, float radius, float normal, floa
 // solve_pair on two lanes, for packed entry j: `second` tells which of the pair's bodies this lane holds. Both lanes
 // of a pair call it together with the same entry. Each lane loads its own body's anchors and the impulses of the
 // points the pair has.
-template <int SWEEP, bool REVERSE>
-__device__ void solve_pair_side(const BoxDev &D, int j, bool second, float *res = nullptr) {
+// M: where the bodies' motion records are (GlobalMotion, LocalMotion).
+template <int SWEEP, bool REVERSE, class M>
+__device__ void solve_pair_side(const BoxDev &D, int j, bool second, float *res, const M &motion) {
     size_t T = (size_t)D.packCap;
     int side = second ? 1 : 0;
     int count = D.packCount[j];
     float nrm[3] = {D.packNormal[j], D.packNormal[T + j], D.packNormal[2 * T + j]};
     Side S;
     S.second = second;
-    S.me = load_framed(D.frame, D.motion, D.packBody[side * T + j]);
+    int body = D.packBody[side * T + j];
+    S.me = load_framed_at(D.frame, motion.at(body), body);
     if (!(count > 0)) return;
     HeldPoint p0, p1, p2, p3;
     auto load = [&](HeldPoint &p, int slot) {
@@ -744,7 +746,12 @@ __device__ void solve_pair_side(const BoxDev &D, int j, bool second, float *res 
         store(p2, 2);
         store(p3, 3);
     }
-    store_motion(D.motion, S.me);
+    store_motion_at(motion.at(body), S.me);
+}
+
+template <int SWEEP, bool REVERSE>
+__device__ void solve_pair_side(const BoxDev &D, int j, bool second, float *res = nullptr) {
+    solve_pair_side<SWEEP, REVERSE>(D, j, second, res, GlobalMotion{D.motion});
 }
 
 // A round's pairs, packed entries base .. base + pairs - 1, two neighbouring lanes an entry (solve_pair_side).
@@ -1005,8 +1012,18 @@ __global__ void k_group_ranges(BoxDev D, Groups G, const int32_t *starts, int us
 // The packed fields' stride is made opaque to the compiler at each iteration (an empty asm that may change it), so
 // the fifty-odd field addresses a pair's loads and stores use are worked out inside the iteration, as in a launch of
 // k_solve_round, rather than kept live across the loop in scalar registers, which would spill into vector ones.
-template <int SWEEP, bool REVERSE, int MODE>
-__device__ void group_round(const BoxDev &D0, int from, int to, const SweepArgs &a) {
+// A group's scenes' stop words in LDS over a sweep (k_group_rounds_local, stopped and not measured): whether each of
+// scenes s0 ... takes part in the sweep (sweeping), and whether a point of it has gone past the tolerance in it; the
+// latter reach the world's sceneOpen once, at the sweep's end, in place of a record a wave.
+struct LocalStop {
+    int s0;
+    const int *live;
+    int *open;
+};
+
+template <int SWEEP, bool REVERSE, int MODE, class M = GlobalMotion>
+__device__ void group_round(const BoxDev &D0, int from, int to, const SweepArgs &a, const M &motion,
+                            const LocalStop *stop = nullptr) {
     int half = (int)blockDim.x >> 1;
     bool second = (threadIdx.x & 1) != 0;
     BoxDev D = D0;
@@ -1015,13 +1032,21 @@ __device__ void group_round(const BoxDev &D0, int from, int to, const SweepArgs 
         asm volatile("" : "+s"(D.packCap));
         int k = base + ((int)threadIdx.x >> 1);
         if (MODE == 0) {
-            if (k < to) solve_pair_side<SWEEP, REVERSE>(D, k, second);
+            if (k < to) solve_pair_side<SWEEP, REVERSE>(D, k, second, nullptr, motion);
+            continue;
+        }
+        if (MODE == SWEEP_STOP && stop) {
+            int j = k < to ? D.packScene[k] - stop->s0 : 0;
+            bool live = k < to && stop->live[j];
+            float res = 0.f;
+            if (live) solve_pair_side<SWEEP, REVERSE>(D, k, second, &res, motion);
+            if (live && !second && !(res <= a.tol)) stop->open[j] = 1;  // record_one's test, lane by lane
             continue;
         }
         int scene = k < to ? D.packScene[k] : -1;
         bool live = k < to && (!(MODE & SWEEP_STOP) || sweeping(D, a, scene));
         float res = 0.f;
-        if (live) solve_pair_side<SWEEP, REVERSE>(D, k, second, &res);
+        if (live) solve_pair_side<SWEEP, REVERSE>(D, k, second, &res, motion);
         record<MODE>(D, a, live && !second, scene, res);
     }
 }
@@ -1042,10 +1067,55 @@ __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_rounds(BoxDev D, G
     for (int order = first; order < last; order++) {  // BoxStep.sweep, BoxStep.java:217-231
         int r = REVERSE ? first + last - 1 - order : order;
         if (from[r] == to[r]) continue;  // no pair of the group in round r: nothing to wait for
-        group_round<SWEEP, REVERSE, MODE>(D, from[r], to[r], a);
+        group_round<SWEEP, REVERSE, MODE>(D, from[r], to[r], a, GlobalMotion{D.motion});
         __syncthreads();
     }
 }
+
+// k_group_rounds with the group's motion records in LDS from the first round to the last: a round's body gathers of
+// the records the sweep changes become LDS accesses, and the records cross HBM once each way a sweep. (The frame
+// records, which no round writes, are still read from the world's.)
+template <int SWEEP, bool REVERSE, int MODE>
+__global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_rounds_local(BoxDev D, Groups G, int first, int last,
+                                                                           SweepArgs a) {
+    extern __shared__ float4 sMotion[];  // the group's motion records, then (stopped) its scenes' stop words
+    __shared__ int from[MOST_ROUNDS], to[MOST_ROUNDS];
+    int g = blockIdx.x;
+    int b0 = G.bodies[2 * g], b1 = G.bodies[2 * g + 1];
+    int s0 = G.scenes[2 * g], s1 = G.scenes[2 * g + 1];
+    for (int r = first + (int)threadIdx.x; r < last; r += blockDim.x) {
+        from[r] = G.ranges[2 * (g * MOST_ROUNDS + r)];
+        to[r] = G.ranges[2 * (g * MOST_ROUNDS + r) + 1];
+    }
+    const float4 *in = D.motion + (size_t)b0 * MOTION_VECTORS;
+    int words = (b1 - b0) * MOTION_VECTORS;
+    for (int k = threadIdx.x; k < words; k += blockDim.x) sMotion[k] = in[k];
+    constexpr bool localStop = MODE == SWEEP_STOP;
+    int *live = reinterpret_cast<int *>(sMotion + (size_t)G.mostBodies * MOTION_VECTORS);
+    int *open = live + G.mostScenes;
+    if (localStop)
+        for (int j = threadIdx.x; j < s1 - s0; j += blockDim.x) {
+            live[j] = sweeping(D, a, s0 + j) ? 1 : 0;
+            open[j] = 0;
+        }
+    __syncthreads();
+    LocalMotion motion{sMotion, b0};
+    LocalStop stop{s0, live, open};
+#pragma unroll 1
+    for (int order = first; order < last; order++) {  // BoxStep.sweep, BoxStep.java:217-231
+        int r = REVERSE ? first + last - 1 - order : order;
+        if (from[r] == to[r]) continue;
+        group_round<SWEEP, REVERSE, MODE>(D, from[r], to[r], a, motion, localStop ? &stop : nullptr);
+        __syncthreads();
+    }
+    __syncthreads();  // every round's records and stop words written (the last rounds may have had no pair)
+    float4 *out = D.motion + (size_t)b0 * MOTION_VECTORS;
+    for (int k = threadIdx.x; k < words; k += blockDim.x) out[k] = sMotion[k];
+    if (localStop)
+        for (int j = threadIdx.x; j < s1 - s0; j += blockDim.x)
+            if (open[j]) D.sw.sceneOpen[s0 + j] = a.g;
+}
+
 
 // BoxBroad.bin, BoxBroad.java:74-89, in the body's scene's grid.
 __global__ void k_bin(BoxDev D) {
@@ -1706,7 +1776,24 @@ void launch_group_ranges(const BoxDev &D, const Groups &G, const int32_t *starts
 
 template <int SWEEP, bool REVERSE>
 static void group_then(const BoxDev &D, const Groups &G, int first, int last, int threads, hipStream_t s, int mode,
-                       const SweepArgs &a) {
+                       const SweepArgs &a, bool local) {
+    if (local) {
+        size_t lds = sizeof(float4) * MOTION_VECTORS * (size_t)G.mostBodies + 2 * sizeof(int) * (size_t)G.mostScenes;
+        switch (SWEEP == WARM ? 0 : mode) {
+            case 0: k_group_rounds_local<SWEEP, REVERSE, 0><<<G.count, threads, lds, s>>>(D, G, first, last, a); break;
+            case SWEEP_STOP:
+                k_group_rounds_local<SWEEP, REVERSE, SWEEP_STOP><<<G.count, threads, lds, s>>>(D, G, first, last, a);
+                break;
+            case SWEEP_MEASURE:
+                k_group_rounds_local<SWEEP, REVERSE, SWEEP_MEASURE><<<G.count, threads, lds, s>>>(D, G, first, last, a);
+                break;
+            default:
+                k_group_rounds_local<SWEEP, REVERSE, SWEEP_STOP | SWEEP_MEASURE>
+                    <<<G.count, threads, lds, s>>>(D, G, first, last, a);
+                break;
+        }
+        return;
+    }
     switch (SWEEP == WARM ? 0 : mode) {
         case 0: k_group_rounds<SWEEP, REVERSE, 0><<<G.count, threads, 0, s>>>(D, G, first, last, a); break;
         case SWEEP_STOP: k_group_rounds<SWEEP, REVERSE, SWEEP_STOP><<<G.count, threads, 0, s>>>(D, G, first, last, a); break;
@@ -1720,18 +1807,21 @@ static void group_then(const BoxDev &D, const Groups &G, int first, int last, in
 }
 
 void launch_group_rounds(const BoxDev &D, const Groups &G, int sweep, bool reverse, int first, int last, int threads,
-                         hipStream_t s, int mode, const SweepArgs &a) {
+                         hipStream_t s, int mode, const SweepArgs &a, bool local) {
     if (G.count <= 0 || first >= last) return;
     if (first < 0 || last > MOST_ROUNDS) throw std::runtime_error("group rounds: rounds out of range");
+    if (local && (G.mostBodies < 1 || G.mostBodies > GROUP_LOCAL_BODIES || G.mostScenes < 1 ||
+                  G.mostBodies * sizeof(float4) * MOTION_VECTORS + 2 * sizeof(int) * G.mostScenes > GROUP_LOCAL_BYTES))
+        throw std::runtime_error("group rounds: a group's motion records and stop words do not fit in LDS");
     if (threads < 64 || threads > GROUP_MOST_THREADS || threads % 64 != 0)
         throw std::runtime_error("group rounds: a workgroup of whole waves, at most GROUP_MOST_THREADS threads");
     switch (sweep * 2 + (reverse ? 1 : 0)) {
-        case 0: group_then<WARM, false>(D, G, first, last, threads, s, 0, a); break;
-        case 1: group_then<WARM, true>(D, G, first, last, threads, s, 0, a); break;
-        case 2: group_then<SOLVE, false>(D, G, first, last, threads, s, mode, a); break;
-        case 3: group_then<SOLVE, true>(D, G, first, last, threads, s, mode, a); break;
-        case 4: group_then<RELAX, false>(D, G, first, last, threads, s, mode, a); break;
-        default: group_then<RELAX, true>(D, G, first, last, threads, s, mode, a); break;
+        case 0: group_then<WARM, false>(D, G, first, last, threads, s, 0, a, local); break;
+        case 1: group_then<WARM, true>(D, G, first, last, threads, s, 0, a, local); break;
+        case 2: group_then<SOLVE, false>(D, G, first, last, threads, s, mode, a, local); break;
+        case 3: group_then<SOLVE, true>(D, G, first, last, threads, s, mode, a, local); break;
+        case 4: group_then<RELAX, false>(D, G, first, last, threads, s, mode, a, local); break;
+        default: group_then<RELAX, true>(D, G, first, last, threads, s, mode, a, local); break;
     }
     check_launch("group rounds");
 }

@@ -197,9 +197,9 @@ __device__ void store_moving_changed(const Bufs &B, const Moving &m, const Movin
 // rounds a step, and 28 against 26 ms of walls sweeps, whose coalesced loads of the motion then stride.)
 constexpr int FRAME_VECTORS = 4, MOTION_VECTORS = 2;
 
-__device__ __forceinline__ Moving load_framed(const float4 *frame, const float4 *motion, int s) {
+// Body s's frame record and the motion record at v.
+__device__ __forceinline__ Moving load_framed_at(const float4 *frame, const float4 *v, int s) {
     const float4 *f = frame + (size_t)s * FRAME_VECTORS;
-    const float4 *v = motion + (size_t)s * MOTION_VECTORS;
     float4 a = f[0], b = f[1], c = f[2], d = f[3], e = v[0], g = v[1];
     Moving m;
     m.s = s;
@@ -212,6 +212,22 @@ __device__ __forceinline__ Moving load_framed(const float4 *frame, const float4 
     m.inverseInertiaTimes(m.mom, m.omega);
     return m;
 }
+
+__device__ __forceinline__ Moving load_framed(const float4 *frame, const float4 *motion, int s) {
+    return load_framed_at(frame, motion + (size_t)s * MOTION_VECTORS, s);
+}
+
+// Where the sweeps find a body's motion record: the world's records, or a workgroup's copy of a run of them in LDS
+// (bodies b0 ..., k_group_rounds).
+struct GlobalMotion {
+    float4 *m;
+    __device__ __forceinline__ float4 *at(int s) const { return m + (size_t)s * MOTION_VECTORS; }
+};
+struct LocalMotion {
+    float4 *m;
+    int b0;
+    __device__ __forceinline__ float4 *at(int s) const { return m + (s - b0) * MOTION_VECTORS; }
+};
 
 // The state's fields with the motion record's velocity and momentum (the walls sweep, whose neighbouring lanes hold
 // neighbouring bodies, so the fields' loads coalesce).
@@ -234,10 +250,13 @@ __device__ __forceinline__ Moving load_moved(const Bufs &B, const float4 *motion
     return m;
 }
 
-__device__ __forceinline__ void store_motion(float4 *motion, const Moving &m) {
-    float4 *v = motion + (size_t)m.s * MOTION_VECTORS;
+__device__ __forceinline__ void store_motion_at(float4 *v, const Moving &m) {
     v[0] = make_float4(m.vel[0], m.vel[1], m.vel[2], 0.f);
     v[1] = make_float4(m.mom[0], m.mom[1], m.mom[2], 0.f);
+}
+
+__device__ __forceinline__ void store_motion(float4 *motion, const Moving &m) {
+    store_motion_at(motion + (size_t)m.s * MOTION_VECTORS, m);
 }
 
 // store_motion for a body loaded as `was`: a vector none of whose words changed is left as it is.

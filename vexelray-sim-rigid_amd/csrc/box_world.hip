@@ -85,12 +85,18 @@ struct BoxWorld {
     hipGraphExec_t graphExec = nullptr;
     int graphUpdates = 0, graphInstantiations = 0;
     // The scene groups of the group plan: consecutive scenes of at most groupBodies bodies together (a larger scene
-    // alone), listed largest first, each swept by a workgroup of groupThreads. VR_BOX_GROUP_BODIES and
-    // VR_BOX_GROUP_THREADS set them.
+    // alone), listed largest first, each swept by a workgroup of groupThreads. With every group within
+    // GROUP_LOCAL_BODIES, groups of up to 4,096 bodies on workgroups of 1,024 threads keep their motion records in LDS
+    // over a sweep's rounds; else groups of up to 1,024 bodies on 512 threads read them from HBM (the faster of each
+    // on the 2,048-scene batch: 61.6 and 79.1 ms of rounds a step, against 64-110 for the others tried). Each group's
+    // walls in the same launch as its rounds, tried, took as long as the separate walls launch (88.3 against 61.7 +
+    // 25.5 ms a step), so the walls stay a launch over the batch.
+    // VR_BOX_GROUP_BODIES, VR_BOX_GROUP_THREADS and VR_BOX_GROUP_LOCAL=0|1 set them.
     // Rounds of more than groupTail pairs, and every round before them, are launched one by one over the batch
     // (VR_BOX_GROUP_TAIL); groupSplit: the first round of the step that goes to the groups.
     Groups G{};
-    int groupBodies = 1024, groupThreads = 512, groupTail = INT_MAX, groupSplit = 0;
+    int groupBodies = 4096, groupThreads = 1024, groupTail = INT_MAX, groupSplit = 0;
+    bool groupLocal = true;  // the groups' motion records in LDS over a sweep's rounds
     int32_t *packStarts = nullptr;  // the packed order's sort: where each (round, group, class) begins
     Colouring C{};
     // The learned warm start (no counterpart in the reference): per-point data, a log of (features, targets), and
@@ -142,7 +148,12 @@ struct BoxWorld {
 
     // The group plan's scene groups from the scenes' first bodies (first[k] .. first[k + 1] - 1 scene k's).
     void make_groups(const std::vector<int32_t> &first) {
+        int largest = *std::max_element(sceneBodies.begin(), sceneBodies.end());
+        groupLocal = env_int("VR_BOX_GROUP_LOCAL", 1, 0, 1) == 1 && largest <= GROUP_LOCAL_BODIES;
+        groupBodies = groupLocal ? 4096 : 1024;
+        groupThreads = groupLocal ? 1024 : 512;
         groupBodies = env_int("VR_BOX_GROUP_BODIES", groupBodies, 1, 1 << 30);
+        if (groupLocal) groupBodies = std::min(groupBodies, GROUP_LOCAL_BODIES);
         groupThreads = env_int("VR_BOX_GROUP_THREADS", groupThreads, 64, GROUP_MOST_THREADS) / 64 * 64;
         groupTail = env_int("VR_BOX_GROUP_TAIL", groupTail, 0, INT_MAX);
         struct Group {
@@ -163,6 +174,20 @@ struct BoxWorld {
             for (int k = groups[g].scene0; k < groups[g].scene1; k++) groupOf[(size_t)k] = (int32_t)g;
         G.count = (int)groups.size();
         G.ranges = alloc<int32_t>(2 * (int64_t)G.count * MOST_ROUNDS);
+        std::vector<int32_t> bo, sc;
+        G.mostBodies = G.mostScenes = 0;
+        for (const Group &g : groups) {
+            bo.push_back(g.body0), bo.push_back(g.body1);
+            sc.push_back(g.scene0), sc.push_back(g.scene1);
+            G.mostBodies = std::max(G.mostBodies, g.body1 - g.body0);
+            G.mostScenes = std::max(G.mostScenes, g.scene1 - g.scene0);
+        }
+        G.bodies = alloc<int32_t>(2 * (int64_t)G.count);
+        G.scenes = alloc<int32_t>(2 * (int64_t)G.count);
+        HIP_OK(hipMemcpy(G.bodies, bo.data(), sizeof(int32_t) * bo.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(G.scenes, sc.data(), sizeof(int32_t) * sc.size(), hipMemcpyHostToDevice));
+        size_t ldsBytes = sizeof(float4) * MOTION_VECTORS * (size_t)G.mostBodies + 2 * sizeof(int) * G.mostScenes;
+        if (groupLocal && ldsBytes > GROUP_LOCAL_BYTES) groupLocal = false;
         int32_t *sceneGroup = alloc<int32_t>(scenesTotal);
         HIP_OK(hipMemcpy(sceneGroup, groupOf.data(), sizeof(int32_t) * groupOf.size(), hipMemcpyHostToDevice));
         D.sceneGroup = sceneGroup;
@@ -723,7 +748,8 @@ struct BoxWorld {
 
     // BoxStep.sweep, BoxStep.java:217-231.
     void sweep(int kind, bool reverse, int rounds, int mode = 0, const SweepArgs &a = SweepArgs{0, 1, -1.f, -1}) {
-        hipEvent_t *ev = profiling && kind != WARM ? timed_sweep((plan == PLAN_GROUP ? 2 : 0) + (reverse ? 1 : 0)) : nullptr;
+        int layout = (plan == PLAN_GROUP ? 2 : 0) + (reverse ? 1 : 0);
+        hipEvent_t *ev = profiling && kind != WARM ? timed_sweep(layout) : nullptr;
         if (plan == PLAN_GROUP) {
             // Rounds before groupSplit launched one by one over the batch, the rest in one launch a workgroup a scene
             // group; the walls as below.
@@ -733,7 +759,7 @@ struct BoxWorld {
             if (ev && reverse) HIP_OK(hipEventRecord(ev[1], stream));
             for (int order = 0; order < split && !reverse; order++)
                 launch_solve_round(D, kind, reverse, roundBase[order], roundPairs[order], stream, mode, a);
-            launch_group_rounds(D, G, kind, reverse, split, rounds, groupThreads, stream, mode, a);
+            launch_group_rounds(D, G, kind, reverse, split, rounds, groupThreads, stream, mode, a, groupLocal);
             for (int order = 0; order < split && reverse; order++) {
                 int r = split - 1 - order;
                 launch_solve_round(D, kind, reverse, roundBase[r], roundPairs[r], stream, mode, a);
@@ -765,6 +791,7 @@ struct BoxWorld {
             HIP_OK(hipEventElapsedTime(&second, ev[1], ev[2]));
             bool reversed = (sweepReversed[k] & 1) != 0;
             float rounds = reversed ? second : first, walls = reversed ? first : second;
+            prof.walls_launches++;
             if (sweepReversed[k] & 2) {
                 prof.sweep_ns += (int64_t)((double)rounds * 1e6);
                 prof.sweep_launches++;
@@ -774,7 +801,6 @@ struct BoxWorld {
             prof.walls_ns += (int64_t)((double)walls * 1e6);
         }
         prof.passes += sweepsTimed;
-        prof.walls_launches += sweepsTimed;
         prof.round_launches += roundsThisStep;
         prof.listed_sum += (int64_t)touching * sweepsTimed;
         prof.points_sum += (int64_t)points * sweepsTimed;
@@ -827,8 +853,13 @@ struct BoxWorld {
     // The first solve sweep of the pairs alone (the warm sweep's rounds, then the solve sweep's, no walls), measured
     // as sweep g: what a start leaves each scene's pairs to do.
     void trial_sweep(int rounds, int g) {
-        for (int r = 0; r < rounds; r++) launch_solve_round(D, WARM, false, roundBase[r], roundPairs[r], stream);
         SweepArgs a{g, 1, -1.f, -1};
+        if (plan == PLAN_GROUP) {  // the rounds of each in one launch, as the plan's sweeps run them
+            launch_group_rounds(D, G, WARM, false, 0, rounds, groupThreads, stream, 0, a, groupLocal);
+            launch_group_rounds(D, G, SOLVE, false, 0, rounds, groupThreads, stream, SWEEP_MEASURE, a, groupLocal);
+            return;
+        }
+        for (int r = 0; r < rounds; r++) launch_solve_round(D, WARM, false, roundBase[r], roundPairs[r], stream);
         for (int r = 0; r < rounds; r++)
             launch_solve_round(D, SOLVE, false, roundBase[r], roundPairs[r], stream, SWEEP_MEASURE, a);
     }
