@@ -141,7 +141,9 @@ typedef struct vr_profile {
      * passes counts those sweeps; listed_sum and points_sum add up the touching pairs and their manifold points the
      * sweeps took (P and P k-bar of the algorithmic bytes), bodies_sum the bodies. A batch whose sweeps' rounds
      * run a workgroup a scene group (VR_PLAN_GROUP: all of a sweep's rounds in one launch) counts them in sweep_ns
-     * and sweep_launches (one a sweep) in place of round_ns and round_launches; its walls as above.
+     * and sweep_launches (one a sweep) in place of round_ns and round_launches, its walls as above; and one that runs
+     * a whole phase of sweeps a launch (k_group_phase: each group through the phase's walls and rounds) counts each
+     * phase launch in sweep_ns and sweep_launches, walls_ns and walls_launches then staying at zero.
      */
     int64_t walls_ns;
     int64_t walls_launches;

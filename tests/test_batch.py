@@ -127,26 +127,30 @@ def test_a_fixed_step_mode_matches_the_oracle(mode, monkeypatch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("symmetric", [False, True])
 @pytest.mark.parametrize("tol", [-1.0, 2e-3])
-@pytest.mark.parametrize("plan", ["all", "graph", "group", "group:200:128", "group:200:128:local", "group:1:512:local"])
-def test_each_launch_plan_matches_the_oracle(plan, tol, monkeypatch):
+@pytest.mark.parametrize("plan", ["all", "graph", "group", "group:200:128", "group:200:128:local", "group:1:512:local",
+                                  "group:200:128:phase", "group:1:512:phase"])
+def test_each_launch_plan_matches_the_oracle(plan, tol, symmetric, monkeypatch):
     """VR_BOX_PLAN fixes how a step is launched: `all` launches every sweep of a phase with no host look between them
     (a scene that has stopped passes over the rest), `graph` does the same captured as a HIP graph each step, the
     executable graph updated in place from step to step, `group` runs each sweep's rounds in one launch, a workgroup a
-    group of consecutive scenes (here also in groups of at most 200 bodies, scenes of more alone, on workgroups of 128
-    threads, so that a group's round takes several passes of its workgroup; and with the groups' motion records in
-    LDS over the rounds, in those groups and a scene a group). Six steps, each a replay of the step's graph, with eager
+    group of consecutive scenes, or by default a whole phase of sweeps, walls too, in one launch (here also in groups
+    of at most 200 bodies, scenes of more alone, on workgroups of 128 threads, so that a group's round takes several
+    passes of its workgroup; with the groups' motion records in LDS over a sweep's rounds, or over the phase, in those
+    groups and a scene a group). With every other sweep backwards or not. Six steps, each a replay of the step's graph, with eager
     reads of the state between them, and the sweeps stopped at a tolerance or not: bit for bit against the oracle."""
     plan, *group = plan.split(":")
     monkeypatch.setenv("VR_BOX_PLAN", plan)
     if group:
         monkeypatch.setenv("VR_BOX_GROUP_BODIES", group[0])
         monkeypatch.setenv("VR_BOX_GROUP_THREADS", group[1])
-        monkeypatch.setenv("VR_BOX_GROUP_LOCAL", "1" if group[2:] == ["local"] else "0")
+        monkeypatch.setenv("VR_BOX_GROUP_LOCAL", "1" if group[2:] in (["local"], ["phase"]) else "0")
+        monkeypatch.setenv("VR_BOX_GROUP_PHASE", "1" if group[2:] == ["phase"] else "0")
     spec = scenes.batched_2k(seed=11, scenes=40, fewest=4, most=300, substeps=3, sweeps=5)
     out = {}
     for backend in ("oracle", "gpu"):
-        sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps,
+        sim = boxes.BoxBatch(spec.bodies, spec.extents, spec.grids, spec.substeps, spec.sweeps, symmetric,
                              abi=sc.abi_for(backend))
         start(sim, spec)
         sim.tolerance(tol, False)

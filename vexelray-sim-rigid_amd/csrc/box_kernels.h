@@ -203,6 +203,7 @@ struct Groups {
     int mostScenes;   // scenes in the group of most scenes
     int32_t *scenes;  // 2 a group: its first scene, and one past its last
     int32_t *bodies;  // 2 a group: its first body, and one past its last
+    int32_t *order;   // n: places bodies[2g] .. bodies[2g + 1] - 1 hold group g's bodies, its boxes first
     int32_t *ranges;  // count * MOST_ROUNDS * 2: group g's packed entries in round r, [from, to)
 };
 // Each group's packed entries in each of the step's rounds (starts: launch_pack_order's, sorted by group).
@@ -214,6 +215,14 @@ void launch_group_ranges(const BoxDev &D, const Groups &G, const int32_t *starts
 // the sweeps stopped (not measured) two words a scene of the group of most scenes, all within GROUP_LOCAL_BYTES.
 constexpr int GROUP_LOCAL_BODIES = 5000;
 constexpr size_t GROUP_LOCAL_BYTES = 160 * 1024 - 512;
+// A phase of `sweeps` sweeps numbered g0 ... (a warm sweep: one), each its walls and its `used` rounds in the order of
+// launch_walls and launch_solve_round, a workgroup of `threads` a group through all of them, its motion records and
+// its scenes' stop words in LDS (k_group_phase). mode and tol as for the launches; curveBase: where a measured
+// phase keeps its sweeps' largest residuals (-1: not kept).
+void launch_group_phase(const BoxDev &D, const Groups &G, int sweep, int used, int sweeps, bool symmetric, int g0,
+                        float tol, int curveBase, int threads, hipStream_t s, int mode);
+// Each group's bodies in `order`, its boxes first (each kind in body order), for the shapes as they are.
+void launch_group_order(const BoxDev &D, const Groups &G, hipStream_t s);
 
 void launch_group_rounds(const BoxDev &D, const Groups &G, int sweep, bool reverse, int first, int last, int threads,
                          hipStream_t s, int mode = 0, const SweepArgs &a = SweepArgs{0, 1, -1.f, -1},

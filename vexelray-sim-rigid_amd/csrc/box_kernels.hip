@@ -290,15 +290,18 @@ __device__ unsigned long long corner_sweep(const Bufs &B, Moving &box, const flo
 }
 
 // Body s, its wall impulse words at place `place`. Its cornerHeld word is rewritten by a solve or relax sweep (a
-// warm sweep changes no impulse).
-template <int SWEEP, bool REVERSE, bool MOTION = false>
-__device__ void walls_of(const BoxDev &D, int s, int place, float *res = nullptr) {
+// warm sweep changes no impulse). With MOTION its velocity and momentum are its motion record's, found through
+// `motion` (the world's records when null); else the state's.
+template <int SWEEP, bool REVERSE, bool MOTION = false, class M = GlobalMotion>
+__device__ void walls_of(const BoxDev &D, int s, int place, float *res = nullptr, const M *motion = nullptr) {
     if (!(D.im[s] > 0)) return;
     Bufs B = bufs(D);
     const float *P = D.params;
     const SceneDev sc = D.sceneInfo[D.scene[s]];
     float extent[3] = {sc.sx, sc.sy, sc.sz};
-    Moving box = MOTION ? load_moved(B, D.motion, s) : load_moving(B, s);
+    float4 *record = nullptr;
+    if (MOTION) record = motion ? motion->at(s) : D.motion + (size_t)s * MOTION_VECTORS;
+    Moving box = MOTION ? load_moved_at(B, record, s) : load_moving(B, s);
     const Moving loaded = box;
     Soft soft(P, true);
     size_t n = (size_t)D.n;
@@ -311,7 +314,7 @@ __device__ void walls_of(const BoxDev &D, int s, int place, float *res = nullptr
     }
     if (SWEEP != WARM && nowHeld != held) D.cornerHeld[s] = nowHeld;
     if (MOTION)
-        store_motion_changed(D.motion, box, loaded);
+        store_motion_changed_at(record, box, loaded);
     else
         store_moving_changed(B, box, loaded);
 }
@@ -1035,12 +1038,14 @@ __device__ void group_round(const BoxDev &D0, int from, int to, const SweepArgs 
             if (k < to) solve_pair_side<SWEEP, REVERSE>(D, k, second, nullptr, motion);
             continue;
         }
-        if (MODE == SWEEP_STOP && stop) {
-            int j = k < to ? D.packScene[k] - stop->s0 : 0;
+        if ((MODE & SWEEP_STOP) && stop) {
+            int scene = k < to ? D.packScene[k] : -1;
+            int j = k < to ? scene - stop->s0 : 0;
             bool live = k < to && stop->live[j];
             float res = 0.f;
             if (live) solve_pair_side<SWEEP, REVERSE>(D, k, second, &res, motion);
             if (live && !second && !(res <= a.tol)) stop->open[j] = 1;  // record_one's test, lane by lane
+            if (MODE & SWEEP_MEASURE) record<SWEEP_MEASURE>(D, a, live && !second, scene, res);
             continue;
         }
         int scene = k < to ? D.packScene[k] : -1;
@@ -1116,6 +1121,153 @@ __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_rounds_local(BoxDe
             if (open[j]) D.sw.sceneOpen[s0 + j] = a.g;
 }
 
+
+// ---- A whole phase of sweeps a workgroup a scene group ----
+//
+// Between two sweeps of a phase, and between a sweep's rounds and its walls, a scene waits only on itself. So a
+// workgroup can take its group through all of a phase's sweeps (each its walls and rounds in BoxStep.sweep's order),
+// its motion records in LDS throughout, with a barrier where the launches have a kernel boundary: a group whose
+// sweeps are done, or whose scenes have all stopped at the tolerance, leaves its CU to the next group instead of
+// waiting for the batch's slowest group at every sweep and every walls launch. Each scene's stop is decided as the
+// launches decide it (sweeping, record_one), from the scene's own words in LDS.
+
+// Group g's bodies, boxes first, then balls, each kind in body order (a stable partition by shape over the group), so
+// that the waves of a group's walls run one path.
+__global__ void __launch_bounds__(THREADS) k_group_order(BoxDev D, Groups G) {
+    __shared__ int sWave[THREADS / 64];
+    int g = blockIdx.x;
+    int b0 = G.bodies[2 * g], b1 = G.bodies[2 * g + 1];
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int boxes = 0;  // the group's boxes, the same in every thread
+    for (int base = b0; base < b1; base += THREADS) {
+        int s = base + (int)threadIdx.x;
+        unsigned long long mask = __ballot(s < b1 && !(D.shape[s] > 0.5f));  // Moving::sphere
+        if (lane == 0) sWave[wave] = __popcll(mask);
+        __syncthreads();
+        for (int w = 0; w < THREADS / 64; w++) boxes += sWave[w];
+        __syncthreads();
+    }
+    int boxesDone = 0, ballsDone = 0;  // before this chunk, the same in every thread
+    for (int base = b0; base < b1; base += THREADS) {
+        int s = base + (int)threadIdx.x;
+        bool in = s < b1;
+        bool box = in && !(D.shape[s] > 0.5f);
+        unsigned long long boxMask = __ballot(box);
+        if (lane == 0) sWave[wave] = __popcll(boxMask);
+        __syncthreads();
+        int before = 0, chunkBoxes = 0;
+        for (int w = 0; w < THREADS / 64; w++) {
+            before += w < wave ? sWave[w] : 0;
+            chunkBoxes += sWave[w];
+        }
+        __syncthreads();
+        int boxesBefore = before + __popcll(boxMask & ((1ull << lane) - 1ull));
+        int ballsBefore = (int)threadIdx.x - boxesBefore;  // the threads before this one are all in the group
+        if (in) G.order[b0 + (box ? boxesDone + boxesBefore : boxes + ballsDone + ballsBefore)] = s;
+        boxesDone += chunkBoxes;
+        ballsDone += min(THREADS, b1 - base) - chunkBoxes;
+    }
+}
+
+// A group's walls in a sweep: k_walls's work for the group's bodies, a thread a body (boxes first), the motion in
+// LDS. Every thread runs the same number of iterations, so the waves' residual records see whole waves.
+template <int SWEEP, bool REVERSE, int MODE>
+__device__ void group_walls(const BoxDev &D0, const Groups &G, int b0, int b1, const SweepArgs &a,
+                            const LocalMotion &motion, const LocalStop &stop) {
+    BoxDev D = D0;
+#pragma unroll 1
+    for (int base = b0; base < b1; base += (int)blockDim.x) {
+        asm volatile("" : "+s"(D.n));  // the wall words' addresses worked out in the iteration, as in group_round
+        int p = base + (int)threadIdx.x;
+        int s = p < b1 ? G.order[p] : -1;
+        if (MODE == 0) {
+            if (s >= 0) walls_of<SWEEP, REVERSE, true>(D, s, D.wallPlace[s], nullptr, &motion);
+            continue;
+        }
+        int scene = s >= 0 ? D.scene[s] : -1;
+        bool live = s >= 0 && (!(MODE & SWEEP_STOP) || stop.live[scene - stop.s0]);
+        float res = 0.f;
+        if (live) walls_of<SWEEP, REVERSE, true>(D, s, D.wallPlace[s], &res, &motion);
+        if ((MODE & SWEEP_STOP) && live && !(res <= a.tol)) stop.open[scene - stop.s0] = 1;
+        if (MODE & SWEEP_MEASURE) record<SWEEP_MEASURE>(D, a, live, scene, res);
+    }
+}
+
+// One sweep of a group: walls, then rounds in reverse with REVERSE; rounds, then walls otherwise (BoxStep.sweep,
+// BoxStep.java:217-231). Ends with a barrier.
+template <int SWEEP, bool REVERSE, int MODE>
+__device__ void group_sweep(const BoxDev &D, const Groups &G, int b0, int b1, int used, const int *from,
+                            const int *to, const SweepArgs &a, const LocalMotion &motion, const LocalStop &stop) {
+    const LocalStop *stops = (MODE & SWEEP_STOP) ? &stop : nullptr;
+    if (REVERSE) {
+        group_walls<SWEEP, REVERSE, MODE>(D, G, b0, b1, a, motion, stop);
+        __syncthreads();
+    }
+#pragma unroll 1
+    for (int order = 0; order < used; order++) {
+        int r = REVERSE ? used - 1 - order : order;
+        if (from[r] == to[r]) continue;  // no pair of the group in round r: nothing to wait for
+        group_round<SWEEP, REVERSE, MODE>(D, from[r], to[r], a, motion, stops);
+        __syncthreads();
+    }
+    if (!REVERSE) group_walls<SWEEP, REVERSE, MODE>(D, G, b0, b1, a, motion, stop);
+    __syncthreads();
+}
+
+// `sweeps` sweeps of a phase, numbered g0 ..., each a group's walls and rounds; with SYMMETRIC every other sweep runs
+// backwards (BoxStep.java:204-214). Stopped (MODE), a scene takes part in a sweep when it is the phase's first or the
+// scene had a point past `tol` in the sweep before, as sweeping decides it, and a group whose scenes have all stopped
+// is done. Measured, each sweep records as the launches do, its largest residual at curveBase + k (curveBase -1:
+// not kept).
+template <int SWEEP, int MODE, bool SYMMETRIC>
+__global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_phase(BoxDev D, Groups G, int used, int sweeps, int g0,
+                                                                    float tol, int curveBase) {
+    extern __shared__ float4 sMotion[];  // the group's motion records, then two words a scene of the group
+    __shared__ int from[MOST_ROUNDS], to[MOST_ROUNDS];
+    __shared__ int anyLive;
+    int g = blockIdx.x;
+    int b0 = G.bodies[2 * g], b1 = G.bodies[2 * g + 1];
+    int s0 = G.scenes[2 * g], s1 = G.scenes[2 * g + 1];
+    for (int r = threadIdx.x; r < used; r += blockDim.x) {
+        from[r] = G.ranges[2 * (g * MOST_ROUNDS + r)];
+        to[r] = G.ranges[2 * (g * MOST_ROUNDS + r) + 1];
+    }
+    const float4 *in = D.motion + (size_t)b0 * MOTION_VECTORS;
+    int words = (b1 - b0) * MOTION_VECTORS;
+    for (int k = threadIdx.x; k < words; k += blockDim.x) sMotion[k] = in[k];
+    int *live = reinterpret_cast<int *>(sMotion + (size_t)G.mostBodies * MOTION_VECTORS);
+    int *open = live + G.mostScenes;
+    constexpr bool stopped = (MODE & SWEEP_STOP) != 0;
+    __syncthreads();
+    LocalMotion motion{sMotion, b0};
+    LocalStop stop{s0, live, open};
+#pragma unroll 1
+    for (int k = 0; k < sweeps; k++) {
+        SweepArgs a{g0 + k, k == 0 ? 1 : 0, tol, curveBase >= 0 && k < SOLVE_MOST_SWEEPS ? curveBase + k : -1};
+        if (stopped) {
+            if (threadIdx.x == 0) anyLive = 0;
+            __syncthreads();
+            for (int j = threadIdx.x; j < s1 - s0; j += blockDim.x) {
+                int l = (k == 0 || open[j]) ? 1 : 0;  // sweeping: the first sweep, or open in the one before
+                live[j] = l;
+                open[j] = 0;
+                if (l) anyLive = 1;
+            }
+            __syncthreads();
+            if (!anyLive) break;  // no scene of the group takes part in this sweep or any after it
+        }
+        if (SYMMETRIC && k % 2 == 1)
+            group_sweep<SWEEP, true, MODE>(D, G, b0, b1, used, from, to, a, motion, stop);
+        else
+            group_sweep<SWEEP, false, MODE>(D, G, b0, b1, used, from, to, a, motion, stop);
+        if (stopped)  // the world's stop words as the launches leave them
+            for (int j = threadIdx.x; j < s1 - s0; j += blockDim.x)
+                if (open[j]) D.sw.sceneOpen[s0 + j] = a.g;
+    }
+    __syncthreads();
+    float4 *out = D.motion + (size_t)b0 * MOTION_VECTORS;
+    for (int k = threadIdx.x; k < words; k += blockDim.x) out[k] = sMotion[k];
+}
 
 // BoxBroad.bin, BoxBroad.java:74-89, in the body's scene's grid.
 __global__ void k_bin(BoxDev D) {
@@ -1824,6 +1976,55 @@ void launch_group_rounds(const BoxDev &D, const Groups &G, int sweep, bool rever
         default: group_then<RELAX, true>(D, G, first, last, threads, s, mode, a, local); break;
     }
     check_launch("group rounds");
+}
+
+void launch_group_order(const BoxDev &D, const Groups &G, hipStream_t s) {
+    if (G.count <= 0) return;
+    k_group_order<<<G.count, THREADS, 0, s>>>(D, G);
+    check_launch("group order");
+}
+
+template <int SWEEP, bool SYMMETRIC>
+static void group_phase_then(const BoxDev &D, const Groups &G, int used, int sweeps, int g0, float tol, int curveBase,
+                             int threads, size_t lds, hipStream_t s, int mode) {
+    switch (SWEEP == WARM ? 0 : mode) {
+        case 0:
+            k_group_phase<SWEEP, 0, SYMMETRIC><<<G.count, threads, lds, s>>>(D, G, used, sweeps, g0, tol, curveBase);
+            break;
+        case SWEEP_STOP:
+            k_group_phase<SWEEP, SWEEP_STOP, SYMMETRIC>
+                <<<G.count, threads, lds, s>>>(D, G, used, sweeps, g0, tol, curveBase);
+            break;
+        case SWEEP_MEASURE:
+            k_group_phase<SWEEP, SWEEP_MEASURE, SYMMETRIC>
+                <<<G.count, threads, lds, s>>>(D, G, used, sweeps, g0, tol, curveBase);
+            break;
+        default:
+            k_group_phase<SWEEP, SWEEP_STOP | SWEEP_MEASURE, SYMMETRIC>
+                <<<G.count, threads, lds, s>>>(D, G, used, sweeps, g0, tol, curveBase);
+            break;
+    }
+}
+
+void launch_group_phase(const BoxDev &D, const Groups &G, int sweep, int used, int sweeps, bool symmetric, int g0,
+                        float tol, int curveBase, int threads, hipStream_t s, int mode) {
+    if (G.count <= 0 || sweeps <= 0) return;
+    if (used < 0 || used > MOST_ROUNDS) throw std::runtime_error("group phase: rounds out of range");
+    size_t lds = sizeof(float4) * MOTION_VECTORS * (size_t)G.mostBodies + 2 * sizeof(int) * (size_t)G.mostScenes;
+    if (G.mostBodies < 1 || G.mostBodies > GROUP_LOCAL_BODIES || G.mostScenes < 1 || lds > GROUP_LOCAL_BYTES)
+        throw std::runtime_error("group phase: a group's motion records and stop words do not fit in LDS");
+    if (threads < 64 || threads > GROUP_MOST_THREADS || threads % 64 != 0)
+        throw std::runtime_error("group phase: a workgroup of whole waves, at most GROUP_MOST_THREADS threads");
+    if (sweep == WARM) {
+        group_phase_then<WARM, false>(D, G, used, 1, g0, tol, -1, threads, lds, s, 0);
+    } else if (sweep == SOLVE) {
+        if (symmetric) group_phase_then<SOLVE, true>(D, G, used, sweeps, g0, tol, curveBase, threads, lds, s, mode);
+        else group_phase_then<SOLVE, false>(D, G, used, sweeps, g0, tol, curveBase, threads, lds, s, mode);
+    } else {
+        if (symmetric) group_phase_then<RELAX, true>(D, G, used, sweeps, g0, tol, curveBase, threads, lds, s, mode);
+        else group_phase_then<RELAX, false>(D, G, used, sweeps, g0, tol, curveBase, threads, lds, s, mode);
+    }
+    check_launch("group phase");
 }
 
 void launch_pack(const BoxDev &D, const Colouring &C, int touching, hipStream_t s) {

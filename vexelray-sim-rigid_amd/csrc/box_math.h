@@ -231,8 +231,8 @@ struct LocalMotion {
 
 // The state's fields with the motion record's velocity and momentum (the walls sweep, whose neighbouring lanes hold
 // neighbouring bodies, so the fields' loads coalesce).
-__device__ __forceinline__ Moving load_moved(const Bufs &B, const float4 *motion, int s) {
-    const float4 *v = motion + (size_t)s * MOTION_VECTORS;
+// The state's fields of body s with the velocity and momentum of the motion record at v.
+__device__ __forceinline__ Moving load_moved_at(const Bufs &B, const float4 *v, int s) {
     float4 e = v[0], g = v[1];
     Moving m;
     m.s = s;
@@ -250,6 +250,10 @@ __device__ __forceinline__ Moving load_moved(const Bufs &B, const float4 *motion
     return m;
 }
 
+__device__ __forceinline__ Moving load_moved(const Bufs &B, const float4 *motion, int s) {
+    return load_moved_at(B, motion + (size_t)s * MOTION_VECTORS, s);
+}
+
 __device__ __forceinline__ void store_motion_at(float4 *v, const Moving &m) {
     v[0] = make_float4(m.vel[0], m.vel[1], m.vel[2], 0.f);
     v[1] = make_float4(m.mom[0], m.mom[1], m.mom[2], 0.f);
@@ -260,14 +264,17 @@ __device__ __forceinline__ void store_motion(float4 *motion, const Moving &m) {
 }
 
 // store_motion for a body loaded as `was`: a vector none of whose words changed is left as it is.
-__device__ __forceinline__ void store_motion_changed(float4 *motion, const Moving &m, const Moving &was) {
-    float4 *v = motion + (size_t)m.s * MOTION_VECTORS;
+__device__ __forceinline__ void store_motion_changed_at(float4 *v, const Moving &m, const Moving &was) {
     if ((bits(m.vel[0]) != bits(was.vel[0])) | (bits(m.vel[1]) != bits(was.vel[1])) |
         (bits(m.vel[2]) != bits(was.vel[2])))
         v[0] = make_float4(m.vel[0], m.vel[1], m.vel[2], 0.f);
     if ((bits(m.mom[0]) != bits(was.mom[0])) | (bits(m.mom[1]) != bits(was.mom[1])) |
         (bits(m.mom[2]) != bits(was.mom[2])))
         v[1] = make_float4(m.mom[0], m.mom[1], m.mom[2], 0.f);
+}
+
+__device__ __forceinline__ void store_motion_changed(float4 *motion, const Moving &m, const Moving &was) {
+    store_motion_changed_at(motion + (size_t)m.s * MOTION_VECTORS, m, was);
 }
 
 // Boxes.Soft, Boxes.java:421-450.

@@ -97,6 +97,7 @@ struct BoxWorld {
     Groups G{};
     int groupBodies = 4096, groupThreads = 1024, groupTail = INT_MAX, groupSplit = 0;
     bool groupLocal = true;  // the groups' motion records in LDS over a sweep's rounds
+    bool groupPhase = true;  // with groupLocal, a whole phase of sweeps, walls too, a launch (k_group_phase)
     int32_t *packStarts = nullptr;  // the packed order's sort: where each (round, group, class) begins
     Colouring C{};
     // The learned warm start (no counterpart in the reference): per-point data, a log of (features, targets), and
@@ -123,6 +124,7 @@ struct BoxWorld {
     vr_profile prof{};
     std::vector<hipEvent_t> sweepEvents;
     std::vector<char> sweepReversed;
+    std::vector<int> sweepWeight;  // the sweeps a timed launch holds
     int sweepsTimed = 0;
     int roundsThisStep = 0;
     int32_t *fellBack = nullptr;    // device: scenes with pairs that kept the carried start (learned_trial)
@@ -156,6 +158,7 @@ struct BoxWorld {
         if (groupLocal) groupBodies = std::min(groupBodies, GROUP_LOCAL_BODIES);
         groupThreads = env_int("VR_BOX_GROUP_THREADS", groupThreads, 64, GROUP_MOST_THREADS) / 64 * 64;
         groupTail = env_int("VR_BOX_GROUP_TAIL", groupTail, 0, INT_MAX);
+        groupPhase = env_int("VR_BOX_GROUP_PHASE", 1, 0, 1) == 1;
         struct Group {
             int scene0, scene1, body0, body1;
         };
@@ -188,6 +191,8 @@ struct BoxWorld {
         HIP_OK(hipMemcpy(G.scenes, sc.data(), sizeof(int32_t) * sc.size(), hipMemcpyHostToDevice));
         size_t ldsBytes = sizeof(float4) * MOTION_VECTORS * (size_t)G.mostBodies + 2 * sizeof(int) * G.mostScenes;
         if (groupLocal && ldsBytes > GROUP_LOCAL_BYTES) groupLocal = false;
+        groupPhase = groupPhase && groupLocal;
+        G.order = alloc<int32_t>(n);  // filled for the shapes as they are (place_walls)
         int32_t *sceneGroup = alloc<int32_t>(scenesTotal);
         HIP_OK(hipMemcpy(sceneGroup, groupOf.data(), sizeof(int32_t) * groupOf.size(), hipMemcpyHostToDevice));
         D.sceneGroup = sceneGroup;
@@ -570,6 +575,7 @@ struct BoxWorld {
         }
         if (move) launch_wall_move(D, D.wallPlace, wallPlaceNext, stream);
         std::swap(D.wallPlace, wallPlaceNext);
+        launch_group_order(D, G, stream);
         HIP_OK(hipStreamSynchronize(stream));
     }
 
@@ -732,8 +738,8 @@ struct BoxWorld {
     }
 
     // Three events for the next timed sweep; layout 0: rounds, then walls; 1: walls, then rounds; 2 and 3: the same
-    // with the rounds in one launch (k_group_rounds).
-    hipEvent_t *timed_sweep(int layout) {
+    // with the rounds in one launch (k_group_rounds); 4: a phase of `weight` sweeps in one launch (k_group_phase).
+    hipEvent_t *timed_sweep(int layout, int weight = 1) {
         size_t at = 3 * (size_t)sweepsTimed;
         while (sweepEvents.size() < at + 3) {
             hipEvent_t e;
@@ -742,12 +748,18 @@ struct BoxWorld {
         }
         if (sweepReversed.size() <= (size_t)sweepsTimed) sweepReversed.resize(sweepsTimed + 1);
         sweepReversed[sweepsTimed] = (char)layout;
+        if (sweepWeight.size() <= (size_t)sweepsTimed) sweepWeight.resize(sweepsTimed + 1);
+        sweepWeight[sweepsTimed] = weight;
         sweepsTimed++;
         return &sweepEvents[at];
     }
 
     // BoxStep.sweep, BoxStep.java:217-231.
     void sweep(int kind, bool reverse, int rounds, int mode = 0, const SweepArgs &a = SweepArgs{0, 1, -1.f, -1}) {
+        if (plan == PLAN_GROUP && groupPhase) {  // a warm sweep as a phase of one (the group plan's solve and relax
+            launch_group_phase(D, G, kind, rounds, 1, false, a.g, a.tol, a.curveAt, groupThreads, stream, mode);
+            return;                               // phases never come here)
+        }
         int layout = (plan == PLAN_GROUP ? 2 : 0) + (reverse ? 1 : 0);
         hipEvent_t *ev = profiling && kind != WARM ? timed_sweep(layout) : nullptr;
         if (plan == PLAN_GROUP) {
@@ -784,11 +796,18 @@ struct BoxWorld {
 
     // After a profiled step (the stream is idle): its sweeps' round and walls times into prof.
     void profile_step(int touching, int points) {
+        int sweeps = 0;
         for (int k = 0; k < sweepsTimed; k++) {
             hipEvent_t *ev = &sweepEvents[3 * (size_t)k];
             float first = 0, second = 0;
             HIP_OK(hipEventElapsedTime(&first, ev[0], ev[1]));
             HIP_OK(hipEventElapsedTime(&second, ev[1], ev[2]));
+            sweeps += sweepWeight[k];
+            if (sweepReversed[k] == 4) {  // a phase in one launch, its walls and rounds together
+                prof.sweep_ns += (int64_t)((double)first * 1e6);
+                prof.sweep_launches++;
+                continue;
+            }
             bool reversed = (sweepReversed[k] & 1) != 0;
             float rounds = reversed ? second : first, walls = reversed ? first : second;
             prof.walls_launches++;
@@ -800,11 +819,11 @@ struct BoxWorld {
             }
             prof.walls_ns += (int64_t)((double)walls * 1e6);
         }
-        prof.passes += sweepsTimed;
+        prof.passes += sweeps;
         prof.round_launches += roundsThisStep;
-        prof.listed_sum += (int64_t)touching * sweepsTimed;
-        prof.points_sum += (int64_t)points * sweepsTimed;
-        prof.bodies_sum += (int64_t)n * sweepsTimed;
+        prof.listed_sum += (int64_t)touching * sweeps;
+        prof.points_sum += (int64_t)points * sweeps;
+        prof.bodies_sum += (int64_t)n * sweeps;
         prof.steps += 1;
         sweepsTimed = 0;
         roundsThisStep = 0;
@@ -828,7 +847,16 @@ struct BoxWorld {
             sweep(kind, d.symmetric && k % 2 == 1, rounds, runMode, a);
         };
         int launched = 0;
-        if (!(mode & SWEEP_STOP) || !looks()) {
+        if (plan == PLAN_GROUP && groupPhase) {  // every sweep of the phase in one launch, each group through them
+            hipEvent_t *ev = profiling ? timed_sweep(4, d.sweeps) : nullptr;
+            if (ev) HIP_OK(hipEventRecord(ev[0], stream));
+            launch_group_phase(D, G, kind, rounds, d.sweeps, d.symmetric != 0, g0, tol,
+                               kept ? at * SOLVE_MOST_SWEEPS : -1, groupThreads, stream, runMode);
+            if (ev) HIP_OK(hipEventRecord(ev[1], stream));
+            if (ev) HIP_OK(hipEventRecord(ev[2], stream));
+            sweepNo += d.sweeps;
+            launched = d.sweeps;
+        } else if (!(mode & SWEEP_STOP) || !looks()) {
             for (; launched < d.sweeps; launched++) one(launched);
         } else {
             int want = std::min(std::max(predicted[at], 1), d.sweeps);
