@@ -114,6 +114,25 @@ def test_the_nearest_old_point_gives_its_words(backend):
 
 
 @pytest.mark.parametrize("backend", BACKENDS)
+def test_of_two_old_points_at_the_same_distance_the_first_gives_its_words(backend):
+    """Old points 0 and 3 both at one anchor, within reach of new point 0: only an old point strictly nearer than
+    those before it is taken, so old point 0's words carry, not point 3's."""
+    with pair(backend) as s:
+        collide_list(s)
+        m = s.floats("manifold")[:M].copy()
+        moved = m[anchor_at(0):anchor_at(0) + 3] + np.array([0.002, 0, 0], np.float32)
+        m[anchor_at(0):anchor_at(0) + 3] = moved
+        m[anchor_at(3):anchor_at(3) + 3] = moved
+        write_slot(s, "manifold", 0, m, M)
+        w = marked()
+        write_slot(s, "impulses", 0, w, WORDS)
+        collide_list(s)
+        got = slot_words(s).reshape(4, 7)
+        assert np.array_equal(got[0].view(np.int32), w.reshape(4, 7)[0].view(np.int32))
+        assert not got[3].any()
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
 def test_one_point_then_and_now_always_carries(backend):
     """A ball on a box: one point. Its old anchor moved far off (a rolling ball's would) still carries; with two old
     points, the nearest rule applies instead."""
