@@ -25,10 +25,12 @@ over the scenes), and the deepest manifold point.
 
 value = learned body-substeps per second = bodies x substeps x K / seconds. The line carries:
   plain, stopped, learned: each run's rate, residuals, deepest point, sweeps;
-  roofline: the colour rounds of a sweep (k_group_rounds, one launch a sweep, on the group plan the tuner picks for
-    the settled batch; k_solve_round, a launch a round, on the others), between HIP events on the world's stream, in
-    the plain window: algorithmic bytes P (236 + 88 k-bar) a sweep over its time (P touching pairs, k-bar points a
-    pair), against MI355X's 8 TB/s; roofline_walls the same for k_walls (96 N + 32 W a sweep);
+  roofline: the kernel that runs the solve and relax sweeps, between HIP events on the world's stream, in the plain
+    window. On the group plan the tuner picks for the settled batch that is k_group_phase, one launch a phase of 10
+    sweeps (each scene group through its walls and colour rounds): algorithmic bytes P (236 + 88 k-bar) + 96 N + 32 W a
+    sweep (P touching pairs, k-bar points a pair, N bodies, W active wall contacts) over its time, against MI355X's
+    8 TB/s. On the other plans the colour rounds (k_solve_round, a launch a round) are the roofline and k_walls
+    (96 N + 32 W a sweep) is roofline_walls;
   cpu_baseline: the oracle (oracle/, a C++ restatement of the reference's passes) on the first scenes of the batch from
     the plain window's state, one thread and all threads (the scenes share nothing: one thread a scene at a time);
   pile_64k: the 65,536-sphere pile (the sphere configuration), the last K1 steps of its 600, with its own roofline.
@@ -56,6 +58,7 @@ STATE = ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw")
 # profiles/traffic_round_kernels_pile_64k.md. None where not measured.
 SOLVE_ROUND_TRAFFIC_BYTES = None
 GROUP_ROUNDS_TRAFFIC_BYTES = None
+GROUP_PHASE_TRAFFIC_BYTES = None
 WALLS_TRAFFIC_BYTES = None
 SPHERE_ROUND_TRAFFIC_BYTES = None
 
@@ -210,42 +213,64 @@ def batch_run(pkg, label, spec, args, net=None, tol=-1.0, profile=False, keep_st
 
 
 def box_roofline(prof, walls_per_sweep):
-    """The colour rounds (k_solve_round, or k_group_rounds on the group plan) and k_walls over the timed sweeps:
-    algorithmic bytes over their event-timed durations."""
+    """The solve and relax sweeps' kernels over the timed sweeps: algorithmic bytes over their event-timed durations.
+    On the group plan with whole phases a launch (k_group_phase: each scene group through a phase's sweeps, its walls
+    and its colour rounds), that one kernel carries both the rounds' P (236 + 88 k-bar) and the walls' 96 N + 32 W a
+    sweep; otherwise the colour rounds (k_solve_round, or k_group_rounds_local a sweep) and k_walls are timed apart
+    and reported apart."""
+    phased = prof.sweep_launches > 0 and prof.walls_launches == 0
     grouped = prof.sweep_launches > 0
     round_bytes = 236.0 * prof.listed_sum + 88.0 * prof.points_sum
+    walls_bytes = 96.0 * prof.bodies_sum + 32.0 * walls_per_sweep * prof.passes
     round_ns = prof.sweep_ns if grouped else prof.round_ns
     launches = prof.sweep_launches if grouped else prof.round_launches
+    kbar = prof.points_sum / max(prof.listed_sum, 1)
+    per_sweep = {"touching_pairs_per_sweep": prof.listed_sum / max(prof.passes, 1), "points_per_pair": kbar,
+                 "bodies_per_sweep": prof.bodies_sum / max(prof.passes, 1), "active_wall_contacts": walls_per_sweep}
+    if phased:
+        total = round_bytes + walls_bytes
+        seconds = round_ns * 1e-9
+        achieved = total / seconds / 1e9 if seconds > 0 else 0.0
+        rounds = {
+            "bound": "latency",
+            "kernel": "k_group_phase (a solve or relax phase of sweeps in one launch: each scene group, a workgroup, "
+                      "through the phase's walls and colour rounds, its motion records in LDS), plain window",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": GROUP_PHASE_TRAFFIC_BYTES,
+            "bytes_per_launch": total / max(launches, 1),
+            "avg_launch_us": round_ns / 1e3 / max(launches, 1),
+            "launches": int(launches), "sweeps": int(prof.passes), **per_sweep,
+            "formula": "P (236 + 88 k-bar) + 96 N + 32 W B a sweep: P touching pairs, k-bar points a pair (the colour "
+                       "rounds), N bodies, W active wall contacts (the walls)",
+        }
+        return rounds, None
     round_s = round_ns * 1e-9
     achieved = round_bytes / round_s / 1e9 if round_s > 0 else 0.0
-    walls_bytes = 96.0 * prof.bodies_sum + 32.0 * walls_per_sweep * prof.passes
     walls_s = prof.walls_ns * 1e-9
     walls_achieved = walls_bytes / walls_s / 1e9 if walls_s > 0 else 0.0
-    kbar = prof.points_sum / max(prof.listed_sum, 1)
     if grouped:
         kernel = ("k_group_rounds_local (a solve or relax sweep's colour rounds in one launch, a workgroup a scene "
                   "group, its motion records in LDS)")
     else:
         kernel = "k_solve_round (solve and relax sweeps' colour rounds, a launch a round)"
     rounds = {
-        "bound": "hbm",
+        "bound": "latency",
         "kernel": kernel + ", plain window",
         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
         "traffic": (GROUP_ROUNDS_TRAFFIC_BYTES if grouped else SOLVE_ROUND_TRAFFIC_BYTES),
         "bytes_per_launch": round_bytes / max(launches, 1),
         "avg_launch_us": round_ns / 1e3 / max(launches, 1),
-        "launches": int(launches), "sweeps": int(prof.passes),
-        "touching_pairs_per_sweep": prof.listed_sum / max(prof.passes, 1), "points_per_pair": kbar,
+        "launches": int(launches), "sweeps": int(prof.passes), **per_sweep,
         "formula": "P (236 + 88 k-bar) B a sweep, P touching pairs, k-bar points a pair",
     }
     walls = {
-        "bound": "hbm",
+        "bound": "latency",
         "kernel": "k_walls (solve and relax sweeps), plain window",
         "achieved": walls_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": walls_achieved / HBM_PEAK_GBS,
         "traffic": WALLS_TRAFFIC_BYTES,
         "bytes_per_launch": walls_bytes / max(prof.walls_launches, 1),
         "avg_launch_us": prof.walls_ns / 1e3 / max(prof.walls_launches, 1),
-        "launches": int(prof.walls_launches), "active_wall_contacts": walls_per_sweep,
+        "launches": int(prof.walls_launches),
         "formula": "96 N + 32 W B a sweep, N bodies, W active wall contacts (normal impulse above zero, after the "
                    "window)",
     }
@@ -317,6 +342,8 @@ WINDOW_KERNELS = {
                        "k_group_rounds_local<2, false, 0>", "k_group_rounds_local<1, true, 0>",
                        "k_group_rounds_local<2, true, 0>"),
     "k_walls": ("k_walls<1, false, 0>", "k_walls<2, false, 0>", "k_walls<1, true, 0>", "k_walls<2, true, 0>"),
+    "k_group_phase": ("k_group_phase<1, 0, false>", "k_group_phase<2, 0, false>", "k_group_phase<1, 0, true>",
+                      "k_group_phase<2, 0, true>"),
 }
 
 
@@ -335,8 +362,10 @@ def summarize(trace, counters, line, out):
     import csv
     d = json.loads(line)
     name = d["roofline"]["kernel"]
-    rounds_kind = "k_group_rounds" if name.startswith("k_group_rounds") else "k_solve_round"
-    want = {rounds_kind: d["roofline"]["launches"], "k_walls": d["roofline_walls"]["launches"]}
+    rounds_kind = next(k for k in ("k_group_phase", "k_group_rounds", "k_solve_round") if name.startswith(k))
+    want = {rounds_kind: d["roofline"]["launches"]}
+    if d.get("roofline_walls"):
+        want["k_walls"] = d["roofline_walls"]["launches"]
     rows = {k: [] for k in want}
     if trace:
         for r in csv.DictReader(open(trace)):

@@ -215,6 +215,11 @@ void launch_group_ranges(const BoxDev &D, const Groups &G, const int32_t *starts
 // the sweeps stopped (not measured) two words a scene of the group of most scenes, all within GROUP_LOCAL_BYTES.
 constexpr int GROUP_LOCAL_BODIES = 5000;
 constexpr size_t GROUP_LOCAL_BYTES = 160 * 1024 - 512;
+// A group phase's dynamic LDS: the motion records (two float4 a body), two words a scene, and the walls list (a
+// 16-bit place a body), for the group of most bodies and the group of most scenes.
+inline size_t group_phase_lds(int mostBodies, int mostScenes) {
+    return 32 * (size_t)mostBodies + 8 * (size_t)mostScenes + 2 * (((size_t)mostBodies + 1) / 2 * 2);
+}
 // A phase of `sweeps` sweeps numbered g0 ... (a warm sweep: one), each its walls and its `used` rounds in the order of
 // launch_walls and launch_solve_round, a workgroup of `threads` a group through all of them, its motion records and
 // its scenes' stop words in LDS (k_group_phase). mode and tol as for the launches; curveBase: where a measured

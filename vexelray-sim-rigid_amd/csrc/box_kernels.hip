@@ -1131,55 +1131,165 @@ __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_rounds_local(BoxDe
 // waiting for the batch's slowest group at every sweep and every walls launch. Each scene's stop is decided as the
 // launches decide it (sweeping, record_one), from the scene's own words in LDS.
 
-// Group g's bodies, boxes first, then balls, each kind in body order (a stable partition by shape over the group), so
-// that the waves of a group's walls run one path.
+// The order a group's walls take its bodies in: boxes, then balls, so that a wave runs one path; within each kind,
+// first the bodies that may do something at a wall in this step (a corner or wall whose bit in cornerHeld is set, or
+// the body within reach of a wall, the margin and a quarter of its own reach), then the rest. A body's walls read and
+// write only that body, so the order changes no result. A wave's time in the walls is that of its busiest lane, and
+// most bodies of a pile are nowhere near a wall: kept together, the bodies with something to do fill a few waves
+// and the others' waves pass over the corner loop. 0: active boxes, 1: other boxes, 2: active balls, 3: other balls.
+__device__ __forceinline__ int walls_class(const BoxDev &D, int s) {
+    bool ball = D.shape[s] > 0.5f;  // Moving::sphere
+    bool active = false;
+    if (D.im[s] > 0) {  // a fixed body's walls return at once
+        active = (D.cornerHeld[s] & (ball ? (1ull << WALLS) - 1ull : ~0ull)) != 0ull;
+        if (!active) {
+            float h0 = D.half[0][s], h1 = D.half[1][s], h2 = D.half[2][s];
+            float reach = ball ? h0 : sqrtf((h0 * h0 + h1 * h1) + h2 * h2);
+            const SceneDev sc = D.sceneInfo[D.scene[s]];
+            float extent[3] = {sc.sx, sc.sy, sc.sz};
+            float near = reach + (D.params[MARGIN] + 0.25f * reach);
+            for (int a = 0; a < 3; a++) {
+                float c = D.x[a][s];
+                active = active || !(c >= near) || !(extent[a] - c >= near);  // NaN: active
+            }
+        }
+    }
+    return (ball ? 2 : 0) + (active ? 0 : 1);
+}
+
+// Group g's bodies in the order of their walls class (walls_class), each class in body order: a stable partition
+// over the group, into G.order.
 __global__ void __launch_bounds__(THREADS) k_group_order(BoxDev D, Groups G) {
-    __shared__ int sWave[THREADS / 64];
+    constexpr int CLASSES = 4, WAVES = THREADS / 64;
+    __shared__ int sWave[CLASSES][WAVES];
     int g = blockIdx.x;
     int b0 = G.bodies[2 * g], b1 = G.bodies[2 * g + 1];
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int boxes = 0;  // the group's boxes, the same in every thread
+    int total[CLASSES] = {0, 0, 0, 0};  // the group's bodies of each class, the same in every thread
     for (int base = b0; base < b1; base += THREADS) {
         int s = base + (int)threadIdx.x;
-        unsigned long long mask = __ballot(s < b1 && !(D.shape[s] > 0.5f));  // Moving::sphere
-        if (lane == 0) sWave[wave] = __popcll(mask);
-        __syncthreads();
-        for (int w = 0; w < THREADS / 64; w++) boxes += sWave[w];
-        __syncthreads();
-    }
-    int boxesDone = 0, ballsDone = 0;  // before this chunk, the same in every thread
-    for (int base = b0; base < b1; base += THREADS) {
-        int s = base + (int)threadIdx.x;
-        bool in = s < b1;
-        bool box = in && !(D.shape[s] > 0.5f);
-        unsigned long long boxMask = __ballot(box);
-        if (lane == 0) sWave[wave] = __popcll(boxMask);
-        __syncthreads();
-        int before = 0, chunkBoxes = 0;
-        for (int w = 0; w < THREADS / 64; w++) {
-            before += w < wave ? sWave[w] : 0;
-            chunkBoxes += sWave[w];
+        int c = s < b1 ? walls_class(D, s) : -1;
+        for (int k = 0; k < CLASSES; k++) {
+            unsigned long long mask = __ballot(c == k);
+            if (lane == 0) sWave[k][wave] = __popcll(mask);
         }
         __syncthreads();
-        int boxesBefore = before + __popcll(boxMask & ((1ull << lane) - 1ull));
-        int ballsBefore = (int)threadIdx.x - boxesBefore;  // the threads before this one are all in the group
-        if (in) G.order[b0 + (box ? boxesDone + boxesBefore : boxes + ballsDone + ballsBefore)] = s;
-        boxesDone += chunkBoxes;
-        ballsDone += min(THREADS, b1 - base) - chunkBoxes;
+        for (int k = 0; k < CLASSES; k++)
+            for (int w = 0; w < WAVES; w++) total[k] += sWave[k][w];
+        __syncthreads();
+    }
+    int done[CLASSES];  // where each class's next body goes, the same in every thread
+    done[0] = b0;
+    for (int k = 1; k < CLASSES; k++) done[k] = done[k - 1] + total[k - 1];
+    for (int base = b0; base < b1; base += THREADS) {
+        int s = base + (int)threadIdx.x;
+        int c = s < b1 ? walls_class(D, s) : -1;
+        int mine = 0;
+        for (int k = 0; k < CLASSES; k++) {
+            unsigned long long mask = __ballot(c == k);
+            if (lane == 0) sWave[k][wave] = __popcll(mask);
+            if (c == k) mine = __popcll(mask & ((1ull << lane) - 1ull));
+        }
+        __syncthreads();
+        int chunk[CLASSES];
+        for (int k = 0; k < CLASSES; k++) {
+            int before = 0, all = 0;
+            for (int w = 0; w < WAVES; w++) {
+                before += w < wave ? sWave[k][w] : 0;
+                all += sWave[k][w];
+            }
+            if (c == k) mine += before;
+            chunk[k] = all;
+        }
+        __syncthreads();
+        if (c >= 0) G.order[(c == 0 ? done[0] : c == 1 ? done[1] : c == 2 ? done[2] : done[3]) + mine] = s;
+        for (int k = 0; k < CLASSES; k++) done[k] += chunk[k];
     }
 }
 
-// A group's walls in a sweep: k_walls's work for the group's bodies, a thread a body (boxes first), the motion in
-// LDS. Every thread runs the same number of iterations, so the waves' residual records see whole waves.
+// What body s's walls do over a phase of sweeps (walls_of), found at the phase's start: 0 nothing, 1 something, 2 a
+// ball touching none of its walls, whose walls only add +0 to its momentum. Over a phase, positions and turns stay
+// as they are (only velocities and momenta change), and a corner (or a ball's wall) comes to hold an impulse only
+// where the sweep takes it: a box's listed corners, a ball's walls that hold or lie within the margin. So a box with
+// no corner listed at the phase's start (corner_list: none holding, none within the margin) has none listed in any
+// of its sweeps, and its walls leave it as it is; a ball whose walls are all quiet and beyond the margin, with no
+// rolling friction (Boxes.roll's turning stop needs it), has each wall turn it by +0 in every sweep (ball_walls),
+// which changes a momentum word of -0 to +0 and nothing else. A fixed body returns at once.
+template <int SWEEP>
+__device__ __forceinline__ int walls_kind(const BoxDev &D, int s) {
+    if (!(D.im[s] > 0)) return 0;
+    const float *P = D.params;
+    unsigned long long held = D.cornerHeld[s];
+    const SceneDev sc = D.sceneInfo[D.scene[s]];
+    float extent[3] = {sc.sx, sc.sy, sc.sz};
+    float at[3] = {D.x[0][s], D.x[1][s], D.x[2][s]};
+    float half[3] = {D.half[0][s], D.half[1][s], D.half[2][s]};
+    Soft soft(P, true);
+    if (D.shape[s] > 0.5f) {  // Moving::sphere: a bit a wall (the word's other bits mean nothing for a ball)
+        if ((held & ((1ull << WALLS) - 1ull)) != 0ull || P[MU_ROLLING] > 0) return 1;
+        if (SWEEP == WARM) return 2;
+        for (int wall = 0; wall < WALLS; wall++) {  // ball_walls's gap at each wall
+            int axis = wall / 2;
+            bool high = wall % 2 == 1;
+            float nrm[3], r[3];
+            for (int a = 0; a < 3; a++) nrm[a] = a == axis ? (high ? -1.0f : 1.0f) : 0.0f;
+            for (int a = 0; a < 3; a++) r[a] = -(half[0] * nrm[a]);
+            float reach = pick(at, axis) + pick(r, axis);
+            float gap = high ? pick(extent, axis) - reach : reach;
+            if (gap < soft.margin) return 1;
+        }
+        return 2;
+    }
+    float q[4] = {D.q[0][s], D.q[1][s], D.q[2][s], D.q[3][s]};
+    return corner_list<SWEEP, false>(at, q, half, extent, held, soft) != 0ull ? 1 : 0;
+}
+
+// A group's bodies by what their walls do in a phase: `list` holds the bodies of kind 1 at 0 .. swept - 1 and those
+// of kind 2 at size - 1 down to size - still, each as its place past the group's first body.
+struct WallList {
+    unsigned short *list;
+    int size, swept, still;
+};
+
+// Lists the group's bodies for its walls over a phase (walls_kind), taking them in the walls' order (G.order) so
+// that a wave's bodies are mostly of one shape. Called by every thread of the workgroup; ends with a barrier.
+template <int SWEEP>
+__device__ void list_walls(const BoxDev &D, const Groups &G, int b0, int b1, WallList &W, int *counts) {
+    if (threadIdx.x == 0) counts[0] = counts[1] = 0;
+    __syncthreads();
+    int lane = threadIdx.x & 63;
+    unsigned long long below = (1ull << lane) - 1ull;
+    for (int base = b0; base < b1; base += (int)blockDim.x) {
+        int p = base + (int)threadIdx.x;
+        int s = p < b1 ? G.order[p] : -1;
+        int kind = s >= 0 ? walls_kind<SWEEP>(D, s) : 0;
+        unsigned long long swept = __ballot(kind == 1), still = __ballot(kind == 2);
+        int at1 = 0, at2 = 0;
+        if (lane == 0 && swept) at1 = atomicAdd(&counts[0], __popcll(swept));
+        if (lane == 0 && still) at2 = atomicAdd(&counts[1], __popcll(still));
+        at1 = __builtin_amdgcn_readfirstlane(at1);
+        at2 = __builtin_amdgcn_readfirstlane(at2);
+        if (kind == 1) W.list[at1 + __popcll(swept & below)] = (unsigned short)(s - b0);
+        if (kind == 2) W.list[W.size - 1 - (at2 + __popcll(still & below))] = (unsigned short)(s - b0);
+    }
+    __syncthreads();
+    W.swept = counts[0];
+    W.still = counts[1];
+}
+
+// A group's walls in a sweep: k_walls's work for the group's bodies that have some (WallList), a thread a body, the
+// motion in LDS; each still ball's momentum words +0. Every thread runs the same number of iterations, so the waves'
+// residual records see whole waves. Measured, each scene taking part records a residual of 0 as well, which its
+// bodies of no walls work would have recorded (a scene's record is the largest of its bodies').
 template <int SWEEP, bool REVERSE, int MODE>
-__device__ void group_walls(const BoxDev &D0, const Groups &G, int b0, int b1, const SweepArgs &a,
+__device__ void group_walls(const BoxDev &D0, const Groups &G, int b0, int s1, const WallList &W, const SweepArgs &a,
                             const LocalMotion &motion, const LocalStop &stop) {
     BoxDev D = D0;
 #pragma unroll 1
-    for (int base = b0; base < b1; base += (int)blockDim.x) {
+    for (int base = 0; base < W.swept; base += (int)blockDim.x) {
         asm volatile("" : "+s"(D.n));  // the wall words' addresses worked out in the iteration, as in group_round
         int p = base + (int)threadIdx.x;
-        int s = p < b1 ? G.order[p] : -1;
+        int s = p < W.swept ? b0 + (int)W.list[p] : -1;
         if (MODE == 0) {
             if (s >= 0) walls_of<SWEEP, REVERSE, true>(D, s, D.wallPlace[s], nullptr, &motion);
             continue;
@@ -1191,16 +1301,28 @@ __device__ void group_walls(const BoxDev &D0, const Groups &G, int b0, int b1, c
         if ((MODE & SWEEP_STOP) && live && !(res <= a.tol)) stop.open[scene - stop.s0] = 1;
         if (MODE & SWEEP_MEASURE) record<SWEEP_MEASURE>(D, a, live, scene, res);
     }
+    for (int p = (int)threadIdx.x; p < W.still; p += (int)blockDim.x) {
+        int s = b0 + (int)W.list[W.size - 1 - p];
+        if ((MODE & SWEEP_STOP) && !stop.live[D.scene[s] - stop.s0]) continue;
+        float4 *mom = motion.at(s) + 1;
+        float4 m = *mom;
+        float x = m.x + 0.f, y = m.y + 0.f, z = m.z + 0.f;  // Moving::turn by +0 (store_motion_changed_at)
+        if ((bits(x) != bits(m.x)) | (bits(y) != bits(m.y)) | (bits(z) != bits(m.z))) *mom = make_float4(x, y, z, 0.f);
+    }
+    if (MODE & SWEEP_MEASURE)
+        for (int j = (int)threadIdx.x; j < s1 - stop.s0; j += (int)blockDim.x)
+            if (!(MODE & SWEEP_STOP) || stop.live[j]) record_one<SWEEP_MEASURE>(D, a, stop.s0 + j, 0u);
 }
 
 // One sweep of a group: walls, then rounds in reverse with REVERSE; rounds, then walls otherwise (BoxStep.sweep,
 // BoxStep.java:217-231). Ends with a barrier.
 template <int SWEEP, bool REVERSE, int MODE>
-__device__ void group_sweep(const BoxDev &D, const Groups &G, int b0, int b1, int used, const int *from,
-                            const int *to, const SweepArgs &a, const LocalMotion &motion, const LocalStop &stop) {
+__device__ void group_sweep(const BoxDev &D, const Groups &G, int b0, int s1, const WallList &W, int used,
+                            const int *from, const int *to, const SweepArgs &a, const LocalMotion &motion,
+                            const LocalStop &stop) {
     const LocalStop *stops = (MODE & SWEEP_STOP) ? &stop : nullptr;
     if (REVERSE) {
-        group_walls<SWEEP, REVERSE, MODE>(D, G, b0, b1, a, motion, stop);
+        group_walls<SWEEP, REVERSE, MODE>(D, G, b0, s1, W, a, motion, stop);
         __syncthreads();
     }
 #pragma unroll 1
@@ -1210,7 +1332,7 @@ __device__ void group_sweep(const BoxDev &D, const Groups &G, int b0, int b1, in
         group_round<SWEEP, REVERSE, MODE>(D, from[r], to[r], a, motion, stops);
         __syncthreads();
     }
-    if (!REVERSE) group_walls<SWEEP, REVERSE, MODE>(D, G, b0, b1, a, motion, stop);
+    if (!REVERSE) group_walls<SWEEP, REVERSE, MODE>(D, G, b0, s1, W, a, motion, stop);
     __syncthreads();
 }
 
@@ -1222,9 +1344,11 @@ __device__ void group_sweep(const BoxDev &D, const Groups &G, int b0, int b1, in
 template <int SWEEP, int MODE, bool SYMMETRIC>
 __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_phase(BoxDev D, Groups G, int used, int sweeps, int g0,
                                                                     float tol, int curveBase) {
-    extern __shared__ float4 sMotion[];  // the group's motion records, then two words a scene of the group
+    // The group's motion records, then two words a scene of the group, then its walls list (group_phase_lds).
+    extern __shared__ float4 sMotion[];
     __shared__ int from[MOST_ROUNDS], to[MOST_ROUNDS];
     __shared__ int anyLive;
+    __shared__ int wallCounts[2];
     int g = blockIdx.x;
     int b0 = G.bodies[2 * g], b1 = G.bodies[2 * g + 1];
     int s0 = G.scenes[2 * g], s1 = G.scenes[2 * g + 1];
@@ -1237,8 +1361,9 @@ __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_phase(BoxDev D, Gr
     for (int k = threadIdx.x; k < words; k += blockDim.x) sMotion[k] = in[k];
     int *live = reinterpret_cast<int *>(sMotion + (size_t)G.mostBodies * MOTION_VECTORS);
     int *open = live + G.mostScenes;
+    WallList W{reinterpret_cast<unsigned short *>(open + G.mostScenes), b1 - b0, 0, 0};
+    list_walls<SWEEP>(D, G, b0, b1, W, wallCounts);  // ends with a barrier, after which the records are in
     constexpr bool stopped = (MODE & SWEEP_STOP) != 0;
-    __syncthreads();
     LocalMotion motion{sMotion, b0};
     LocalStop stop{s0, live, open};
 #pragma unroll 1
@@ -1257,9 +1382,9 @@ __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_phase(BoxDev D, Gr
             if (!anyLive) break;  // no scene of the group takes part in this sweep or any after it
         }
         if (SYMMETRIC && k % 2 == 1)
-            group_sweep<SWEEP, true, MODE>(D, G, b0, b1, used, from, to, a, motion, stop);
+            group_sweep<SWEEP, true, MODE>(D, G, b0, s1, W, used, from, to, a, motion, stop);
         else
-            group_sweep<SWEEP, false, MODE>(D, G, b0, b1, used, from, to, a, motion, stop);
+            group_sweep<SWEEP, false, MODE>(D, G, b0, s1, W, used, from, to, a, motion, stop);
         if (stopped)  // the world's stop words as the launches leave them
             for (int j = threadIdx.x; j < s1 - s0; j += blockDim.x)
                 if (open[j]) D.sw.sceneOpen[s0 + j] = a.g;
@@ -2010,9 +2135,9 @@ void launch_group_phase(const BoxDev &D, const Groups &G, int sweep, int used, i
                         float tol, int curveBase, int threads, hipStream_t s, int mode) {
     if (G.count <= 0 || sweeps <= 0) return;
     if (used < 0 || used > MOST_ROUNDS) throw std::runtime_error("group phase: rounds out of range");
-    size_t lds = sizeof(float4) * MOTION_VECTORS * (size_t)G.mostBodies + 2 * sizeof(int) * (size_t)G.mostScenes;
+    size_t lds = group_phase_lds(G.mostBodies, G.mostScenes);
     if (G.mostBodies < 1 || G.mostBodies > GROUP_LOCAL_BODIES || G.mostScenes < 1 || lds > GROUP_LOCAL_BYTES)
-        throw std::runtime_error("group phase: a group's motion records and stop words do not fit in LDS");
+        throw std::runtime_error("group phase: a group's motion records, stop words and walls list do not fit in LDS");
     if (threads < 64 || threads > GROUP_MOST_THREADS || threads % 64 != 0)
         throw std::runtime_error("group phase: a workgroup of whole waves, at most GROUP_MOST_THREADS threads");
     if (sweep == WARM) {

@@ -191,7 +191,7 @@ struct BoxWorld {
         HIP_OK(hipMemcpy(G.scenes, sc.data(), sizeof(int32_t) * sc.size(), hipMemcpyHostToDevice));
         size_t ldsBytes = sizeof(float4) * MOTION_VECTORS * (size_t)G.mostBodies + 2 * sizeof(int) * G.mostScenes;
         if (groupLocal && ldsBytes > GROUP_LOCAL_BYTES) groupLocal = false;
-        groupPhase = groupPhase && groupLocal;
+        groupPhase = groupPhase && groupLocal && group_phase_lds(G.mostBodies, G.mostScenes) <= GROUP_LOCAL_BYTES;
         G.order = alloc<int32_t>(n);  // filled for the shapes as they are (place_walls)
         int32_t *sceneGroup = alloc<int32_t>(scenesTotal);
         HIP_OK(hipMemcpy(sceneGroup, groupOf.data(), sizeof(int32_t) * groupOf.size(), hipMemcpyHostToDevice));
@@ -1005,6 +1005,9 @@ struct BoxWorld {
                     groupSplit = used;
                     while (groupSplit > 0 && roundPairs[groupSplit - 1] <= groupTail) groupSplit--;
                     launch_group_ranges(D, G, packStarts, used, stream);
+                    // The walls' order of each group's bodies, from where the bodies start the step and what they
+                    // hold (k_group_order): the ones with something to do at a wall kept together.
+                    if (groupPhase) launch_group_order(D, G, stream);
                 }
             }
             for (int sub = 0; sub < d.substeps; sub++) {  // the sweeps work on the frame and motion records
