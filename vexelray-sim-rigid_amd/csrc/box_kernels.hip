@@ -72,16 +72,17 @@ __device__ __forceinline__ void position_of(const BoxDev &D, int s) {
     Moving m = load_moving(B, s);
     float cq[4], own[3], omega[3];
     conjugate(m.q, cq);
-    rotate(cq, m.mom, own);
+    rotate_scaled(cq, m.twice, m.mom, own);
     for (int a = 0; a < 3; a++) own[a] = m.inverse[a] * own[a];
-    rotate(m.q, own, omega);
+    rotate_scaled(m.q, m.twice, own, omega);
     float halfH = 0.5f * h;
     float mid[4], omegaMid[3], end[4], cm[4];
     turned(m.q, omega, halfH, mid);
     conjugate(mid, cm);
-    rotate(cm, m.mom, own);
+    float twiceMid = turn_scale(mid);
+    rotate_scaled(cm, twiceMid, m.mom, own);
     for (int a = 0; a < 3; a++) own[a] = m.inverse[a] * own[a];
-    rotate(mid, own, omegaMid);
+    rotate_scaled(mid, twiceMid, own, omegaMid);
     turned(m.q, omegaMid, h, end);
     for (int k = 0; k < 4; k++) B.q[k][s] = end[k];
 }
@@ -95,7 +96,9 @@ __global__ void k_positions(BoxDev D, int frame) {
     if (frame && D.im[s] > 0) {
         float4 *f = D.frame + (size_t)s * FRAME_VECTORS;
         f[0] = make_float4(D.x[0][s], D.x[1][s], D.x[2][s], D.im[s]);
-        f[1] = make_float4(D.q[0][s], D.q[1][s], D.q[2][s], D.q[3][s]);
+        float q[4] = {D.q[0][s], D.q[1][s], D.q[2][s], D.q[3][s]};
+        f[1] = make_float4(q[0], q[1], q[2], q[3]);
+        f[3].w = turn_scale(q);
     }
 }
 
@@ -209,6 +212,7 @@ __device__ unsigned long long corner_list(const float *at, const float *q, const
                                           unsigned long long held, const Soft &soft) {
     const Sweep sweep = (Sweep)SWEEP;
     float diagonal = sqrtf((half[0] * half[0] + half[1] * half[1]) + half[2] * half[2]);
+    float twice = turn_scale(q);  // the conjugate's too
     unsigned long long todo = 0;
     for (int order = 0; order < WALLS; order++) {
         int wall = REVERSE ? WALLS - 1 - order : order;
@@ -236,7 +240,7 @@ __device__ unsigned long long corner_list(const float *at, const float *q, const
         along[1] = axis == 1 ? 1.f : 0.f;
         along[2] = axis == 2 ? 1.f : 0.f;
         conjugate(q, cq);
-        rotate(cq, along, row);
+        rotate_scaled(cq, twice, along, row);
         for (int k = 0; k < CORNERS; k++) {
             int c = REVERSE ? CORNERS - 1 - k : k;
             float offset[3];
@@ -277,7 +281,7 @@ __device__ unsigned long long corner_sweep(const Bufs &B, Moving &box, const flo
             int b = (c / (1 << a)) % 2;
             offset[a] = box.half[a] * (2.0f * (float)b - 1.0f);
         }
-        rotate(box.q, offset, r);
+        box.turned_by(offset, r);
         float reach = pick(box.at, axis) + pick(r, axis);
         float gap = high ? pick(extent, axis) - reach : reach;
         // A corner holding nothing and outside the margin returns at once from Boxes.contact.
@@ -448,8 +452,8 @@ __device__ void solve_pair(const BoxDev &D, int pair, float *res = nullptr) {
     auto point = [&](HeldPoint &p, int slot) {
         if (!(slot < count)) return;
         float rA[3], rB[3];
-        rotate(first.q, p.anchorA, rA);
-        rotate(second.q, p.anchorB, rB);
+        first.turned_by(p.anchorA, rA);
+        second.turned_by(p.anchorB, rB);
         if (first.sphere())
             for (int a = 0; a < 3; a++) rA[a] = first.half[0] * nrm[a];
         if (second.sphere())
@@ -706,7 +710,7 @@ __device__ void solve_pair_side(const BoxDev &D, int j, bool second, float *res,
     auto point = [&](HeldPoint &p, int slot) {
         if (!(slot < count)) return;
         float r[3];
-        rotate(S.me.q, second ? p.anchorB : p.anchorA, r);
+        S.me.turned_by(second ? p.anchorB : p.anchorA, r);
         if (S.me.sphere())
             for (int a = 0; a < 3; a++) r[a] = second ? -(S.me.half[0] * nrm[a]) : S.me.half[0] * nrm[a];
         float at[3];
@@ -787,7 +791,8 @@ __global__ void k_frame_in(BoxDev D, int motion) {
     f[0] = make_float4(D.x[0][s], D.x[1][s], D.x[2][s], im);
     f[1] = make_float4(D.q[0][s], D.q[1][s], D.q[2][s], D.q[3][s]);
     f[2] = make_float4(half[0], half[1], half[2], shape);
-    f[3] = make_float4(inverse[0], inverse[1], inverse[2], 0.f);
+    float q[4] = {D.q[0][s], D.q[1][s], D.q[2][s], D.q[3][s]};
+    f[3] = make_float4(inverse[0], inverse[1], inverse[2], turn_scale(q));
     if (motion) {
         float4 *v = D.motion + (size_t)s * MOTION_VECTORS;
         v[0] = make_float4(D.vel[0][s], D.vel[1][s], D.vel[2][s], 0.f);
@@ -1732,8 +1737,8 @@ __device__ __forceinline__ PointAt point_at(const BoxDev &D, const Bufs &B, int 
     P.first = load_moving(B, ia);
     P.second = load_moving(B, ib);
     size_t where = out + POINTS + k * POINT_WORDS;
-    rotate(P.first.q, D.manifold + where + ANCHOR_FIRST, P.rA);
-    rotate(P.second.q, D.manifold + where + ANCHOR_SECOND, P.rB);
+    P.first.turned_by(D.manifold + where + ANCHOR_FIRST, P.rA);
+    P.second.turned_by(D.manifold + where + ANCHOR_SECOND, P.rB);
     if (P.first.sphere())
         for (int a = 0; a < 3; a++) P.rA[a] = P.first.half[0] * P.n[a];
     if (P.second.sphere())

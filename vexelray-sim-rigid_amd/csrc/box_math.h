@@ -54,9 +54,15 @@ __device__ __forceinline__ void cross(const float *x, const float *y, float *z) 
     z[2] = c;
 }
 
-// Boxes.rotate, Boxes.java:907-921: p turned by q, p + w t + q x t with t = 2 q x p / |q|^2.
-__device__ __forceinline__ void rotate(const float *q, const float *p, float *out) {
-    float twice = 2.0f / ((q[0] * q[0] + q[1] * q[1]) + (q[2] * q[2] + q[3] * q[3]));
+// 2 / |q|^2, the scale of Boxes.rotate's t. A turn's conjugate gives the same bits (each square is of a negated
+// word, which squares to the same bits), so a body whose turn stays as it is over a sweep works it out once.
+__device__ __forceinline__ float turn_scale(const float *q) {
+    return 2.0f / ((q[0] * q[0] + q[1] * q[1]) + (q[2] * q[2] + q[3] * q[3]));
+}
+
+// Boxes.rotate, Boxes.java:907-921: p turned by q, p + w t + q x t with t = 2 q x p / |q|^2, given twice =
+// turn_scale(q).
+__device__ __forceinline__ void rotate_scaled(const float *q, float twice, const float *p, float *out) {
     float t[3], qt[3];
     cross(q, p, t);
     for (int a = 0; a < 3; a++) t[a] = twice * t[a];
@@ -64,6 +70,10 @@ __device__ __forceinline__ void rotate(const float *q, const float *p, float *ou
     float r[3];
     for (int a = 0; a < 3; a++) r[a] = p[a] + (q[3] * t[a] + qt[a]);
     for (int a = 0; a < 3; a++) out[a] = r[a];
+}
+
+__device__ __forceinline__ void rotate(const float *q, const float *p, float *out) {
+    rotate_scaled(q, turn_scale(q), p, out);
 }
 
 __device__ __forceinline__ void conjugate(const float *q, float *out) {
@@ -94,6 +104,9 @@ struct Moving {
     int s;
     float shape, im;
     float at[3], q[4], half[3], inverse[3], vel[3], mom[3], omega[3];
+    float twice;  // turn_scale(q), for every turn of a vector by q or its conjugate
+
+    __device__ void turned_by(const float *p, float *out) const { rotate_scaled(q, twice, p, out); }
 
     __device__ bool sphere() const { return shape > 0.5f; }
 
@@ -101,9 +114,9 @@ struct Moving {
     __device__ void inverseInertiaTimes(const float *x, float *out) const {
         float cq[4], own[3];
         conjugate(q, cq);
-        rotate(cq, x, own);
+        rotate_scaled(cq, twice, x, own);
         for (int a = 0; a < 3; a++) own[a] = inverse[a] * own[a];
-        rotate(q, own, out);
+        rotate_scaled(q, twice, own, out);
     }
 
     __device__ float inverseMassAlong(const float *r, const float *n) const {  // Boxes.java:716-719
@@ -168,6 +181,7 @@ __device__ Moving load_moving(const Bufs &B, int s) {
         m.mom[a] = B.l[a][s];
     }
     for (int k = 0; k < 4; k++) m.q[k] = B.q[k][s];
+    m.twice = turn_scale(m.q);
     inverse_inertia(m.half, m.im, m.shape, m.inverse);
     m.inverseInertiaTimes(m.mom, m.omega);
     return m;
@@ -190,7 +204,7 @@ __device__ void store_moving_changed(const Bufs &B, const Moving &m, const Movin
 
 // ---- a body as the sweeps of the pass-by-pass step hold it: two records a body ----
 //
-// frame, 4 float4 a body: (at, im), q, (half, shape), (inverse, 0); fixed over a run of sweeps (positions rewrite
+// frame, 4 float4 a body: (at, im), q, (half, shape), (inverse, turn_scale(q)); fixed over a run of sweeps (positions rewrite
 // it). motion, 2 float4 a body: (vel, 0), (mom, 0); what the sweeps change. A round's bodies lie anywhere in the
 // world, so it gathers each from these records, two cache lines, not from the state's 18 fields. (One 128-byte record
 // a body, frame and motion in one line, measured slower on the 2,048-scene batch: 116 against 111 ms of colour
@@ -207,6 +221,7 @@ __device__ __forceinline__ Moving load_framed_at(const float4 *frame, const floa
     m.q[0] = b.x, m.q[1] = b.y, m.q[2] = b.z, m.q[3] = b.w;
     m.half[0] = c.x, m.half[1] = c.y, m.half[2] = c.z, m.shape = c.w;
     m.inverse[0] = d.x, m.inverse[1] = d.y, m.inverse[2] = d.z;
+    m.twice = d.w;  // turn_scale(q), kept in the record beside the inverse inertia
     m.vel[0] = e.x, m.vel[1] = e.y, m.vel[2] = e.z;
     m.mom[0] = g.x, m.mom[1] = g.y, m.mom[2] = g.z;
     m.inverseInertiaTimes(m.mom, m.omega);
@@ -243,6 +258,7 @@ __device__ __forceinline__ Moving load_moved_at(const Bufs &B, const float4 *v, 
         m.half[a] = B.half[a][s];
     }
     for (int k = 0; k < 4; k++) m.q[k] = B.q[k][s];
+    m.twice = turn_scale(m.q);
     m.vel[0] = e.x, m.vel[1] = e.y, m.vel[2] = e.z;
     m.mom[0] = g.x, m.mom[1] = g.y, m.mom[2] = g.z;
     inverse_inertia(m.half, m.im, m.shape, m.inverse);
