@@ -54,11 +54,10 @@ HBM_PEAK_GBS = 8000.0
 STATE = ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw")
 
 # HBM bytes a launch from rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, each scaled by what it reads for a
-# copy of known size), over the same windows: profiles/traffic_box_kernels_batched_2k.md and
-# profiles/traffic_round_kernels_pile_64k.md. None where not measured.
+# copy of known size), over the same window as the bench line's roofline. None where not measured for that window.
 SOLVE_ROUND_TRAFFIC_BYTES = None
 GROUP_ROUNDS_TRAFFIC_BYTES = None
-GROUP_PHASE_TRAFFIC_BYTES = None
+GROUP_PHASE_TRAFFIC_BYTES = 13_701_145_525  # k_group_phase, plain window (profiles/traffic_box_kernels_batched_2k.md)
 WALLS_TRAFFIC_BYTES = None
 SPHERE_ROUND_TRAFFIC_BYTES = None
 
@@ -232,7 +231,9 @@ def box_roofline(prof, walls_per_sweep):
         seconds = round_ns * 1e-9
         achieved = total / seconds / 1e9 if seconds > 0 else 0.0
         rounds = {
-            "bound": "latency",
+            "bound": "hbm",
+            "limited_by": "VALU issue: the phase kernel's vector ALU is busy about 80 % of each SIMD's time (SQ counters, "
+                          "profiles/valu_group_phase_batched_2k.md); no MFMA work",
             "kernel": "k_group_phase (a solve or relax phase of sweeps in one launch: each scene group, a workgroup, "
                       "through the phase's walls and colour rounds, its motion records in LDS), plain window",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -254,7 +255,7 @@ def box_roofline(prof, walls_per_sweep):
     else:
         kernel = "k_solve_round (solve and relax sweeps' colour rounds, a launch a round)"
     rounds = {
-        "bound": "latency",
+        "bound": "hbm",
         "kernel": kernel + ", plain window",
         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
         "traffic": (GROUP_ROUNDS_TRAFFIC_BYTES if grouped else SOLVE_ROUND_TRAFFIC_BYTES),
@@ -264,7 +265,7 @@ def box_roofline(prof, walls_per_sweep):
         "formula": "P (236 + 88 k-bar) B a sweep, P touching pairs, k-bar points a pair",
     }
     walls = {
-        "bound": "latency",
+        "bound": "hbm",
         "kernel": "k_walls (solve and relax sweeps), plain window",
         "achieved": walls_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": walls_achieved / HBM_PEAK_GBS,
         "traffic": WALLS_TRAFFIC_BYTES,
@@ -510,7 +511,9 @@ def sphere_pile(pkg, steps, warmup, run=600):
         "contacts_at_start": contacts_start, "contacts_at_end": contacts_end,
         "rounds_per_step": float(np.mean([r.rounds for r in reports])),
         "roofline": {
-            "bound": "latency",
+            "bound": "hbm",
+            "limited_by": "dependent latency, launches and device-scope claim atomics (the world sits in the 256 MB "
+                          "Infinity Cache)",
             "kernel": "Gauss-Seidel round kernels (k_round_wide, k_round_tail), per pass",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": SPHERE_ROUND_TRAFFIC_BYTES,

@@ -518,3 +518,18 @@ def test_the_picture_is_given_each_step_and_the_one_before():
                 assert shown[base + turned_before + k] == earlier[base + turned_after + k], (b, k)
         assert shown[centre_after + 1] < shown[centre_before + 1], "the first box is falling"
         assert shown[turned_after] != shown[turned_before], "and turning"
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("pairs", [[0, 2], [-1, 0], [1, 0, 0, 7]])
+def test_the_narrow_phase_refuses_a_body_out_of_the_world(backend, pairs):
+    """vr_box_collide names each pair's bodies; one outside 0 .. bodies - 1 is an IllegalArgumentException, found
+    before anything runs (the kernel loads both bodies of every pair as given)."""
+    s = BoxScene(2).uniform(0.1, 0.1, 0.1, 1.0)
+    s.x[:] = 0.5
+    s.y[:] = [0.1, 0.25]
+    s.z[:] = 0.5
+    with s.start(backend):
+        with pytest.raises(sc.pkg.IllegalArgumentException, match="names body"):
+            s.sim.collide(np.array(pairs, dtype=np.int32))
+        assert s.sim.collide(np.array([0, 1], dtype=np.int32)).shape[0] == BoxContact.MANIFOLD_WORDS
