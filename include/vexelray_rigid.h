@@ -278,7 +278,9 @@ int vr_mlp_last_error(char *buf, int32_t len);
  * Impulses (features 7-9, targets) are given as sign(v) log(1 + |v|) of the velocity v.
  * vr_box_log starts (or stops) logging (features, targets) of every step, up to `capacity` samples, on the device;
  * vr_box_samples copies up to `most` of them out. vr_box_learned sets the network (null: none) whose prediction
- * (where usable: finite, normal part not negative) is tried against the carried impulses at each step's first substep:
+ * (where usable: finite, normal part not negative, and at a point the carry has nothing for: a new contact, or one
+ * that moved past the carry's reach; a carried point keeps its impulse) is tried against the carried impulses at each
+ * step's first substep:
  * the first solve sweep of each scene's pairs is run from each start and put back, and a scene takes the prediction
  * only where it leaves that sweep's largest residual smaller, the carry otherwise (vr_box_solve.fell_back counts the
  * scenes that kept it). The report's `reserved` word counts the points started from the prediction.

@@ -99,7 +99,7 @@ def test_the_learned_start_reaches_the_residual_in_fewer_sweeps():
     (carry_solve, carry_all, carry_rep, _), (net_solve, net_all, net_rep, used) = sweeps[False], sweeps[True]
     print(f"solve sweeps a scene ran over the step: carry {carry_solve:.2f}, learned {net_solve:.2f}; all sweeps "
           f"{carry_all:.2f} against {net_all:.2f}; {used} points started from the prediction, "
-          f"{net_rep.fell_back} pairs kept the carry")
+          f"{net_rep.fell_back} scenes kept the carry")
     assert used > 0
     assert net_solve < carry_solve
     assert float(net_rep.scene_sweeps[0][0]) < float(carry_rep.scene_sweeps[0][0])  # the first substep, where it starts

@@ -1841,10 +1841,14 @@ __global__ void k_pack_prediction(BoxDev D, Colouring C, Learned L, int touching
         const float *y = L.y + (size_t)(L.pointBase[t] + k) * LEARN_OUTPUTS;
         if (!(isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2]) && y[0] >= 0 && P.first.im + P.second.im > 0))
             continue;
+        float *imp = D.packImpulse + (size_t)(k * POINT_IMPULSE) * T + j;
+        // A point the carry started (it found the point's last impulse) keeps it: the carried impulses are the last
+        // step's converged, balanced set, and a prediction made point by point is not. The prediction starts the
+        // points the carry has nothing for (new contacts, and points that moved past the carry's reach).
+        if (imp[0] != 0.f || imp[T] != 0.f || imp[2 * T] != 0.f || imp[3 * T] != 0.f) continue;
         float t1[3], t2[3];
         plane_basis(P.n, t1, t2);
         float along1 = unsquash(y[1]), along2 = unsquash(y[2]);
-        float *imp = D.packImpulse + (size_t)(k * POINT_IMPULSE) * T + j;
         imp[0] = unsquash(y[0]) / P.wsum;
         for (int a = 0; a < 3; a++) imp[(1 + a) * T] = (along1 * t1[a] + along2 * t2[a]) / P.wsum;
     }
@@ -1875,7 +1879,17 @@ __global__ void k_pack_choose(BoxDev D, Learned L, int touching) {
     const float *from = take ? L.startPredicted : L.startCarry;
     for (int f = 0; f < MOST_POINTS * POINT_IMPULSE; f++)
         D.packImpulse[(size_t)f * T + j] = from[(size_t)f * touching + j];
-    if (take) atomicAdd(L.used, D.packCount[j]);
+    if (!take) return;
+    int started = 0;  // the points whose start is the prediction's, not the carry's
+    for (int k = 0; k < min(D.packCount[j], MOST_POINTS); k++) {
+        bool differs = false;
+        for (int w = 0; w < 4; w++) {
+            size_t f = (size_t)(k * POINT_IMPULSE + w) * touching + j;
+            differs = differs || __float_as_uint(L.startPredicted[f]) != __float_as_uint(L.startCarry[f]);
+        }
+        started += differs ? 1 : 0;
+    }
+    if (started > 0) atomicAdd(L.used, started);
 }
 
 __global__ void k_scene_from_params(const float *P, SceneDev *scene) {
