@@ -89,35 +89,64 @@ __global__ void k_inc_place(BoxDev D, Colouring C) {
 // One step of the colouring. A pair still open colours itself if every earlier pair at its bodies was decided in an
 // earlier step (decided[q] < iter; those deciding in this step still read as open, so nothing here races): its round
 // is the first none of them took, or MOST_ROUNDS (unplaced) if they took all 32. Otherwise it marks the step as
-// leaving pairs open.
-__global__ void k_colour_step(BoxDev D, Colouring C, int iter) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) C.flags[(iter + 1) % COLOUR_FLAGS] = 0;
-    if (t >= *C.touching) return;
-    int p = C.touch[t];
-    if (C.decided[p] < iter) return;
-    int ends[2] = {p / SLOTS, D.contacts[p]};
-    unsigned taken = 0;
-    bool ready = true;
-    for (int e = 0; e < 2 && ready; e++) {
-        int from = C.incOff[ends[e]], to = C.incOff[ends[e] + 1];
-        for (int k = from; k < to; k++) {
-            int q = C.inc[k];
-            if (!(q < p)) continue;
-            if (C.decided[q] >= iter) {
-                ready = false;
-                break;
-            }
-            int r = C.colour[q];
-            if (r < MOST_ROUNDS) taken |= 1u << r;
-        }
-    }
-    if (!ready) {
-        C.flags[iter % COLOUR_FLAGS] = 1;
+// leaving pairs open and stays on the open list. Workgroup g owns places g * THREADS onwards of the open lists: in
+// step 0 it takes the touching pairs there, in each later step the pairs it kept open in the step before (segIn[g] of
+// them, in `in`), and it keeps the ones still open in its places of `out`, counting in LDS; so the later steps,
+// which decide the last few pairs, read only those, with no device-wide counter. Which order a segment holds its
+// pairs in does not matter.
+__global__ void __launch_bounds__(THREADS) k_colour_step(BoxDev D, Colouring C, int iter, const int32_t *in,
+                                                         int32_t *out, const int32_t *segIn, int32_t *segOut) {
+    __shared__ int sKept;
+    int g = blockIdx.x, lane = threadIdx.x & 63;
+    if (g == 0 && threadIdx.x == 0) C.flags[(iter + 1) % COLOUR_FLAGS] = 0;
+    int n = iter == 0 ? min(THREADS, *C.touching - g * THREADS) : segIn[g];
+    if (n <= 0) {
+        if (threadIdx.x == 0) segOut[g] = 0;
         return;
     }
-    C.colour[p] = taken == 0xFFFFFFFFu ? MOST_ROUNDS : __ffs(~taken) - 1;
-    C.decided[p] = iter;
+    if (threadIdx.x == 0) sKept = 0;
+    __syncthreads();
+    int t = threadIdx.x;
+    int p = -1;
+    bool open = false;
+    if (t < n) {
+        p = iter == 0 ? C.touch[g * THREADS + t] : in[g * THREADS + t];
+        int ends[2] = {p / SLOTS, D.contacts[p]};
+        unsigned taken = 0;
+        bool ready = true;
+        for (int e = 0; e < 2 && ready; e++) {
+            int from = C.incOff[ends[e]], to = C.incOff[ends[e] + 1];
+            for (int k = from; k < to; k++) {
+                int q = C.inc[k];
+                if (!(q < p)) continue;
+                if (C.decided[q] >= iter) {
+                    ready = false;
+                    break;
+                }
+                int r = C.colour[q];
+                if (r < MOST_ROUNDS) taken |= 1u << r;
+            }
+        }
+        if (ready) {
+            C.colour[p] = taken == 0xFFFFFFFFu ? MOST_ROUNDS : __ffs(~taken) - 1;
+            C.decided[p] = iter;
+        } else {
+            open = true;
+        }
+    }
+    unsigned long long mask = __ballot(open);
+    if (mask) {
+        int leader = __ffsll((long long)mask) - 1;
+        int at = 0;
+        if (lane == leader) at = atomicAdd(&sKept, __popcll(mask));
+        at = __builtin_amdgcn_readlane(at, leader);
+        if (open) out[g * THREADS + at + __popcll(mask & ((1ull << lane) - 1ull))] = p;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        segOut[g] = sKept;
+        if (sKept > 0) C.flags[iter % COLOUR_FLAGS] = 1;
+    }
 }
 
 // The sort keys: each touching pair's round (MOST_ROUNDS when unplaced), in the touching list's (slot) order.
@@ -187,10 +216,19 @@ void launch_colour_prepare(const BoxDev &D, const Colouring &C, const vr::SortSc
     check_launch("incidence");
 }
 
+// The open lists alternate between the colouring's key and order words, which its sort (launch_colour_finish) fills
+// only after the last step; so do the segments' counts.
 void launch_colour_steps(const BoxDev &D, const Colouring &C, int touchingPairs, int from, int count,
                          hipStream_t s) {
     if (touchingPairs <= 0) return;
-    for (int k = 0; k < count; k++) k_colour_step<<<blocks_for(touchingPairs), THREADS, 0, s>>>(D, C, from + k);
+    if (touchingPairs > C.most) throw std::runtime_error("colour: more touching pairs than slots");
+    int blocks = blocks_for(touchingPairs);
+    for (int k = 0; k < count; k++) {
+        int iter = from + k;
+        int32_t *out = (iter & 1) ? C.order : C.keys;
+        const int32_t *in = (iter & 1) ? C.keys : C.order;
+        k_colour_step<<<blocks, THREADS, 0, s>>>(D, C, iter, in, out, C.segOpen[(iter + 1) % 2], C.segOpen[iter % 2]);
+    }
     check_launch("colour step");
 }
 

@@ -118,6 +118,9 @@ struct Colouring {
     int32_t *keys, *order;   // slots: the sort of the touching list by round
     int32_t *starts;         // MOST_ROUNDS + 2: where each round's pairs begin in that order
     int32_t *report;         // REPORT_WORDS: points, unlisted
+    // slots / 256 + 1 each: the open pairs a colouring step left in each workgroup's segment of its open list (step
+    // i reads segOpen[(i + 1) % 2], writes segOpen[i % 2]).
+    int32_t *segOpen[2];
     int32_t *sumTmp;         // colour_tmp_words(slots)
     vr::SortScratch sort;    // sized for slots keys
 };

@@ -356,6 +356,8 @@ struct BoxWorld {
             C.order = alloc<int32_t>(slots);
             C.starts = alloc<int32_t>(MOST_ROUNDS + 2);
             C.report = alloc<int32_t>(REPORT_WORDS);
+            C.segOpen[0] = alloc<int32_t>(slots / 256 + 2);
+            C.segOpen[1] = alloc<int32_t>(slots / 256 + 2);
             C.sumTmp = alloc<int32_t>(colour_tmp_words(slots));
             int sortBlocks = (int)((slots + 2047) / 2048);
             C.sort.maxBlocks = sortBlocks;
