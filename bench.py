@@ -33,7 +33,9 @@ value = learned body-substeps per second = bodies x substeps x K / seconds. The 
     (96 N + 32 W a sweep) is roofline_walls;
   cpu_baseline: the oracle (oracle/, a C++ restatement of the reference's passes) on the first scenes of the batch from
     the plain window's state, one thread and all threads (the scenes share nothing: one thread a scene at a time);
-  pile_64k: the 65,536-sphere pile (the sphere configuration), the last K1 steps of its 600, with its own roofline.
+  pile_64k: the 65,536-sphere pile (the sphere configuration), the last K1 steps of its 600, with its own roofline;
+  pile_1k, mixed_32k (--others 1): the 1,000-sphere pile (last 20 of its 300 steps) and the 33,408-body mixed box pile
+    (last 20 of its 600), each a single scene.
 """
 
 import argparse
@@ -476,10 +478,11 @@ def batched(pkg, args):
 
 # ---- the sphere pile ----
 
-def sphere_pile(pkg, steps, warmup, run=600):
-    """The 65,536-sphere pile, the last `steps` of its `run` steps, with the round kernels' roofline."""
+def sphere_pile(pkg, steps, warmup, run=600, spec=None):
+    """A sphere pile (the 65,536-sphere pile unless `spec` names another), the last `steps` of its `run` steps, with
+    the round kernels' roofline."""
     scenes, _, _ = modules(pkg)
-    spec = scenes.pile_64k()
+    spec = spec if spec is not None else scenes.pile_64k()
     sim = pkg.SphereSimulation(spec.step)
     sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.r, spec.im, spec.params)
     settle = run - steps - warmup
@@ -505,7 +508,7 @@ def sphere_pile(pkg, steps, warmup, run=600):
     round_s = prof.round_ns * 1e-9
     achieved = bytes_total / round_s / 1e9 if round_s > 0 else 0.0
     return {
-        "workload": f"pile_64k: {n} spheres, Gauss-Seidel until done, {substeps} substeps of "
+        "workload": f"{spec.name}: {n} spheres, Gauss-Seidel until done, {substeps} substeps of "
                     f"{spec.step.iterations} passes, the last {steps} of {run} steps, seed {spec.seed}",
         "value": n * substeps * steps / seconds, "unit": "body-substeps/s", "ms_per_step": seconds * 1e3 / steps,
         "contacts_at_start": contacts_start, "contacts_at_end": contacts_end,
@@ -525,15 +528,18 @@ def sphere_pile(pkg, steps, warmup, run=600):
     }
 
 
-def box_scene(pkg, args):
-    """A box-path workload of one scene (mixed_32k): body-substeps per second, plain stepping."""
+def box_scene(pkg, args, run=0):
+    """A box-path workload of one scene (mixed_32k): body-substeps per second, plain stepping, over the last
+    `args.steps` of `run` steps (at least steps + warm-up)."""
     scenes, boxes, _ = modules(pkg)
     spec = scenes.mixed_32k()
     sim = boxes.BoxSimulation(spec.step)
     sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
               spec.params)
-    for _ in range(args.warmup):
+    settle = max(run - args.steps, args.warmup)
+    for k in range(settle):
         sim.stepAndWait()
+        progress("mixed_32k settle", k + 1, settle)
     t0 = time.perf_counter()
     reps = []
     for k in range(args.steps):
@@ -548,7 +554,8 @@ def box_scene(pkg, args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": seconds * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{spec.name}: {n} bodies (16,384 boxes, 16,384 balls, {spec.towers} towers of 20 "
-                               f"cubes), box path, {substeps} substeps of {spec.step.sweeps} sweeps, seed {spec.seed}",
+                               f"cubes), box path, {substeps} substeps of {spec.step.sweeps} sweeps, seed {spec.seed}, "
+                               f"the last {args.steps} of {settle + args.steps} steps",
                    "bodies": n, "method": "plain", "touching_at_end": reps[-1].touching,
                    "rounds_at_end": reps[-1].rounds, "deepest_point_at_end": depth, "broken": broken},
     }
@@ -566,6 +573,9 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--tolerance", type=float, default=0.01, help="m/s: the residual each scene's sweeps stop at")
     ap.add_argument("--c1-steps", type=int, default=20, help="timed steps of the sphere pile (0: skip it)")
+    ap.add_argument("--others", type=int, default=1,
+                    help="1: also time the 1,000-sphere pile (last 20 of 300 steps) and the mixed box pile (last 20 "
+                         "of 600) as sub-keys")
     ap.add_argument("--only", default="", help="plain: the plain run alone (for the profilers), one JSON line")
     ap.add_argument("--summarize", nargs=3, metavar=("TRACE_OR_-", "COUNTERS_OR_-", "LINE_JSON"),
                     help="write the timed window's kernels from rocprofv3 CSVs (kernel trace or counter collection) "
@@ -611,6 +621,14 @@ def main():
     line = batched(pkg, args)
     if args.c1_steps > 0:
         line["pile_64k"] = sphere_pile(pkg, args.c1_steps, 2)
+    if args.others:
+        scenes, _, _ = modules(pkg)
+        line["pile_1k"] = sphere_pile(pkg, 20, 2, run=300, spec=scenes.pile_1k())
+        mixed = box_scene(pkg, argparse.Namespace(steps=20, warmup=2), run=600)
+        line["mixed_32k"] = {"workload": mixed["config"]["workload"], "value": mixed["value"],
+                             "unit": mixed["unit"], "ms_per_step": mixed["ms_per_step"],
+                             "touching_at_end": mixed["config"]["touching_at_end"],
+                             "rounds_at_end": mixed["config"]["rounds_at_end"]}
     print(json.dumps(line))
 
 
