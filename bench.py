@@ -59,7 +59,7 @@ STATE = ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw")
 # copy of known size), over the same window as the bench line's roofline. None where not measured for that window.
 SOLVE_ROUND_TRAFFIC_BYTES = None
 GROUP_ROUNDS_TRAFFIC_BYTES = None
-GROUP_PHASE_TRAFFIC_BYTES = 13_701_145_525  # k_group_phase, plain window (profiles/traffic_box_kernels_batched_2k.md)
+GROUP_PHASE_TRAFFIC_BYTES = 13_693_875_846  # k_group_phase, plain window (profiles/traffic_box_kernels_batched_2k.md)
 WALLS_TRAFFIC_BYTES = None
 SPHERE_ROUND_TRAFFIC_BYTES = None
 
@@ -234,7 +234,8 @@ def box_roofline(prof, walls_per_sweep):
         achieved = total / seconds / 1e9 if seconds > 0 else 0.0
         rounds = {
             "bound": "hbm",
-            "limited_by": "VALU issue: the phase kernel's vector ALU is busy about 80 % of each SIMD's time (SQ counters, "
+            "limited_by": "dependent latency: the largest scenes' chains of pair solves, round after round; each SIMD's "
+                          "vector ALU issues about 67 % of the time (SQ counters, "
                           "profiles/valu_group_phase_batched_2k.md); no MFMA work",
             "kernel": "k_group_phase (a solve or relax phase of sweeps in one launch: each scene group, a workgroup, "
                       "through the phase's walls and colour rounds, its motion records in LDS), plain window",
