@@ -35,7 +35,10 @@ value = learned body-substeps per second = bodies x substeps x K / seconds. The 
     the plain window's state, one thread and all threads (the scenes share nothing: one thread a scene at a time);
   pile_64k: the 65,536-sphere pile (the sphere configuration), the last K1 steps of its 600, with its own roofline;
   pile_1k, mixed_32k (--others 1): the 1,000-sphere pile (last 20 of its 300 steps) and the 33,408-body mixed box pile
-    (last 20 of its 600), each a single scene.
+    (last 20 of its 600), each a single scene;
+  reference_scenes (--others 1): the reference's own measured sphere scenes (its column of 20, pile of 343 and pile of
+    4,096 with friction, 10 substeps), timed as the reference times them (2 frames untimed, then the rest of the run,
+    each step waited for), beside the ms a step it publishes for them on its own GPU.
 """
 
 import argparse
@@ -529,6 +532,71 @@ def sphere_pile(pkg, steps, warmup, run=600, spec=None):
     }
 
 
+# The reference's published ms a step for its measured scenes, on an RTX 5070 Ti through Vulkan: the sphere scenes
+# with friction mu = 0.5, 10 substeps, run until done; the box piles, mu = 0.5, 10 substeps of 4 sweeps.
+SWEEP_PUBLISHED = {
+    "sweep_column": (1.83, "docs/TODO.md:321"),
+    "sweep_pile_343": (2.39, "docs/TODO.md:322"),
+    "sweep_pile_4096": (2.90, "docs/TODO.md:367-368"),
+    "box_pile_343": (6.9, "vexelray-sim-rigid-core/src/test/java/dev/vexelray/sim/rigid/box/BoxStackTest.java:165"),
+    "box_pile_1000": (7.9, "vexelray-sim-rigid-core/src/test/java/dev/vexelray/sim/rigid/box/BoxStackTest.java:178"),
+}
+
+
+def reference_scenes(pkg):
+    """The reference's measured scenes, each timed as its test does: the sphere scenes as SphereSweepTest.run
+    (SphereSweepTest.java:175-182), 2 frames untimed and then the rest of the run's frames; the box piles as
+    BoxStackTest.pile (:213-216), 3 s untimed and then 1 s; each step waited for, on the wall clock."""
+    scenes, boxes, _ = modules(pkg)
+    out = {}
+    for spec in (scenes.sweep_column(), scenes.sweep_pile(7), scenes.sweep_pile(16)):
+        sim = pkg.SphereSimulation(spec.step)
+        sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.r, spec.im, spec.params)
+        for _ in range(2):
+            sim.stepAndWait()
+        sim.finish()
+        t0 = time.perf_counter()
+        rounds = 0
+        for _ in range(spec.steps - 2):
+            rounds += sim.stepAndWait().solve.rounds
+        seconds = time.perf_counter() - t0
+        sim.close()
+        ms = seconds * 1e3 / (spec.steps - 2)
+        published, source = SWEEP_PUBLISHED[spec.name]
+        out[spec.name] = {
+            "spheres": spec.step.spheres, "frames_timed": spec.steps - 2, "ms_per_step": ms,
+            "value": spec.step.spheres * spec.step.substeps / (ms * 1e-3), "unit": "body-substeps/s",
+            "rounds_per_step": rounds / (spec.steps - 2),
+            "reference_ms_per_step": published, "reference_source": source + " (RTX 5070 Ti, Vulkan)",
+            "reference_over_this": published / ms,
+        }
+        progress("reference scenes", len(out), 5)
+    for spec in (scenes.box_pile(7), scenes.box_pile(10)):
+        sim = boxes.BoxSimulation(spec.step)
+        sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
+                  spec.params)
+        untimed = 180
+        for _ in range(untimed):
+            sim.stepAndWait()
+        sim.finish()
+        t0 = time.perf_counter()
+        for _ in range(spec.steps - untimed):
+            rep = sim.stepAndWait()
+        seconds = time.perf_counter() - t0
+        sim.close()
+        ms = seconds * 1e3 / (spec.steps - untimed)
+        published, source = SWEEP_PUBLISHED[spec.name]
+        out[spec.name] = {
+            "boxes": spec.step.boxes, "frames_timed": spec.steps - untimed, "ms_per_step": ms,
+            "value": spec.step.boxes * spec.step.substeps / (ms * 1e-3), "unit": "body-substeps/s",
+            "touching_at_end": rep.touching, "rounds_at_end": rep.rounds,
+            "reference_ms_per_step": published, "reference_source": source + " (RTX 5070 Ti, Vulkan)",
+            "reference_over_this": published / ms,
+        }
+        progress("reference scenes", len(out), 5)
+    return out
+
+
 def box_scene(pkg, args, run=0):
     """A box-path workload of one scene (mixed_32k): body-substeps per second, plain stepping, over the last
     `args.steps` of `run` steps (at least steps + warm-up)."""
@@ -575,9 +643,10 @@ def main():
     ap.add_argument("--tolerance", type=float, default=0.01, help="m/s: the residual each scene's sweeps stop at")
     ap.add_argument("--c1-steps", type=int, default=20, help="timed steps of the sphere pile (0: skip it)")
     ap.add_argument("--others", type=int, default=1,
-                    help="1: also time the 1,000-sphere pile (last 20 of 300 steps) and the mixed box pile (last 20 "
-                         "of 600) as sub-keys")
-    ap.add_argument("--only", default="", help="plain: the plain run alone (for the profilers), one JSON line")
+                    help="1: also time the 1,000-sphere pile (last 20 of 300 steps), the mixed box pile (last 20 "
+                         "of 600) and the reference's measured sphere scenes as sub-keys")
+    ap.add_argument("--only", default="", help="plain: the plain run alone (for the profilers); scenes: the "
+                    "reference's measured sphere scenes alone; one JSON line")
     ap.add_argument("--summarize", nargs=3, metavar=("TRACE_OR_-", "COUNTERS_OR_-", "LINE_JSON"),
                     help="write the timed window's kernels from rocprofv3 CSVs (kernel trace or counter collection) "
                          "and a --only plain line to --summary-out, then exit")
@@ -609,6 +678,9 @@ def main():
     if args.only == "plain":
         print(json.dumps(plain_only(pkg, args)))
         return
+    if args.only == "scenes":
+        print(json.dumps(reference_scenes(pkg)))
+        return
     if args.workload == "mixed_32k":
         print(json.dumps(box_scene(pkg, args)))
         return
@@ -630,6 +702,7 @@ def main():
                              "unit": mixed["unit"], "ms_per_step": mixed["ms_per_step"],
                              "touching_at_end": mixed["config"]["touching_at_end"],
                              "rounds_at_end": mixed["config"]["rounds_at_end"]}
+        line["reference_scenes"] = reference_scenes(pkg)
     print(json.dumps(line))
 
 

@@ -74,6 +74,47 @@ def test_pile_with_friction_matches_oracle_bitwise():
     assert np.array_equal(imp_a.view(np.int32), imp_b.view(np.int32))
 
 
+@pytest.mark.parametrize("plan", ["wide", "local"])
+def test_both_pass_plans_match_the_oracle_bitwise(plan, monkeypatch):
+    """A Gauss-Seidel pass runs as chip-wide round launches finished by one workgroup ("wide"), or wholly in one
+    workgroup with the claim arrays in LDS ("local"); VR_SPHERE_PLAN fixes the plan. Both run the reference's rounds:
+    each matches the oracle to the bit, reports included."""
+    monkeypatch.setenv("VR_SPHERE_PLAN", plan)
+    a, b = run_both(friction_pile, 4, extra=("contactCount", "contactImpulse"))
+    assert a["reports"] == b["reports"]
+    assert np.array_equal(a["contactCount"], b["contactCount"])
+    assert_bits(a, b, f"friction pile, {plan} plan")
+    listed = int(a["contactCount"][0])
+    assert np.array_equal(a["contactImpulse"][:7 * listed].view(np.int32),
+                          b["contactImpulse"][:7 * listed].view(np.int32))
+
+
+def crowded_pile():
+    """900 spheres in the pile's box: more listed contacts a pass than the local plan's 1,024 threads take at once."""
+    s = sc.pile(n=900)
+    s.muStatic = s.muKinetic = 0.5
+    s.muRolling = 0.05
+    return s
+
+
+def test_the_local_plan_over_rounds_longer_than_its_workgroup_matches_the_oracle(monkeypatch):
+    monkeypatch.setenv("VR_SPHERE_PLAN", "local")
+    a, b = run_both(crowded_pile, 3, extra=("contactCount",))
+    assert int(a["contactCount"][2]) > 1024, "the rounds must take the workgroup more than once"  # LONGEST
+    assert a["reports"] == b["reports"]
+    assert_bits(a, b, "crowded pile, local plan")
+
+
+def test_the_plan_chosen_step_by_step_matches_the_oracle(monkeypatch):
+    """Unfixed, the plan is local for a first step of at most 1,024 spheres and then follows the contacts the step
+    before listed a pass: the crowded pile (up to 5,261 contacts listed a substep in its first step) starts local,
+    goes wide, and stays bit-exact across the change."""
+    monkeypatch.delenv("VR_SPHERE_PLAN", raising=False)
+    a, b = run_both(crowded_pile, 3, extra=("contactCount",))
+    assert a["reports"] == b["reports"]
+    assert_bits(a, b, "crowded pile, chosen plan")
+
+
 def test_frictionless_pile_fixed_rounds_matches_oracle():
     def make():
         s = sc.pile()

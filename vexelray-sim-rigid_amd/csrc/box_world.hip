@@ -240,16 +240,20 @@ struct BoxWorld {
         }
         sceneStep = !wallsFromParams && sceneBodies.size() >= SCENE_STEP_LEAST && most <= SCENE_STEP_MOST;
         plan = sceneStep ? PLAN_SCENE : PLAN_PASS;
-        tunable = !wallsFromParams && grid && sceneBodies.size() > 1;
+        // A lone scene small enough for one workgroup's LDS is tuned as a batch is, its one group and its one
+        // workgroup among the candidates; the walls it reads from the params are copied into its scene record at
+        // every step's start (launch_scene_from_params), where the scene and group kernels read them.
+        bool loneSmall = sceneBodies.size() == 1 && n <= GROUP_LOCAL_BODIES;
+        tunable = grid && (sceneBodies.size() > 1 || loneSmall);
         const char *fixed = std::getenv("VR_BOX_PLAN");
         if (!fixed) fixed = std::getenv("VR_BOX_STEP_MODE");
         if (fixed) {
             std::string m(fixed);
-            if (m == "scene" && !wallsFromParams) plan = PLAN_SCENE;
+            if (m == "scene" && (!wallsFromParams || loneSmall)) plan = PLAN_SCENE;
             if (m == "pass") plan = PLAN_PASS;
             if (m == "all") plan = PLAN_ALL;
             if (m == "graph" && grid) plan = PLAN_GRAPH;
-            if (m == "group" && grid && !wallsFromParams) plan = PLAN_GROUP;
+            if (m == "group" && grid && (!wallsFromParams || loneSmall)) plan = PLAN_GROUP;
             sceneStep = plan == PLAN_SCENE;
             tunable = false;
         }
@@ -262,7 +266,7 @@ struct BoxWorld {
         D.scenes = (int)scenes.size();
         D.scene = sceneIdx;
         D.sceneInfo = sceneDev;
-        if (grid && !wallsFromParams && sceneBodies.size() > 1) make_groups(first);
+        if (grid && (sceneBodies.size() > 1 || loneSmall)) make_groups(first);
         if (plan == PLAN_GROUP && G.count == 0) plan = PLAN_PASS;
         D.width = std::max(1, n / 2);
         static const char *field[] = {"x",  "y",  "z",  "u",  "v",  "w",  "lx", "ly", "lz",

@@ -97,8 +97,13 @@ struct PassSubmit {
     int prefix, carry, slice;
 };
 
+// `local`: the whole pass in one workgroup of LOCAL_THREADS with the claims in LDS (no wide rounds), for at most
+// LOCAL_MOST spheres. It pays while a round's open contacts fit in one go of the workgroup's threads: each further
+// go waits out the memory latency again, where the wide rounds take them side by side.
+constexpr int LOCAL_MOST = 6144, LOCAL_THREADS = 1024;
 void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLists &L, int32_t *opening,
-                 int32_t *enough, int it, int iterations, bool rub, int wide, const PassSubmit &submit, hipStream_t s);
+                 int32_t *enough, int it, int iterations, bool rub, int wide, bool local, const PassSubmit &submit,
+                 hipStream_t s);
 void launch_step_begin(PassCtl *ctl, PassSlot *slots, int nslots, hipStream_t s);
 void launch_walls(const SphereDev &D, hipStream_t s);
 void launch_velocity(const SphereDev &D, hipStream_t s);

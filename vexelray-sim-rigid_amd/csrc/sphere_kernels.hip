@@ -573,11 +573,22 @@ __global__ void k_round(SphereDev D, RoundConst rc) {
 // loaded before it is used): one memory latency fewer for the contacts that solve, but every open contact loads its
 // pair, and only one in six to ten solves in a round. The wide rounds, bound by how many loads the memory system
 // takes, load after the check; the tail, a few contacts in one workgroup and bound by latency, loads with it.
+// The three claim arrays a round reads and writes: the world's in device memory, or a copy of them in LDS.
+// Picked by selects, not by an index into an array of pointers, which would put the array in scratch memory.
+struct Claims {
+    long long *c0, *c1, *c2;
+    __device__ __forceinline__ long long *at(int q) const { return q == 0 ? c0 : (q == 1 ? c1 : c2); }
+};
+
+__device__ __forceinline__ Claims device_claims(const SphereDev &D) {
+    return Claims{D.claims[0], D.claims[1], D.claims[2]};
+}
+
 template <bool RUBS, bool EAGER>
-__device__ __forceinline__ bool round_contact(const SphereDev &D, const Params &P, const RoundConst &rc, int c, int a,
-                                              int o) {
+__device__ __forceinline__ bool round_contact(const SphereDev &D, const Claims &C, const Params &P,
+                                              const RoundConst &rc, int c, int a, int o) {
     if (rc.checks == 0) return true;
-    const long long *checked = D.claims[(rc.slot + 2) % 3];
+    const long long *checked = C.at((rc.slot + 2) % 3);
     long long last = priority(D, a, o, rc.checksDispatch);
     long long ca = checked[a], co = checked[o];
     if (EAGER) {
@@ -618,9 +629,10 @@ __device__ __forceinline__ long long scan_step(long long most, bool take) {
 // each run of claiming lanes with one a clears and claims a alone, with the run's largest priority (a segmented max
 // scan over the wave), and every word ends the round as it would with one clear and one atomic per contact: fewer
 // device atomics, the same claims. Sphere o is claimed once per contact. Called by every lane of the wave.
-__device__ __forceinline__ void claim_wave(const SphereDev &D, const RoundConst &rc, bool claims, int a, int o) {
-    long long *claimed = D.claims[rc.slot % 3];
-    long long *cleared = D.claims[(rc.slot + 1) % 3];
+__device__ __forceinline__ void claim_wave(const SphereDev &D, const Claims &C, const RoundConst &rc, bool claims, int a,
+                                           int o) {
+    long long *claimed = C.at(rc.slot % 3);
+    long long *cleared = C.at((rc.slot + 1) % 3);
     int lane = threadIdx.x & 63;
     long long mine = claims ? priority(D, a, o, rc.dispatch) : 0;
     int key = claims ? a : -1 - lane;  // a lane that does not claim is a run of its own
@@ -752,8 +764,9 @@ __global__ void __launch_bounds__(THREADS) k_round_wide(SphereDev D, const PassC
     RoundConst rc = cycle_round(it, position(j), RUB, iterations);
     const Params P = load_params(D.params);
     bool open = false;
-    if (i < nIn) open = round_contact<RUB, false>(D, P, rc, e.x, e.y, e.z);
-    claim_wave(D, rc, open, e.y, e.z);
+    const Claims C = device_claims(D);
+    if (i < nIn) open = round_contact<RUB, false>(D, C, P, rc, e.x, e.y, e.z);
+    claim_wave(D, C, rc, open, e.y, e.z);
     if (last) {
         append_open(open, e, out, &slot->tailCount);
     } else {
@@ -763,18 +776,23 @@ __global__ void __launch_bounds__(THREADS) k_round_wide(SphereDev D, const PassC
     }
 }
 
-// The rest of a pass in one workgroup, rounds separated by barriers rather than launches. It starts at the first
-// round the wide rounds found empty, or else at round `wide` on the list the last of them compacted. A parting pass
-// runs until no contact is open, then settles k, the rounds the reference's stepper would have run: its opening
-// batch, then batches of 8, up to the first report that finds nothing open (SphereStepper.java:72-96); the opening
-// batch then adapts as there (:118-126). A rubbing pass runs the parting pass's k rounds, or until nothing is open.
+// The rest of a pass in one workgroup of T threads, rounds separated by barriers rather than launches. It starts at
+// the first round the wide rounds found empty, or else at round `wide` on the list the last of them compacted. A
+// parting pass runs until no contact is open, then settles k, the rounds the reference's stepper would have run: its
+// opening batch, then batches of 8, up to the first report that finds nothing open (SphereStepper.java:72-96); the
+// opening batch then adapts as there (:118-126). A rubbing pass runs the parting pass's k rounds, or until nothing is
+// open.
+// LOCAL: the whole pass (wide is 0) with the three claim arrays copied into LDS at the start and back at the end, so
+// a round's claim checks, clears and atomics are LDS operations; for worlds of at most LOCAL_MOST spheres. A claim
+// word holds the same value at every point as in device memory, so the rounds are the same.
 constexpr int TAIL = 512;
 
-template <bool RUB>
-__global__ void __launch_bounds__(TAIL) k_round_tail(SphereDev D, PassCtl *ctl, const PassSlot *slot, OpenLists L,
-                                                     int32_t *opening, int32_t *enough, int it, int iterations,
-                                                     int wide, int limit, PassSubmit submit) {
+template <bool RUB, bool LOCAL, int T>
+__global__ void __launch_bounds__(T) k_round_tail(SphereDev D, PassCtl *ctl, const PassSlot *slot, OpenLists L,
+                                                  int32_t *opening, int32_t *enough, int it, int iterations,
+                                                  int wide, int limit, PassSubmit submit) {
     __shared__ int sCount;
+    extern __shared__ long long sClaims[];  // LOCAL: 3 * D.n words
     int listed = min(D.count[LISTED], D.cap);
     int j = 0, n = listed;
     if (wide > 0) {
@@ -790,6 +808,16 @@ __global__ void __launch_bounds__(TAIL) k_round_tail(SphereDev D, PassCtl *ctl, 
     int open0 = RUB ? 0 : opening[it];
     int kRub = RUB ? ctl->k : 0;
     const Params P = load_params(D.params);
+    Claims C = device_claims(D);
+    if (LOCAL) {
+        C = Claims{sClaims, sClaims + D.n, sClaims + 2 * (size_t)D.n};
+        for (int s = threadIdx.x; s < D.n; s += T) {
+            C.c0[s] = D.claims[0][s];
+            C.c1[s] = D.claims[1][s];
+            C.c2[s] = D.claims[2][s];
+        }
+        __syncthreads();
+    }
     int k = 0;
     bool failed = false;
     while (true) {
@@ -811,21 +839,28 @@ __global__ void __launch_bounds__(TAIL) k_round_tail(SphereDev D, PassCtl *ctl, 
         int4 *out = L.buf[(j + 1) & 1];
         if (threadIdx.x == 0) sCount = 0;
         __syncthreads();
-        for (int base = 0; base < n; base += TAIL) {
+        for (int base = 0; base < n; base += T) {
             int i = base + threadIdx.x;
             bool open = false;
             int4 e = make_int4(0, 0, 0, 0);
             if (i < n) {
                 e = j == 0 ? listed_entry(D, i) : open_entry(in, i);
-                open = round_contact<RUB, true>(D, P, rc, e.x, e.y, e.z);
+                open = round_contact<RUB, true>(D, C, P, rc, e.x, e.y, e.z);
             }
-            claim_wave(D, rc, open, e.y, e.z);
+            claim_wave(D, C, rc, open, e.y, e.z);
             append_open(open, e, out, &sCount);
         }
         __syncthreads();
         n = sCount;
         j++;
         __syncthreads();
+    }
+    if (LOCAL) {
+        for (int s = threadIdx.x; s < D.n; s += T) {
+            D.claims[0][s] = C.c0[s];
+            D.claims[1][s] = C.c1[s];
+            D.claims[2][s] = C.c2[s];
+        }
     }
     // The rounds this pass needed, for the host to launch that many wide next step.
     int used = (RUB ? iterations : 0) + it;
@@ -1381,12 +1416,12 @@ void launch_step_begin(PassCtl *ctl, PassSlot *slots, int nslots, hipStream_t s)
 }
 
 void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLists &L, int32_t *opening,
-                 int32_t *enough, int it, int iterations, bool rub, int wide, const PassSubmit &submit,
+                 int32_t *enough, int it, int iterations, bool rub, int wide, bool local, const PassSubmit &submit,
                  hipStream_t s) {
     if (wide < 0 || wide > MAX_WIDE) throw std::runtime_error("pass: wide rounds out of range");
     if ((int64_t)L.segments * SEG < D.cap) throw std::runtime_error("pass: open lists shorter than the contact list");
     int limit = D.cap + 2 * CYCLE;
-    if (L.segments == 0) wide = 0;  // no contact can be listed: the tail alone finds the pass empty
+    if (L.segments == 0 || local) wide = 0;  // no contact can be listed, or the tail takes the whole pass
     for (int j = 0; j < wide; j++) {
         int last = j == wide - 1 ? 1 : 0;
         const int4 *in = L.buf[j & 1];
@@ -1397,10 +1432,33 @@ void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLis
                           : (j == 0 ? k_round_wide<false, true> : k_round_wide<false, false>);
         kernel<<<L.segments, THREADS, 0, s>>>(D, ctl, slot, in, segIn, out, segOut, it, iterations, j, last);
     }
-    if (rub)
-        k_round_tail<true><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide, limit, submit);
-    else
-        k_round_tail<false><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide, limit, submit);
+    if (local) {
+        if (D.n > LOCAL_MOST) throw std::runtime_error("pass: a pass in LDS takes at most LOCAL_MOST spheres");
+        // The claims follow the kernel's one static word, at the next 8-byte boundary.
+        size_t lds = (size_t)3 * D.n * sizeof(long long) + sizeof(long long);
+        static bool sized = false;  // both instances allowed the LDS of LOCAL_MOST spheres, once a process
+        if (!sized) {
+            int most = (int)((3 * LOCAL_MOST + 1) * sizeof(long long));
+            if (hipFuncSetAttribute((const void *)k_round_tail<true, true, LOCAL_THREADS>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, most) != hipSuccess ||
+                hipFuncSetAttribute((const void *)k_round_tail<false, true, LOCAL_THREADS>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, most) != hipSuccess)
+                throw std::runtime_error("pass: the claims of LOCAL_MOST spheres do not fit in LDS");
+            sized = true;
+        }
+        if (rub)
+            k_round_tail<true, true, LOCAL_THREADS><<<1, LOCAL_THREADS, lds, s>>>(D, ctl, slot, L, opening, enough, it,
+                                                                                 iterations, wide, limit, submit);
+        else
+            k_round_tail<false, true, LOCAL_THREADS><<<1, LOCAL_THREADS, lds, s>>>(D, ctl, slot, L, opening, enough,
+                                                                                  it, iterations, wide, limit, submit);
+    } else if (rub) {
+        k_round_tail<true, false, TAIL><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide,
+                                                           limit, submit);
+    } else {
+        k_round_tail<false, false, TAIL><<<1, TAIL, 0, s>>>(D, ctl, slot, L, opening, enough, it, iterations, wide,
+                                                            limit, submit);
+    }
     check_launch("pass");
 }
 

@@ -75,6 +75,8 @@ struct vr_world {
     OpenLists openLists{};
     int32_t *devOpening = nullptr, *devEnough = nullptr;  // SphereStepper's opening batches, kept on the device
     std::vector<int> wide, wideRub;    // wide rounds to launch per pass, from what the last step used
+    bool localPasses = false;          // every pass in one workgroup, the claims in LDS (launch_pass's `local`)
+    bool planFixed = false;            // VR_SPHERE_PLAN set: localPasses is not revised after a step
     bool hostWaits = false;            // run the reference's launch model: host reads each report
     int slice = 0;                     // SphereSimulation.slice: most dispatches a submission, 0 for no bound
     double *diagPartials = nullptr, *diagOut = nullptr;  // device scratch of vr_world_diagnostics, made on first use
@@ -117,6 +119,17 @@ struct vr_world {
         // VR_HOST_WAITS=1 runs a Gauss-Seidel pass as the reference's stepper does, the host reading each report.
         const char *hw = std::getenv("VR_HOST_WAITS");
         hostWaits = hw && hw[0] == '1';
+        // Each pass in one workgroup with its claims in LDS (a round a barrier, not a launch) while a pass lists
+        // at most LOCAL_THREADS contacts: guessed from the spheres for the first step, then from the contacts the
+        // step before listed a pass (read after step_device's wait). Both plans run the same rounds.
+        // VR_SPHERE_PLAN=wide|local fixes the choice (local only up to LOCAL_MOST spheres).
+        localPasses = n <= LOCAL_THREADS;
+        if (const char *plan = std::getenv("VR_SPHERE_PLAN")) {
+            if (std::string(plan) == "wide") localPasses = false;
+            else if (std::string(plan) == "local") localPasses = n <= LOCAL_MOST;
+            else throw ArgError("VR_SPHERE_PLAN is wide or local, got " + std::string(plan));
+            planFixed = true;
+        }
         HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIP_OK(hipEventCreate(&t0));
         HIP_OK(hipEventCreate(&t1));
@@ -362,9 +375,9 @@ struct vr_world {
             }
             HIP_OK(hipEventRecord(passEvents[2 * q], stream));
         }
-        launch_pass(D, ctl, slots + q, openLists, devOpening, devEnough, it, d.iterations, rub, wideRounds, submit,
-                    stream);
-        launchesThisStep += wideRounds + 1;
+        launch_pass(D, ctl, slots + q, openLists, devOpening, devEnough, it, d.iterations, rub, wideRounds,
+                    localPasses, submit, stream);
+        launchesThisStep += localPasses ? 1 : wideRounds + 1;
         if (profiling) HIP_OK(hipEventRecord(passEvents[2 * q + 1], stream));
     }
 
@@ -420,6 +433,8 @@ struct vr_world {
             prof.steps += 1;
             prof.enqueue_ns += enqueued;
         }
+        if (!planFixed && q > 0)
+            localPasses = n <= LOCAL_MOST && ctlHost->listedSum <= (long long)LOCAL_THREADS * q;
         rep->waits = ctlHost->waits;
         rep->submissions = ctlHost->submissions + pieces(closing_passes(friction ? ctlHost->k : 0));
         rep->rounds = ctlHost->rounds;

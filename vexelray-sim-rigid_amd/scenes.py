@@ -9,6 +9,19 @@ Each builder returns a SceneSpec: the SphereStep to make, the initial state, and
   walls), velocity N(0, 1) m/s per axis; mu = 0.4, e = 0.1; dt = 1/60 s in 8 substeps, 10 passes, 1 m cells. More
   spheres than the reference's 31-bit contact priorities order, so the step uses 63-bit ones (wide pairs).
 
+* sweep_column, sweep_pile(side): the reference's own measured sphere scenes (SphereSweepTest.java:93-143, with
+  Scene.java's defaults): a column of 20 touching spheres of r = 5 cm, 1 kg, in a 0.1 x 2.5 x 0.1 m box; and a
+  lattice of side^3 spheres of r = 3 cm, 0.1 kg (x, z = 0.07 + 0.06 i + 0.005 U, y = 0.3 + 0.07 j, drawn from
+  java.util.Random(7)) in a box 0.08 + 0.06 side wide and 0.5 + 0.07 side tall. Both with friction mu = 0.5 static
+  and kinetic, no rolling resistance, e = 0, dt = 1/60 s in 10 substeps of one Gauss-Seidel pass run until done,
+  cells as wide as a sphere (SphereSweepTest's Solver.friction(10, 0.5)). run 180 frames (column) or 240 (piles).
+
+* box_pile(side): the reference's measured box pile (BoxStackTest.java:184-232, with BoxScene.java's defaults): side^3
+  cubes of half-extent 4 cm, 0.5 kg, x, z = 0.25 + 0.1 i + 0.02 U, y = 0.3 + 0.1 j, each turned by 0.6 U rad about the
+  axis (U - 0.5, U - 0.5, U - 0.5), drawn from java.util.Random(7), in a box 0.1 side + 0.4 wide and 0.1 side + 1.5
+  tall; mu = 0.5, margin 0.005 m; dt = 1/60 s in 10 substeps of 4 sweeps. Side 7 is its pile of 343, 10 its 1,000.
+  run 240 frames: 3 s settling, then the second it times.
+
 * mixed_32k: the mixed-shape pile, on the box path. 16,384 oriented boxes (half-extents uniform in [0.2, 1.0] m per
   axis, orientation a normalised Gaussian quaternion), 16,384 spheres (r uniform in [0.2, 0.6] m), density
   1000 kg/m^3, dropped at rest from heights 22-70 m into a 128 x 72 x 128 m walled box, beside 32 towers of 20
@@ -99,6 +112,47 @@ def pile_64k(seed: int = 2, n: int = 65_536) -> SceneSpec:
                      v32[:, 0].copy(), v32[:, 1].copy(), v32[:, 2].copy(), r32, im, box, 600, dt, seed)
 
 
+def _sweep_scene(name, n, box, x, y, z, radius, mass, steps, seed, substeps=10, mu=0.5):
+    """Scene.start for a Gauss-Seidel friction solve (Scene.java:107-128): one pass a substep, omega 1, averaged,
+    cells of the widest sphere."""
+    r = np.full(n, radius, dtype=np.float32)
+    im = np.full(n, np.float32(1 / mass), dtype=np.float32)
+    zeros = np.zeros(n, dtype=np.float32)
+    dt = 1.0 / 60
+    step = SphereStep(n, substeps, 1, SphereGrid.covering(*box, 2 * radius), Solve.GAUSS_SEIDEL_BY_CONTACT)
+    params = Spheres.params(dt / substeps, 0, -9.81, 0, *box, 1.0, True, 0.0, mu, mu, 0.0)
+    return SceneSpec(name, step, params, np.asarray(x, dtype=np.float32), np.asarray(y, dtype=np.float32),
+                     np.asarray(z, dtype=np.float32), zeros, zeros.copy(), zeros.copy(), r, im, box, steps, dt, seed)
+
+
+def sweep_column() -> SceneSpec:
+    """SphereSweepTest.column (SphereSweepTest.java:93-110): 20 spheres of r = 5 cm stacked touching."""
+    n = 20
+    box = (0.1, 2 * 0.05 * n + 0.5, 0.1)
+    return _sweep_scene("sweep_column", n, box, [0.05] * n, [0.05 + 0.1 * k for k in range(n)], [0.05] * n, 0.05,
+                        1.0, 180, 0)
+
+
+def sweep_pile(side: int = 16) -> SceneSpec:
+    """SphereSweepTest.pile (SphereSweepTest.java:124-143): side^3 spheres of r = 3 cm as a loose lattice; side 7 is
+    its pile of 343, side 16 its large pile of 4,096."""
+    n = side ** 3
+    rnd = JavaRandom(7)
+    x = np.empty(n, dtype=np.float32)
+    y = np.empty(n, dtype=np.float32)
+    z = np.empty(n, dtype=np.float32)
+    k = 0
+    for i in range(side):
+        for j in range(side):
+            for l in range(side):
+                x[k] = np.float32(0.07 + 0.06 * i + 0.005 * rnd.nextDouble())
+                y[k] = np.float32(0.3 + 0.07 * j)
+                z[k] = np.float32(0.07 + 0.06 * l + 0.005 * rnd.nextDouble())
+                k += 1
+    wide = 0.08 + 0.06 * side
+    return _sweep_scene(f"sweep_pile_{n}", n, (wide, 0.5 + 0.07 * side, wide), x, y, z, 0.03, 0.1, 240, 7)
+
+
 @dataclass
 class BoxSceneSpec:
     """A box-path workload: the BoxStep to make, the initial state (BoxSimulation.start's arguments), the params."""
@@ -171,6 +225,42 @@ def mixed_32k(seed: int = 3, boxes: int = 16_384, spheres: int = 16_384, towers:
     return BoxSceneSpec("mixed_32k", step, params, f32(pos[0]), f32(pos[1]), f32(pos[2]), zeros, zeros.copy(),
                         zeros.copy(), np.zeros((3, n), dtype=np.float32), f32(q), f32(half), shape,
                         f32(1.0 / mass), box, 600, dt, seed, towers)
+
+
+def box_pile(side: int = 7) -> BoxSceneSpec:
+    """BoxStackTest.pile (BoxStackTest.java:184-232): a loose block of side^3 cubes, jittered and turned."""
+    n = side ** 3
+    rnd = JavaRandom(7)
+    pos = np.empty((3, n), dtype=np.float32)
+    q = np.zeros((4, n), dtype=np.float32)
+    k = 0
+    for i in range(side):
+        for j in range(side):
+            for l in range(side):
+                pos[0, k] = np.float32(0.25 + 0.1 * i + 0.02 * rnd.nextDouble())
+                pos[1, k] = np.float32(0.3 + 0.1 * j)
+                pos[2, k] = np.float32(0.25 + 0.1 * l + 0.02 * rnd.nextDouble())
+                axis = [rnd.nextDouble() - 0.5, rnd.nextDouble() - 0.5, rnd.nextDouble() - 0.5]
+                length = math.sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2])
+                angle = 0.6 * rnd.nextDouble()
+                sin = math.sin(angle / 2) / length
+                for a in range(3):
+                    q[a, k] = np.float32(axis[a] * sin)
+                q[3, k] = np.float32(math.cos(angle / 2))
+                k += 1
+    half = np.full((3, n), np.float32(0.04), dtype=np.float32)
+    h = half.astype(np.float64)
+    widest = float(np.max(np.sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2])))
+    margin = 0.005
+    box = (0.1 * side + 0.4, 0.1 * side + 1.5, 0.1 * side + 0.4)
+    substeps, dt = 10, 1.0 / 60
+    step = BoxStep(n, substeps, Boxes.WALL_PASSES, False, BoxBroad.grid(*box, widest, margin))
+    params = Boxes.params(dt / substeps, 0, -9.81, 0, *box, 0.5, 0.5, margin, 0.0)
+    zeros = np.zeros(n, dtype=np.float32)
+    return BoxSceneSpec(f"box_pile_{n}", step, params, pos[0].copy(), pos[1].copy(), pos[2].copy(), zeros,
+                        zeros.copy(), zeros.copy(), np.zeros((3, n), dtype=np.float32), q, half,
+                        np.zeros(n, dtype=np.float32), np.full(n, np.float32(1 / 0.5), dtype=np.float32), box, 240,
+                        dt, 7, 0)
 
 
 @dataclass
