@@ -376,17 +376,35 @@ __device__ __forceinline__ void turn(const SphereDev &D, int s, const float angl
     D.qx[s] = now[0] * inverse; D.qy[s] = now[1] * inverse; D.qz[s] = now[2] * inverse; D.qw[s] = now[3] * inverse;
 }
 
-// Spheres.around, Spheres.java:1207-1229: the 27 cells about `key`, each cell's spheres in sorted order.
+// Spheres.around, Spheres.java:1207-1229: the 27 cells about `key`, each cell's spheres in sorted order. The walk
+// takes x fastest, then y, then z; the (up to) three cells of one y, z row are consecutive cells, so their spheres are
+// one run of the sorted order, from the first cell's start to the end of the last. So the walk is 9 runs, whose 18
+// bounds are loaded together (one memory latency, not one or two a cell), visited in the walk's order.
+struct Runs {
+    int lo[9], hi[9];
+};
+
+__device__ __forceinline__ Runs runs_around(const SphereDev &D, int key) {
+    int ix = key % D.nx, iy = (key / D.nx) % D.ny, iz = key / (D.nx * D.ny);
+    int x0 = max(ix - 1, 0), x1 = min(ix + 1, D.nx - 1);
+    Runs R;
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        int jy = iy + (r % 3 - 1), jz = iz + (r / 3 - 1);
+        bool inside = jy >= 0 && jy < D.ny && jz >= 0 && jz < D.nz;
+        int row = D.nx * (jy + D.ny * jz);
+        R.lo[r] = inside ? D.starts[row + x0] : 0;
+        R.hi[r] = inside ? D.starts[row + x1 + 1] : 0;
+    }
+    return R;
+}
+
 template <class Visit>
 __device__ __forceinline__ void around(const SphereDev &D, int key, Visit visit) {
-    int ix = key % D.nx, iy = (key / D.nx) % D.ny, iz = key / (D.nx * D.ny);
-    for (int m = 0; m < 27; m++) {
-        int jx = ix + (m % 3 - 1), jy = iy + ((m / 3) % 3 - 1), jz = iz + (m / 9 - 1);
-        if (jx < 0 || jx >= D.nx || jy < 0 || jy >= D.ny || jz < 0 || jz >= D.nz) continue;
-        int cell = jx + D.nx * (jy + D.ny * jz);
-        int end = D.starts[cell + 1];
-        for (int k = D.starts[cell]; k < end; k++) visit(D.order[k]);
-    }
+    const Runs R = runs_around(D, key);
+#pragma unroll
+    for (int r = 0; r < 9; r++)
+        for (int k = R.lo[r]; k < R.hi[r]; k++) visit(D.order[k]);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -484,6 +502,29 @@ __device__ __forceinline__ bool near_pair(const SphereDev &D, const float mine[3
     return d2 < near * near;
 }
 
+// The spheres of the walk about `key` of higher index than a and within the list margin of it, handed to `take` in
+// the walk's order. The candidates of a run are loaded four at a time, their tests made, and only then handed on,
+// so `take`'s stores do not hold back the next candidates' loads.
+template <class Take>
+__device__ __forceinline__ void near_around(const SphereDev &D, int key, int a, const float mine[3], float ra,
+                                            Take take) {
+    const Runs R = runs_around(D, key);
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        for (int k = R.lo[r]; k < R.hi[r]; k += 4) {
+            int o[4];
+            bool near[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) o[q] = k + q < R.hi[r] ? D.order[k + q] : -1;
+#pragma unroll
+            for (int q = 0; q < 4; q++) near[q] = o[q] > a && near_pair(D, mine, ra, o[q]);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (near[q]) take(o[q]);
+        }
+    }
+}
+
 __global__ void k_list_count(SphereDev D) {
     // Threads take spheres in cell order, so a wave walks neighbouring cells together.
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -492,9 +533,7 @@ __global__ void k_list_count(SphereDev D) {
     float mine[3] = {D.x[a], D.y[a], D.z[a]};
     float ra = D.r[a];
     int found = 0;
-    around(D, D.keys[a], [&](int o) {
-        if (o > a && near_pair(D, mine, ra, o)) found++;
-    });
+    near_around(D, D.keys[a], a, mine, ra, [&](int) { found++; });
     D.cur.cnt[a] = found;
 }
 
@@ -510,20 +549,29 @@ __global__ void k_list_write(SphereDev D) {
     int kStart = D.kept.off[a];
     int kEnd = min(kStart + D.kept.cnt[a], cap);
     int kLow = max(kStart, kEnd - MOST_KEPT);
-    around(D, D.keys[a], [&](int o) {
-        if (!(o > a) || !near_pair(D, mine, ra, o)) return;
+    // The last KEPT_HELD partners of a's kept row, from its end back (-1 past its start), loaded once for the whole
+    // walk: the search for each pair's old place looks through them first, then through the rest of the row in
+    // memory, in the same order, so it finds the same place.
+    constexpr int KEPT_HELD = 8;
+    int held[KEPT_HELD];
+#pragma unroll
+    for (int q = 0; q < KEPT_HELD; q++) held[q] = kEnd - 1 - q >= kLow ? D.kept.b[kEnd - 1 - q] : -1;
+    near_around(D, D.keys[a], a, mine, ra, [&](int o) {
         int here = slot++;
         if (here >= cap) return;
         D.cur.a[here] = a;
         D.cur.b[here] = o;
         D.cur.done[here] = 0;
+        int was = -1;
+#pragma unroll
+        for (int q = 0; q < KEPT_HELD; q++)
+            if (was < 0 && held[q] == o) was = kEnd - 1 - q;
+        for (int j = kEnd - 1 - KEPT_HELD; was < 0 && j >= kLow; j--)
+            if (D.kept.b[j] == o) was = j;
         float warmth[IMPULSE_WORDS] = {0, 0, 0, 0, 0, 0, 0};
-        for (int j = kEnd - 1; j >= kLow; j--) {
-            if (D.kept.b[j] == o) {
-                const float *src = D.kept.impulse + (size_t)j * IMPULSE_WORDS;
-                for (int k = 0; k < IMPULSE_WORDS; k++) warmth[k] = src[k];
-                break;
-            }
+        if (was >= 0) {
+            const float *src = D.kept.impulse + (size_t)was * IMPULSE_WORDS;
+            for (int k = 0; k < IMPULSE_WORDS; k++) warmth[k] = src[k];
         }
         float *dst = D.cur.impulse + (size_t)here * IMPULSE_WORDS;
         for (int k = 0; k < IMPULSE_WORDS; k++) dst[k] = warmth[k];
