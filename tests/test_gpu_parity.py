@@ -76,9 +76,10 @@ def test_pile_with_friction_matches_oracle_bitwise():
 
 @pytest.mark.parametrize("plan", ["wide", "local"])
 def test_both_pass_plans_match_the_oracle_bitwise(plan, monkeypatch):
-    """A Gauss-Seidel pass runs as chip-wide round launches finished by one workgroup ("wide"), or wholly in one
-    workgroup with the claim arrays in LDS ("local"); VR_SPHERE_PLAN fixes the plan. Both run the reference's rounds:
-    each matches the oracle to the bit, reports included."""
+    """A Gauss-Seidel pass runs as chip-wide round launches finished by one workgroup ("wide"), or as wide rounds
+    while more than 1,024 contacts are open and then one workgroup with the claim arrays in LDS ("local");
+    VR_SPHERE_PLAN fixes the plan. Both run the reference's rounds: each matches the oracle to the bit, reports
+    included."""
     monkeypatch.setenv("VR_SPHERE_PLAN", plan)
     a, b = run_both(friction_pile, 4, extra=("contactCount", "contactImpulse"))
     assert a["reports"] == b["reports"]
@@ -105,10 +106,10 @@ def test_the_local_plan_over_rounds_longer_than_its_workgroup_matches_the_oracle
     assert_bits(a, b, "crowded pile, local plan")
 
 
-def test_the_plan_chosen_step_by_step_matches_the_oracle(monkeypatch):
-    """Unfixed, the plan is local for a first step of at most 1,024 spheres and then follows the contacts the step
-    before listed a pass: the crowded pile (up to 5,261 contacts listed a substep in its first step) starts local,
-    goes wide, and stays bit-exact across the change."""
+def test_the_local_plans_wide_rounds_changing_between_steps_match_the_oracle(monkeypatch):
+    """Unfixed, a world of at most 6,144 spheres takes the local plan, whose wide rounds a pass are those the step
+    before needed to bring the pass to 1,024 open contacts: on the crowded pile (up to 5,261 contacts listed a
+    substep in its first step) they change from step to step, and the state stays bit-exact."""
     monkeypatch.delenv("VR_SPHERE_PLAN", raising=False)
     a, b = run_both(crowded_pile, 3, extra=("contactCount",))
     assert a["reports"] == b["reports"]

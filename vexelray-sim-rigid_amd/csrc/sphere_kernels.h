@@ -55,6 +55,7 @@ constexpr int SEG = 256;      // open-list places a wide round's workgroup owns 
 struct PassSlot {
     int tailCount;          // contacts the pass's last wide round left open, compacted for the tail
     int opened[MAX_WIDE];   // opened[j] != 0: round j had an open contact to take
+    int openAfter[MAX_WIDE];  // contacts left open by wide round j (every wide round but the last)
 };
 
 struct PassCtl {
@@ -64,6 +65,8 @@ struct PassCtl {
     int submissions;                    // the reference's submissions for the step's opening and more lists
     long long listedSum;                // contacts listed, added up over every pass (parting and rubbing)
     int wideRan[MAX_WIDE];              // per pass of a substep: the most rounds run wide this step
+    int localFrom[MAX_WIDE];            // per pass of a substep: the latest first round with at most LOCAL_THREADS
+                                        // contacts open, over the step's substeps (the local plan's wide rounds)
 };
 
 // Round j's open contacts, as {c, a, o, 0}: in segment g of buf[j & 1], seg[j & 1][g] of them, for the wide rounds
@@ -97,9 +100,11 @@ struct PassSubmit {
     int prefix, carry, slice;
 };
 
-// `local`: the whole pass in one workgroup of LOCAL_THREADS with the claims in LDS (no wide rounds), for at most
-// LOCAL_MOST spheres. It pays while a round's open contacts fit in one go of the workgroup's threads: each further
-// go waits out the memory latency again, where the wide rounds take them side by side.
+// `local`: the pass's tail in one workgroup of LOCAL_THREADS with the claims in LDS, for at most LOCAL_MOST spheres;
+// with `wide` 0, the whole pass. A round in one workgroup pays while its open contacts fit in one go of the
+// workgroup's threads: each further go waits out the memory latency again, where the wide rounds take them side by
+// side. So the local plan launches wide rounds until a pass has at most LOCAL_THREADS contacts open (as the step
+// before found, PassCtl::localFrom) and finishes in the workgroup.
 constexpr int LOCAL_MOST = 6144, LOCAL_THREADS = 1024;
 void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLists &L, int32_t *opening,
                  int32_t *enough, int it, int iterations, bool rub, int wide, bool local, const PassSubmit &submit,

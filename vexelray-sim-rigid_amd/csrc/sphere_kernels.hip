@@ -759,6 +759,7 @@ __global__ void k_step_begin(PassCtl *ctl, PassSlot *slots, int words) {
         ctl->unlisted = 0;
         ctl->listedSum = 0;
         for (int k = 0; k < MAX_WIDE; k++) ctl->wideRan[k] = 0;
+        for (int k = 0; k < MAX_WIDE; k++) ctl->localFrom[k] = 0;
     }
 }
 
@@ -820,7 +821,10 @@ __global__ void __launch_bounds__(THREADS) k_round_wide(SphereDev D, const PassC
     } else {
         append_open(open, e, out + (size_t)g * SEG, &sCount);
         __syncthreads();
-        if (threadIdx.x == 0) segOut[g] = sCount;
+        if (threadIdx.x == 0) {
+            segOut[g] = sCount;
+            if (sCount > 0) atomicAdd(&slot->openAfter[j], sCount);
+        }
     }
 }
 
@@ -853,6 +857,13 @@ __global__ void __launch_bounds__(T) k_round_tail(SphereDev D, PassCtl *ctl, con
         }
         n = j < wide ? 0 : slot->tailCount;
     }
+    // The first round this pass entered with at most LOCAL_THREADS contacts open: round 0 takes the listed, wide
+    // round j + 1 what wide round j left (its last, the tail's list); later rounds are found below. -1: not yet.
+    int fits = -1;
+    if (listed <= LOCAL_THREADS) fits = 0;
+    for (int r = 1; fits < 0 && r < wide; r++)
+        if (slot->openAfter[r - 1] <= LOCAL_THREADS) fits = r;
+    if (fits < 0 && wide > 0 && j == wide && n <= LOCAL_THREADS) fits = wide;
     int open0 = RUB ? 0 : opening[it];
     int kRub = RUB ? ctl->k : 0;
     const Params P = load_params(D.params);
@@ -901,6 +912,7 @@ __global__ void __launch_bounds__(T) k_round_tail(SphereDev D, PassCtl *ctl, con
         __syncthreads();
         n = sCount;
         j++;
+        if (fits < 0 && n <= LOCAL_THREADS) fits = j;
         __syncthreads();
     }
     if (LOCAL) {
@@ -913,6 +925,7 @@ __global__ void __launch_bounds__(T) k_round_tail(SphereDev D, PassCtl *ctl, con
     // The rounds this pass needed, for the host to launch that many wide next step.
     int used = (RUB ? iterations : 0) + it;
     if (threadIdx.x == 0 && used < MAX_WIDE) ctl->wideRan[used] = max(ctl->wideRan[used], j);
+    if (threadIdx.x == 0 && used < MAX_WIDE) ctl->localFrom[used] = max(ctl->localFrom[used], fits < 0 ? j : fits);
     if (threadIdx.x == 0) ctl->listedSum += listed;
     if (threadIdx.x != 0 || RUB) return;
     if (failed) {
@@ -1469,7 +1482,7 @@ void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLis
     if (wide < 0 || wide > MAX_WIDE) throw std::runtime_error("pass: wide rounds out of range");
     if ((int64_t)L.segments * SEG < D.cap) throw std::runtime_error("pass: open lists shorter than the contact list");
     int limit = D.cap + 2 * CYCLE;
-    if (L.segments == 0 || local) wide = 0;  // no contact can be listed, or the tail takes the whole pass
+    if (L.segments == 0) wide = 0;  // no contact can be listed: the tail alone finds the pass empty
     for (int j = 0; j < wide; j++) {
         int last = j == wide - 1 ? 1 : 0;
         const int4 *in = L.buf[j & 1];

@@ -75,8 +75,7 @@ struct vr_world {
     OpenLists openLists{};
     int32_t *devOpening = nullptr, *devEnough = nullptr;  // SphereStepper's opening batches, kept on the device
     std::vector<int> wide, wideRub;    // wide rounds to launch per pass, from what the last step used
-    bool localPasses = false;          // every pass in one workgroup, the claims in LDS (launch_pass's `local`)
-    bool planFixed = false;            // VR_SPHERE_PLAN set: localPasses is not revised after a step
+    bool localPasses = false;          // each pass finished in one workgroup, the claims in LDS (launch_pass's `local`)
     bool hostWaits = false;            // run the reference's launch model: host reads each report
     int slice = 0;                     // SphereSimulation.slice: most dispatches a submission, 0 for no bound
     double *diagPartials = nullptr, *diagOut = nullptr;  // device scratch of vr_world_diagnostics, made on first use
@@ -119,16 +118,14 @@ struct vr_world {
         // VR_HOST_WAITS=1 runs a Gauss-Seidel pass as the reference's stepper does, the host reading each report.
         const char *hw = std::getenv("VR_HOST_WAITS");
         hostWaits = hw && hw[0] == '1';
-        // Each pass in one workgroup with its claims in LDS (a round a barrier, not a launch) while a pass lists
-        // at most LOCAL_THREADS contacts: guessed from the spheres for the first step, then from the contacts the
-        // step before listed a pass (read after step_device's wait). Both plans run the same rounds.
-        // VR_SPHERE_PLAN=wide|local fixes the choice (local only up to LOCAL_MOST spheres).
-        localPasses = n <= LOCAL_THREADS;
+        // A world of at most LOCAL_MOST spheres finishes each pass in one workgroup with its claims in LDS (a round
+        // a barrier, not a launch), after wide rounds while more than LOCAL_THREADS contacts are open. Both plans
+        // run the same rounds. VR_SPHERE_PLAN=wide|local fixes the choice (local only up to LOCAL_MOST spheres).
+        localPasses = n <= LOCAL_MOST;
         if (const char *plan = std::getenv("VR_SPHERE_PLAN")) {
             if (std::string(plan) == "wide") localPasses = false;
             else if (std::string(plan) == "local") localPasses = n <= LOCAL_MOST;
             else throw ArgError("VR_SPHERE_PLAN is wide or local, got " + std::string(plan));
-            planFixed = true;
         }
         HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIP_OK(hipEventCreate(&t0));
@@ -377,7 +374,7 @@ struct vr_world {
         }
         launch_pass(D, ctl, slots + q, openLists, devOpening, devEnough, it, d.iterations, rub, wideRounds,
                     localPasses, submit, stream);
-        launchesThisStep += localPasses ? 1 : wideRounds + 1;
+        launchesThisStep += wideRounds + 1;
         if (profiling) HIP_OK(hipEventRecord(passEvents[2 * q + 1], stream));
     }
 
@@ -433,8 +430,6 @@ struct vr_world {
             prof.steps += 1;
             prof.enqueue_ns += enqueued;
         }
-        if (!planFixed && q > 0)
-            localPasses = n <= LOCAL_MOST && ctlHost->listedSum <= (long long)LOCAL_THREADS * q;
         rep->waits = ctlHost->waits;
         rep->submissions = ctlHost->submissions + pieces(closing_passes(friction ? ctlHost->k : 0));
         rep->rounds = ctlHost->rounds;
@@ -443,8 +438,12 @@ struct vr_world {
         // Wide rounds for the next step: as many as this one needed, and one more.
         for (int it = 0; it < d.iterations; it++) {
             int parted = it, rubbed = d.iterations + it;
-            if (parted < MAX_WIDE) wide[it] = std::min(ctlHost->wideRan[parted] + 1, MAX_WIDE);
-            if (rubbed < MAX_WIDE) wideRub[it] = std::min(ctlHost->wideRan[rubbed] + 1, MAX_WIDE);
+            // The local plan: wide rounds until the pass fits one workgroup, as this step found; else as many as
+            // the pass needed, and one more, so the tail finds it done.
+            const int *from = localPasses ? ctlHost->localFrom : ctlHost->wideRan;
+            int more = localPasses ? 0 : 1;
+            if (parted < MAX_WIDE) wide[it] = std::min(from[parted] + more, MAX_WIDE);
+            if (rubbed < MAX_WIDE) wideRub[it] = std::min(from[rubbed] + more, MAX_WIDE);
         }
     }
 
