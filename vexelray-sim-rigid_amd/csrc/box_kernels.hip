@@ -1457,30 +1457,48 @@ __global__ void __launch_bounds__(THREADS) k_list(BoxDev D) {
     const SceneDev sc = D.sceneInfo[D.scene[s]];
     int key = D.keys[s] - sc.cellBase;
     int ix = key % sc.nx, iy = (key / sc.nx) % sc.ny, iz = key / (sc.nx * sc.ny);
-    for (int m = 0; m < 27; m++) {
-        int jx = ix + (m % 3 - 1), jy = iy + ((m / 3) % 3 - 1), jz = iz + (m / 9 - 1);
-        if (jx < 0 || jx >= sc.nx || jy < 0 || jy >= sc.ny || jz < 0 || jz >= sc.nz) continue;
-        int cell = sc.cellBase + (jx + sc.nx * (jy + sc.ny * jz));
-        int end = D.starts[cell + 1];
-        for (int k = D.starts[cell]; k < end; k++) {
-            const float4 at = D.broad[2 * (size_t)k];
-            int o = __float_as_int(at.w);
-            if (!(o > s)) continue;
-            const float4 reach = D.broad[2 * (size_t)k + 1];
-            float where[3] = {at.x, at.y, at.z}, other[3] = {reach.x, reach.y, reach.z};
-            bool close = true;
-            for (int a = 0; a < 3; a++)
-                close = close && !(fabsf(where[a] - centre[a]) > (rs[a] + other[a]) + margin);
-            if (!close) continue;
-            if (found < SLOTS) {
-                int place = found;
-                while (place > 0 && part[(place - 1) * THREADS] > o) {
-                    part[place * THREADS] = part[(place - 1) * THREADS];
-                    place--;
-                }
-                part[place * THREADS] = o;
+    // The 27 cells x fastest, then y, then z (BoxBroad.around, BoxBroad.java:230): the (up to) three x cells of a
+    // y, z row are consecutive cells, so their bodies are one run of the cell order. 9 runs, their 18 bounds loaded
+    // together, their records four at a time, each run in the walk's order.
+    int x0 = max(ix - 1, 0), x1 = min(ix + 1, sc.nx - 1);
+    int lo[9], hi[9];
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        int jy = iy + (r % 3 - 1), jz = iz + (r / 3 - 1);
+        bool inside = jy >= 0 && jy < sc.ny && jz >= 0 && jz < sc.nz;
+        int row = sc.cellBase + sc.nx * (jy + sc.ny * jz);
+        lo[r] = inside ? D.starts[row + x0] : 0;
+        hi[r] = inside ? D.starts[row + x1 + 1] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        for (int k0 = lo[r]; k0 < hi[r]; k0 += 4) {
+            float4 at[4], reach[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                bool in = k0 + q < hi[r];
+                at[q] = in ? D.broad[2 * (size_t)(k0 + q)] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+                reach[q] = in ? D.broad[2 * (size_t)(k0 + q) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            found++;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int o = __float_as_int(at[q].w);
+                if (!(o > s)) continue;
+                float where[3] = {at[q].x, at[q].y, at[q].z}, other[3] = {reach[q].x, reach[q].y, reach[q].z};
+                bool close = true;
+                for (int a = 0; a < 3; a++)
+                    close = close && !(fabsf(where[a] - centre[a]) > (rs[a] + other[a]) + margin);
+                if (!close) continue;
+                if (found < SLOTS) {
+                    int place = found;
+                    while (place > 0 && part[(place - 1) * THREADS] > o) {
+                        part[place * THREADS] = part[(place - 1) * THREADS];
+                        place--;
+                    }
+                    part[place * THREADS] = o;
+                }
+                found++;
+            }
         }
     }
     for (int m = 0; m < SLOTS; m++) {
