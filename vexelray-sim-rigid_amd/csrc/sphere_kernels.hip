@@ -779,10 +779,14 @@ __device__ __forceinline__ int4 open_entry(const int4 *list, size_t place) {
 // one another, so all three are loaded before any of them is tested: one memory latency, not three, before the
 // contact's own loads.
 // FIRST: round 0 (j == 0), an instance of its own, so neither instance joins two ways of loading.
+// counted: each workgroup adds what it left open to slot->openAfter[j], which only the local plan's tail reads (to
+// find the first round that fits its workgroup). Every workgroup's add lands on the same word, which costs a few
+// microseconds a round when hundreds of workgroups take part, so the wide plan leaves it out.
 template <bool RUB, bool FIRST>
 __global__ void __launch_bounds__(THREADS) k_round_wide(SphereDev D, const PassCtl *ctl, PassSlot *slot,
                                                         const int4 *in, const int32_t *segIn, int4 *out,
-                                                        int32_t *segOut, int it, int iterations, int j, int last) {
+                                                        int32_t *segOut, int it, int iterations, int j, int last,
+                                                        int counted) {
     __shared__ int sCount;
     int g = blockIdx.x, i = threadIdx.x;
     int kRub = RUB ? ctl->k : 0;
@@ -823,7 +827,7 @@ __global__ void __launch_bounds__(THREADS) k_round_wide(SphereDev D, const PassC
         __syncthreads();
         if (threadIdx.x == 0) {
             segOut[g] = sCount;
-            if (sCount > 0) atomicAdd(&slot->openAfter[j], sCount);
+            if (counted && sCount > 0) atomicAdd(&slot->openAfter[j], sCount);
         }
     }
 }
@@ -925,7 +929,8 @@ __global__ void __launch_bounds__(T) k_round_tail(SphereDev D, PassCtl *ctl, con
     // The rounds this pass needed, for the host to launch that many wide next step.
     int used = (RUB ? iterations : 0) + it;
     if (threadIdx.x == 0 && used < MAX_WIDE) ctl->wideRan[used] = max(ctl->wideRan[used], j);
-    if (threadIdx.x == 0 && used < MAX_WIDE) ctl->localFrom[used] = max(ctl->localFrom[used], fits < 0 ? j : fits);
+    if (LOCAL && threadIdx.x == 0 && used < MAX_WIDE)  // the wide plan counts no openAfter and reads no localFrom
+        ctl->localFrom[used] = max(ctl->localFrom[used], fits < 0 ? j : fits);
     if (threadIdx.x == 0) ctl->listedSum += listed;
     if (threadIdx.x != 0 || RUB) return;
     if (failed) {
@@ -1491,7 +1496,8 @@ void launch_pass(const SphereDev &D, PassCtl *ctl, PassSlot *slot, const OpenLis
         int32_t *segOut = L.seg[(j + 1) & 1];
         auto kernel = rub ? (j == 0 ? k_round_wide<true, true> : k_round_wide<true, false>)
                           : (j == 0 ? k_round_wide<false, true> : k_round_wide<false, false>);
-        kernel<<<L.segments, THREADS, 0, s>>>(D, ctl, slot, in, segIn, out, segOut, it, iterations, j, last);
+        kernel<<<L.segments, THREADS, 0, s>>>(D, ctl, slot, in, segIn, out, segOut, it, iterations, j, last,
+                                              local ? 1 : 0);
     }
     if (local) {
         if (D.n > LOCAL_MOST) throw std::runtime_error("pass: a pass in LDS takes at most LOCAL_MOST spheres");
