@@ -6,8 +6,8 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--run R] [--cpu-seconds S] 
 The workload (batched_2k) is the batched configuration: 2,048 independent scenes of log-uniform 16-4,096 bodies
 (1,549,388 in all, half boxes and half balls, mu 0.6), stepped together at dt 1/60 s in 8 substeps of at most 10 solve
 and 10 relax sweeps, over the configuration's R = 300 steps. The timed window is the run's last K steps, after
-R - K - W - 10 untimed steps, 10 in which the batch times its launch plans afresh for the method (vr_box_retune) and
-W warm-up steps: the piles as the run leaves them, not the free fall of its first steps.
+R - K - W - 11 untimed steps, one measured step, 10 in which the batch times its launch plans afresh for the method
+(vr_box_retune) and W warm-up steps: the piles as the run leaves them, not the free fall of its first steps.
 All state is resident in HBM before the timed region; nothing crosses PCIe inside it but each step's small report.
 
 Three runs reach the same state at the window (the stepping is deterministic) and differ only inside it:
@@ -19,9 +19,10 @@ Three runs reach the same state at the window (the stepping is deterministic) an
             elsewhere).
 A point's residual is the impulse change a sweep makes there times the inverse mass along the normal (m/s). stopped and
 learned end every scene's phases at the same residual, the tolerance (or at the 10-sweep cap), so learned / stopped is
-the learned start's gain at equal residual; plain runs every sweep and ends below it. After the window each run steps
-once measured at its tolerance, then once at 1e-5 m/s: each phase's sweeps a scene ran, its residual (largest and mean
-over the scenes), and the deepest manifold point.
+the learned start's gain at equal residual; plain runs every sweep and ends below it. Each run steps once measured at
+its tolerance from the state all three share (before the window; equal_residual compares these), and after the window
+once more at its tolerance and once at 1e-5 m/s: each phase's sweeps a scene ran, its residual (largest and mean over
+the scenes), and the deepest manifold point.
 
 value = learned body-substeps per second = bodies x substeps x K / seconds. The line carries:
   plain, stopped, learned: each run's rate, residuals, deepest point, sweeps;
@@ -160,18 +161,21 @@ TUNE_STEPS = 10  # untimed steps in which a batch times its launch plans (two ea
 
 
 def batch_run(pkg, label, spec, args, net=None, tol=-1.0, profile=False, keep_state=False):
-    """Steps the batch R - K - W - 10 steps plain, then with `net` and `tol` 10 steps in which it times its launch
-    plans afresh (vr_box_retune: the plans' costs in the settled piles are not those of the free fall the batch timed
-    them in), W warm-up and K timed steps, then one measured step."""
+    """Steps the batch R - K - W - 11 steps plain, then with `net` and `tol` one measured step (from the state every
+    run reaches there, stepping being deterministic, so the runs' residuals differ only by their method), 10 steps in
+    which it times its launch plans afresh (vr_box_retune: the plans' costs in the settled piles are not those of the
+    free fall the batch timed them in), W warm-up and K timed steps, then two measured steps (at `tol` and at
+    1e-5 m/s)."""
     _, boxes, _ = modules(pkg)
     sim = batch_of(boxes, spec)
-    settle = args.run - args.steps - args.warmup - TUNE_STEPS
+    settle = args.run - args.steps - args.warmup - TUNE_STEPS - 1
     for k in range(settle):
         sim.stepAndWait()
         progress(f"{label} settle", k + 1, settle)
     state = {k: sim.floats(k).copy() for k in STATE + ("hx", "hy", "hz", "shape", "im")} if keep_state else None
     if net is not None:
         sim.learned(net)
+    shared = measured_step(sim, tol)
     sim.tolerance(tol, False)
     sim.retune()
     for _ in range(TUNE_STEPS):
@@ -205,6 +209,7 @@ def batch_run(pkg, label, spec, args, net=None, tol=-1.0, profile=False, keep_st
         "touching_at_start": reps[0].touching, "touching_at_end": reps[-1].touching,
         "rounds_at_end": reps[-1].rounds, "points_at_end": reps[-1].points,
         "unplaced": int(sum(r.unplaced for r in reps)), "unlisted": int(sum(r.unlisted for r in reps)),
+        "from_shared_state": shared,
         "after_window": after,
         "at_1e-5": {"sweeps_per_scene_per_substep": tight["sweeps_per_scene_per_substep"],
                     "solve_sweeps_per_scene_by_substep": tight["solve_sweeps_per_scene_by_substep"],
@@ -285,9 +290,9 @@ def box_roofline(prof, walls_per_sweep):
 
 
 def cpu_baseline(pkg, spec, state, seconds, scenes_taken=48):
-    """The oracle on the batch's first `scenes_taken` scenes, started from the plain run's state at the window's start
-    (positions, velocities, turns and momenta; the impulses from zero), stepped on one thread and on all of them for
-    about `seconds` each."""
+    """The oracle on the batch's first `scenes_taken` scenes, started from the plain run's state where the three runs
+    part, 11 steps before the window (positions, velocities, turns and momenta; the impulses from zero), stepped on
+    one thread and on all of them for about `seconds` each."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import scene as sc  # the oracle's loader lives with the tests
     import ctypes
@@ -323,8 +328,8 @@ def cpu_baseline(pkg, spec, state, seconds, scenes_taken=48):
         "one_core": {"value": one[0], "steps": one[1], "seconds": one[2]},
         "cpu": cpu_model(), "nproc": os.cpu_count(),
         "sample": f"the batch's first {scenes_taken} scenes ({n} bodies), {spec.substeps} substeps of {spec.sweeps} "
-                  f"sweeps, on the CPU oracle (C++ -O3, the scenes on OpenMP threads), started from the GPU's state at "
-                  f"the plain window's start (impulses from zero): {most[1]} steps in {most[2]:.1f} s on "
+                  f"sweeps, on the CPU oracle (C++ -O3, the scenes on OpenMP threads), started from the GPU's state "
+                  f"11 steps before the plain window (impulses from zero): {most[1]} steps in {most[2]:.1f} s on "
                   f"{max(out)} threads, {one[1]} steps in {one[2]:.1f} s on one",
     }
 
@@ -443,11 +448,13 @@ def batched(pkg, args):
     net.close()
     rounds, walls = box_roofline(prof, plain["after_window"]["active_wall_contacts"])
     within = lambda a, b: abs(a - b) <= 0.01 * abs(b)  # noqa: E731
-    compare = {}
+    # Each run's measured step from the state all three share (before the window): the start and the stop alone
+    # make the difference. After the window the runs have stepped 31 steps apart, so their piles differ.
+    compare = {"measured": "one step from the state the three runs share, before the window"}
     for key in ("residual_max_m_per_s", "residual_mean_m_per_s", "deepest_point_m"):
-        a, b = learned["after_window"][key], stopped["after_window"][key]
+        a, b = learned["from_shared_state"][key], stopped["from_shared_state"][key]
         compare[key] = {"stopped": b, "learned": a, "within_1pct": within(a, b),
-                        "plain": plain["after_window"][key]}
+                        "plain": plain["from_shared_state"][key]}
     learned["network"] = {
         "samples": samples, "held_out_samples": int(len(held_x)), "scenes_logged": seen, "epochs": args.epochs,
         "train_s": train_s, "trained_on": f"{args.train_batches - 1} plain batches of 2,048 scenes (seeds 100..), "
