@@ -366,6 +366,45 @@ def window_kind(name):
     return None
 
 
+def summarize_steps(trace, steps, out):
+    """Markdown of a kernel trace's last `steps` steps, kernel by kernel: dispatches, mean and total duration and share.
+    A step ends at its one `k_show` dispatch (the shown words, written last in every step of a sphere or box world),
+    so the window is every dispatch after the (steps + 1)-th last `k_show` up to the last. The trace may be gzipped."""
+    import csv
+    import gzip
+    opener = gzip.open if trace.endswith(".gz") else open
+    with opener(trace, "rt") as f:
+        rows = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"])
+                      for r in csv.DictReader(f))
+    shows = [i for i, r in enumerate(rows) if "k_show" in r[2]]
+    if len(shows) <= steps:
+        raise SystemExit(f"the trace holds {len(shows)} steps, fewer than {steps} + 1")
+    window = rows[shows[-steps - 1] + 1:shows[-1] + 1]
+
+    def kernel(name):
+        name = name.replace("void ", "").replace("vr::(anonymous namespace)::", "")
+        return name.replace("vrbox::(anonymous namespace)::", "").split("(")[0]
+
+    stats = {}
+    for start, end, name in window:
+        k = stats.setdefault(kernel(name), [0, 0])
+        k[0] += 1
+        k[1] += end - start
+    busy = sum(v[1] for v in stats.values())
+    lines = [f"The last {steps} steps: {len(window)} dispatches, {busy / 1e6 / steps:.2f} ms of kernel time a step.", "",
+             "| Kernel | Dispatches | Mean us | Total ms | Share of the window's kernel time |",
+             "| --- | --- | --- | --- | --- |"]
+    for k, (n, t) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"| `{k}` | {n} | {t / n / 1e3:.2f} | {t / 1e6:.2f} | {100 * t / busy:.2f} % |")
+    rounds = [v for k, v in stats.items() if k.startswith("k_round_wide") or k.startswith("k_round_tail")]
+    if rounds:
+        n, t = sum(v[0] for v in rounds), sum(v[1] for v in rounds)
+        lines += ["", f"Round kernels (`k_round_wide`, `k_round_tail`): {n} dispatches ({n / steps:.0f} a step), "
+                      f"mean {t / n / 1e3:.3f} us, {t / 1e6 / steps:.2f} ms a step."]
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def summarize(trace, counters, line, out):
     """Markdown of a `plain_only` run's timed window from rocprofv3's output: the solve and relax sweeps' round and
     walls dispatches of the plain method (their template's last argument 0) are the window's last `launches` of each
@@ -657,6 +696,9 @@ def main():
     ap.add_argument("--summarize", nargs=3, metavar=("TRACE_OR_-", "COUNTERS_OR_-", "LINE_JSON"),
                     help="write the timed window's kernels from rocprofv3 CSVs (kernel trace or counter collection) "
                          "and a --only plain line to --summary-out, then exit")
+    ap.add_argument("--summarize-steps", nargs=2, metavar=("TRACE", "STEPS"),
+                    help="write the last STEPS steps' kernels of a rocprofv3 kernel trace (csv, or csv.gz) to "
+                         "--summary-out, then exit")
     ap.add_argument("--summary-out", default="window.md")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB: only copy a buffer of that size 5 times (the known-size kernel that calibrates the "
@@ -674,6 +716,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": src.numel() * 4, "copy_probe_bytes_written": dst.numel() * 4}))
+        return
+    if args.summarize_steps:
+        summarize_steps(args.summarize_steps[0], int(args.summarize_steps[1]), args.summary_out)
         return
     if args.summarize:
         trace, counters, line = args.summarize
