@@ -589,13 +589,16 @@ SWEEP_PUBLISHED = {
 }
 
 
-def reference_scenes(pkg):
+def reference_scenes(pkg, only=""):
     """The reference's measured scenes, each timed as its test does: the sphere scenes as SphereSweepTest.run
     (SphereSweepTest.java:175-182), 2 frames untimed and then the rest of the run's frames; the box piles as
-    BoxStackTest.pile (:213-216), 3 s untimed and then 1 s; each step waited for, on the wall clock."""
+    BoxStackTest.pile (:213-216), 3 s untimed and then 1 s; each step waited for, on the wall clock. `only` names one
+    scene (sweep_column, sweep_pile_343, sweep_pile_4096, box_pile_343, box_pile_1000) to time alone."""
     scenes, boxes, _ = modules(pkg)
     out = {}
     for spec in (scenes.sweep_column(), scenes.sweep_pile(7), scenes.sweep_pile(16)):
+        if only and spec.name != only:
+            continue
         sim = pkg.SphereSimulation(spec.step)
         sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.r, spec.im, spec.params)
         for _ in range(2):
@@ -618,6 +621,8 @@ def reference_scenes(pkg):
         }
         progress("reference scenes", len(out), 5)
     for spec in (scenes.box_pile(7), scenes.box_pile(10)):
+        if only and spec.name != only:
+            continue
         sim = boxes.BoxSimulation(spec.step)
         sim.start(spec.x, spec.y, spec.z, spec.u, spec.v, spec.w, spec.spin, spec.q, spec.half, spec.shape, spec.im,
                   spec.params)
@@ -692,7 +697,7 @@ def main():
                     help="1: also time the 1,000-sphere pile (last 20 of 300 steps), the mixed box pile (last 20 "
                          "of 600) and the reference's measured sphere scenes as sub-keys")
     ap.add_argument("--only", default="", help="plain: the plain run alone (for the profilers); scenes: the "
-                    "reference's measured sphere scenes alone; one JSON line")
+                    "reference's measured scenes alone (scenes:NAME, one of them); one JSON line")
     ap.add_argument("--summarize", nargs=3, metavar=("TRACE_OR_-", "COUNTERS_OR_-", "LINE_JSON"),
                     help="write the timed window's kernels from rocprofv3 CSVs (kernel trace or counter collection) "
                          "and a --only plain line to --summary-out, then exit")
@@ -730,8 +735,8 @@ def main():
     if args.only == "plain":
         print(json.dumps(plain_only(pkg, args)))
         return
-    if args.only == "scenes":
-        print(json.dumps(reference_scenes(pkg)))
+    if args.only == "scenes" or args.only.startswith("scenes:"):
+        print(json.dumps(reference_scenes(pkg, args.only[len("scenes:"):])))
         return
     if args.workload == "mixed_32k":
         print(json.dumps(box_scene(pkg, args)))

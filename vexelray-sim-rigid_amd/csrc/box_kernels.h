@@ -208,6 +208,10 @@ struct Groups {
     int32_t *bodies;  // 2 a group: its first body, and one past its last
     int32_t *order;   // n: places bodies[2g] .. bodies[2g + 1] - 1 hold group g's bodies, its boxes first
     int32_t *ranges;  // count * MOST_ROUNDS * 2: group g's packed entries in round r, [from, to)
+    // The phase launches' order: workgroup k takes group launch[k]. Each phase launch leaves in clocks[g] how long
+    // group g took (100 MHz ticks), from which the host orders the next step's launches, longest first.
+    int32_t *launch;
+    int64_t *clocks;
 };
 // Each group's packed entries in each of the step's rounds (starts: launch_pack_order's, sorted by group).
 void launch_group_ranges(const BoxDev &D, const Groups &G, const int32_t *starts, int used, hipStream_t s);

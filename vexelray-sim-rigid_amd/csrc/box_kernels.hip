@@ -1354,7 +1354,8 @@ __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_phase(BoxDev D, Gr
     __shared__ int from[MOST_ROUNDS], to[MOST_ROUNDS];
     __shared__ int anyLive;
     __shared__ int wallCounts[2];
-    int g = blockIdx.x;
+    int64_t began = threadIdx.x == 0 ? (int64_t)wall_clock64() : 0;
+    int g = G.launch[blockIdx.x];
     int b0 = G.bodies[2 * g], b1 = G.bodies[2 * g + 1];
     int s0 = G.scenes[2 * g], s1 = G.scenes[2 * g + 1];
     for (int r = threadIdx.x; r < used; r += blockDim.x) {
@@ -1397,6 +1398,7 @@ __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_phase(BoxDev D, Gr
     __syncthreads();
     float4 *out = D.motion + (size_t)b0 * MOTION_VECTORS;
     for (int k = threadIdx.x; k < words; k += blockDim.x) out[k] = sMotion[k];
+    if (threadIdx.x == 0) G.clocks[g] = (int64_t)wall_clock64() - began;
 }
 
 // BoxBroad.bin, BoxBroad.java:74-89, in the body's scene's grid.

@@ -85,7 +85,8 @@ struct BoxWorld {
     hipGraphExec_t graphExec = nullptr;
     int graphUpdates = 0, graphInstantiations = 0;
     // The scene groups of the group plan: consecutive scenes of at most groupBodies bodies together (a larger scene
-    // alone), listed largest first, each swept by a workgroup of groupThreads. With every group within
+    // alone), listed largest first (the phase launches reorder them, groupTimed), each swept by a workgroup of
+    // groupThreads. With every group within
     // GROUP_LOCAL_BODIES, groups of up to 4,096 bodies on workgroups of 1,024 threads keep their motion records in LDS
     // over a sweep's rounds; else groups of up to 1,024 bodies on 512 threads read them from HBM (the faster of each
     // on the 2,048-scene batch: 61.6 and 79.1 ms of rounds a step, against 64-110 for the others tried). Each group's
@@ -98,6 +99,12 @@ struct BoxWorld {
     int groupBodies = 4096, groupThreads = 1024, groupTail = INT_MAX, groupSplit = 0;
     bool groupLocal = true;  // the groups' motion records in LDS over a sweep's rounds
     bool groupPhase = true;  // with groupLocal, a whole phase of sweeps, walls too, a launch (k_group_phase)
+    // The phase launches take the groups longest first by what each took in the step before (G.clocks), not largest
+    // first by bodies: a launch holds about two groups a CU, so the order decides how evenly the CUs finish.
+    // VR_BOX_GROUP_TIMED=0 keeps the largest-first order.
+    bool groupTimed = true;
+    std::vector<int64_t> groupClock;
+    std::vector<int32_t> groupLaunch;
     int32_t *packStarts = nullptr;  // the packed order's sort: where each (round, group, class) begins
     Colouring C{};
     // The learned warm start (no counterpart in the reference): per-point data, a log of (features, targets), and
@@ -187,6 +194,15 @@ struct BoxWorld {
         }
         G.bodies = alloc<int32_t>(2 * (int64_t)G.count);
         G.scenes = alloc<int32_t>(2 * (int64_t)G.count);
+        // The first launches take the groups largest first; then each step's clocks order the next step's.
+        groupTimed = env_int("VR_BOX_GROUP_TIMED", 1, 0, 1) == 1;
+        groupLaunch.resize((size_t)G.count);
+        for (int g = 0; g < G.count; g++) groupLaunch[(size_t)g] = g;
+        groupClock.assign((size_t)G.count, 0);
+        G.launch = alloc<int32_t>(G.count);
+        G.clocks = alloc<int64_t>(G.count);
+        HIP_OK(hipMemcpy(G.launch, groupLaunch.data(), sizeof(int32_t) * G.count, hipMemcpyHostToDevice));
+        HIP_OK(hipMemset(G.clocks, 0, sizeof(int64_t) * G.count));
         HIP_OK(hipMemcpy(G.bodies, bo.data(), sizeof(int32_t) * bo.size(), hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(G.scenes, sc.data(), sizeof(int32_t) * sc.size(), hipMemcpyHostToDevice));
         size_t ldsBytes = sizeof(float4) * MOTION_VECTORS * (size_t)G.mostBodies + 2 * sizeof(int) * G.mostScenes;
@@ -922,6 +938,16 @@ struct BoxWorld {
         launch_pack_choose(D, L, touching, fellBack, stream);
     }
 
+    // The next step's phase launches take the groups longest first by the time each took in this step's last phase
+    // launch (ties in the present order). Which workgroup takes which group changes no result: groups share no body.
+    void order_groups() {
+        HIP_OK(hipMemcpy(groupClock.data(), G.clocks, sizeof(int64_t) * G.count, hipMemcpyDeviceToHost));
+        if (*std::max_element(groupClock.begin(), groupClock.end()) <= 0) return;  // no phase launch ran
+        std::stable_sort(groupLaunch.begin(), groupLaunch.end(),
+                         [&](int32_t a, int32_t b) { return groupClock[(size_t)a] > groupClock[(size_t)b]; });
+        HIP_OK(hipMemcpy(G.launch, groupLaunch.data(), sizeof(int32_t) * G.count, hipMemcpyHostToDevice));
+    }
+
     // The counts BoxStepper.step reports for the step's collide (BoxStepper.java:40, 62-90).
     struct Prepared {
         int used = 0, touching = 0, points = 0, unplaced = 0, unlisted = 0;
@@ -1061,6 +1087,7 @@ struct BoxWorld {
         float ms = 0;
         HIP_OK(hipEventElapsedTime(&ms, t0, t1));
         plan_timed(ms);
+        if (groupPhase && groupTimed && plan == PLAN_GROUP && G.count > 1) order_groups();
         rep->reserved = 0;
         lastSolve.fell_back = 0;
         if (net && grid && touching > 0) {
