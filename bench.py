@@ -63,7 +63,7 @@ STATE = ("x", "y", "z", "u", "v", "w", "lx", "ly", "lz", "qx", "qy", "qz", "qw")
 # copy of known size), over the same window as the bench line's roofline. None where not measured for that window.
 SOLVE_ROUND_TRAFFIC_BYTES = None
 GROUP_ROUNDS_TRAFFIC_BYTES = None
-GROUP_PHASE_TRAFFIC_BYTES = 13_693_875_846  # k_group_phase, plain window (profiles/traffic_box_kernels_batched_2k.md)
+GROUP_PHASE_TRAFFIC_BYTES = 13_728_569_637  # k_group_phase, plain window (profiles/traffic_box_kernels_batched_2k.md)
 WALLS_TRAFFIC_BYTES = None
 SPHERE_ROUND_TRAFFIC_BYTES = None
 

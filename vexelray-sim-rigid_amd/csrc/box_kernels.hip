@@ -1354,7 +1354,8 @@ __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_phase(BoxDev D, Gr
     __shared__ int from[MOST_ROUNDS], to[MOST_ROUNDS];
     __shared__ int anyLive;
     __shared__ int wallCounts[2];
-    int64_t began = threadIdx.x == 0 ? (int64_t)wall_clock64() : 0;
+    __shared__ long long began;  // the group's start on the 100 MHz clock, kept in LDS rather than a register
+    if (threadIdx.x == 0) began = (long long)wall_clock64();
     int g = G.launch[blockIdx.x];
     int b0 = G.bodies[2 * g], b1 = G.bodies[2 * g + 1];
     int s0 = G.scenes[2 * g], s1 = G.scenes[2 * g + 1];
@@ -1398,7 +1399,8 @@ __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_phase(BoxDev D, Gr
     __syncthreads();
     float4 *out = D.motion + (size_t)b0 * MOTION_VECTORS;
     for (int k = threadIdx.x; k < words; k += blockDim.x) out[k] = sMotion[k];
-    if (threadIdx.x == 0) G.clocks[g] = (int64_t)wall_clock64() - began;
+    if (threadIdx.x == 0)  // the group's index loaded again rather than held over the phase
+        G.clocks[G.launch[blockIdx.x]] = (int64_t)((long long)wall_clock64() - began);
 }
 
 // BoxBroad.bin, BoxBroad.java:74-89, in the body's scene's grid.
