@@ -208,8 +208,10 @@ struct Groups {
     int32_t *bodies;  // 2 a group: its first body, and one past its last
     int32_t *order;   // n: places bodies[2g] .. bodies[2g + 1] - 1 hold group g's bodies, its boxes first
     int32_t *ranges;  // count * MOST_ROUNDS * 2: group g's packed entries in round r, [from, to)
-    // The phase launches' order: workgroup k takes group launch[k]. Each phase launch leaves in clocks[g] how long
-    // group g took (100 MHz ticks), from which the host orders the next step's launches, longest first.
+    // The phase launches' order, one for the solve (and warm) phases and one for the relax phases: workgroup k of a
+    // launch of kind f (0 solve, 1 relax) takes group launch[f * count + k]. Each solve or relax launch adds to
+    // clocks[f * count + g] how long group g took (100 MHz ticks); the host orders the next step's launches of each
+    // kind by the step's sums, longest first, and clears them.
     int32_t *launch;
     int64_t *clocks;
 };

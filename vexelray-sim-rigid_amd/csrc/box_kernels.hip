@@ -1356,7 +1356,8 @@ __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_phase(BoxDev D, Gr
     __shared__ int wallCounts[2];
     __shared__ long long began;  // the group's start on the 100 MHz clock, kept in LDS rather than a register
     if (threadIdx.x == 0) began = (long long)wall_clock64();
-    int g = G.launch[blockIdx.x];
+    constexpr int kind = SWEEP == RELAX ? 1 : 0;  // which launch order and clocks (Groups::launch)
+    int g = G.launch[kind * G.count + blockIdx.x];
     int b0 = G.bodies[2 * g], b1 = G.bodies[2 * g + 1];
     int s0 = G.scenes[2 * g], s1 = G.scenes[2 * g + 1];
     for (int r = threadIdx.x; r < used; r += blockDim.x) {
@@ -1399,8 +1400,8 @@ __global__ void __launch_bounds__(GROUP_MOST_THREADS) k_group_phase(BoxDev D, Gr
     __syncthreads();
     float4 *out = D.motion + (size_t)b0 * MOTION_VECTORS;
     for (int k = threadIdx.x; k < words; k += blockDim.x) out[k] = sMotion[k];
-    if (threadIdx.x == 0)  // the group's index loaded again rather than held over the phase
-        G.clocks[G.launch[blockIdx.x]] = (int64_t)((long long)wall_clock64() - began);
+    if (SWEEP != WARM && threadIdx.x == 0)  // the group's index loaded again rather than held over the phase
+        G.clocks[kind * G.count + G.launch[kind * G.count + blockIdx.x]] += (int64_t)((long long)wall_clock64() - began);
 }
 
 // BoxBroad.bin, BoxBroad.java:74-89, in the body's scene's grid.

@@ -196,13 +196,13 @@ struct BoxWorld {
         G.scenes = alloc<int32_t>(2 * (int64_t)G.count);
         // The first launches take the groups largest first; then each step's clocks order the next step's.
         groupTimed = env_int("VR_BOX_GROUP_TIMED", 1, 0, 1) == 1;
-        groupLaunch.resize((size_t)G.count);
-        for (int g = 0; g < G.count; g++) groupLaunch[(size_t)g] = g;
-        groupClock.assign((size_t)G.count, 0);
-        G.launch = alloc<int32_t>(G.count);
-        G.clocks = alloc<int64_t>(G.count);
-        HIP_OK(hipMemcpy(G.launch, groupLaunch.data(), sizeof(int32_t) * G.count, hipMemcpyHostToDevice));
-        HIP_OK(hipMemset(G.clocks, 0, sizeof(int64_t) * G.count));
+        groupLaunch.resize(2 * (size_t)G.count);  // the solve phases' order, then the relax phases'
+        for (int g = 0; g < G.count; g++) groupLaunch[(size_t)g] = groupLaunch[(size_t)(G.count + g)] = g;
+        groupClock.assign(2 * (size_t)G.count, 0);
+        G.launch = alloc<int32_t>(2 * (int64_t)G.count);
+        G.clocks = alloc<int64_t>(2 * (int64_t)G.count);
+        HIP_OK(hipMemcpy(G.launch, groupLaunch.data(), sizeof(int32_t) * 2 * G.count, hipMemcpyHostToDevice));
+        HIP_OK(hipMemset(G.clocks, 0, sizeof(int64_t) * 2 * G.count));
         HIP_OK(hipMemcpy(G.bodies, bo.data(), sizeof(int32_t) * bo.size(), hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(G.scenes, sc.data(), sizeof(int32_t) * sc.size(), hipMemcpyHostToDevice));
         size_t ldsBytes = sizeof(float4) * MOTION_VECTORS * (size_t)G.mostBodies + 2 * sizeof(int) * G.mostScenes;
@@ -938,14 +938,20 @@ struct BoxWorld {
         launch_pack_choose(D, L, touching, fellBack, stream);
     }
 
-    // The next step's phase launches take the groups longest first by the time each took in this step's last phase
-    // launch (ties in the present order). Which workgroup takes which group changes no result: groups share no body.
+    // The next step's solve and relax launches take the groups longest first by the time each took in this step's
+    // launches of the same kind, summed (ties in the present order); the sums then start again from zero. A stopped
+    // phase's groups take very different times in the solve and the relax, hence an order for each. Which workgroup
+    // takes which group changes no result: groups share no body.
     void order_groups() {
-        HIP_OK(hipMemcpy(groupClock.data(), G.clocks, sizeof(int64_t) * G.count, hipMemcpyDeviceToHost));
-        if (*std::max_element(groupClock.begin(), groupClock.end()) <= 0) return;  // no phase launch ran
-        std::stable_sort(groupLaunch.begin(), groupLaunch.end(),
-                         [&](int32_t a, int32_t b) { return groupClock[(size_t)a] > groupClock[(size_t)b]; });
-        HIP_OK(hipMemcpy(G.launch, groupLaunch.data(), sizeof(int32_t) * G.count, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(groupClock.data(), G.clocks, sizeof(int64_t) * 2 * G.count, hipMemcpyDeviceToHost));
+        for (int f = 0; f < 2; f++) {
+            const int64_t *clock = groupClock.data() + (size_t)f * G.count;
+            if (*std::max_element(clock, clock + G.count) <= 0) continue;  // no launch of this kind ran
+            std::stable_sort(groupLaunch.begin() + (size_t)f * G.count, groupLaunch.begin() + (size_t)(f + 1) * G.count,
+                             [&](int32_t a, int32_t b) { return clock[a] > clock[b]; });
+        }
+        HIP_OK(hipMemcpy(G.launch, groupLaunch.data(), sizeof(int32_t) * 2 * G.count, hipMemcpyHostToDevice));
+        HIP_OK(hipMemsetAsync(G.clocks, 0, sizeof(int64_t) * 2 * G.count, stream));  // before the next launches
     }
 
     // The counts BoxStepper.step reports for the step's collide (BoxStepper.java:40, 62-90).
