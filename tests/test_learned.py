@@ -157,3 +157,31 @@ def test_the_learned_start_on_the_whole_batch():
     net.close()
     for b in runs:
         b.close()
+
+
+def test_the_prediction_made_for_the_carrys_empty_points_alone_changes_nothing():
+    """A learned step makes features and predictions only for the points the carry leaves at zero, the only points a
+    prediction may start; a logged step makes them for every point. The same settled batch with the same network
+    (untrained, so the same on every run) is stepped both ways: every state word, the points started from the
+    prediction and the scenes left to the carry must be the same."""
+    net = mlp.Mlp(seed=5)
+    out = []
+    for logged in (False, True):
+        spec, b = settled(41, steps=30)
+        b.learned(net)
+        b.tolerance(TOL, False)
+        if logged:
+            b.log(True, 1 << 21)
+        started = []
+        for _ in range(4):
+            b.stepAndWait()
+            started.append((int(b.learnedPoints), int(b.solveReport().fell_back)))
+        out.append(({k: b.floats(k).copy() for k in BODY}, started))
+        b.learned(None)
+        b.close()
+    (a, sa), (c, sc_) = out
+    assert sa == sc_
+    assert sum(s[0] for s in sa) > 0  # the prediction did start points
+    for k in BODY:
+        assert np.array_equal(a[k].view(np.int32), c[k].view(np.int32)), k
+    net.close()

@@ -132,6 +132,10 @@ struct Colouring {
 constexpr int LEARN_FEATURES = 16, LEARN_OUTPUTS = 3;
 
 struct Learned {
+    // With fresh set (a network set, nothing logged), features and predictions are made only for the points the
+    // carry leaves at zero, the only ones a prediction may start (launch_pack_prediction); pointBase then counts
+    // those, and pointBase[touching] holds how many there are (the network's rows). Else every point has a row.
+    int fresh;
     int32_t *pointBase;  // slots + 1
     float *x;            // 4 * slots * LEARN_FEATURES
     float *y;            // 4 * slots * LEARN_OUTPUTS: the network's prediction

@@ -1772,10 +1772,32 @@ __device__ __forceinline__ PointAt point_at(const BoxDev &D, const Bufs &B, int 
     return P;
 }
 
+// Whether point k of slot p holds nothing from the carry (its normal and tangent words all zero, -0 too): the
+// points a prediction may start (k_pack_prediction tests the same words once they are packed).
+__device__ __forceinline__ bool carried_nothing(const BoxDev &D, int p, int k) {
+    const float *held = D.impulses + (size_t)p * IMPULSE_WORDS + POINT_IMPULSE * k;
+    return !(held[0] != 0.f || held[1] != 0.f || held[2] != 0.f || held[3] != 0.f);
+}
+
+// The rows pair t's points take in the features: every point, or with L.fresh the ones the carry left at zero.
+__device__ __forceinline__ int point_rows(const BoxDev &D, const Learned &L, int p) {
+    int count = (int)D.manifold[(size_t)p * MANIFOLD_WORDS + COUNT];
+    if (!L.fresh) return count;
+    int rows = 0;
+    for (int k = 0; k < count; k++) rows += carried_nothing(D, p, k) ? 1 : 0;
+    return rows;
+}
+
 __global__ void k_point_counts(BoxDev D, Colouring C, Learned L) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= *C.touching) return;
-    L.pointBase[t] = (int)D.manifold[(size_t)C.touch[t] * MANIFOLD_WORDS + COUNT];
+    L.pointBase[t] = point_rows(D, L, C.touch[t]);
+}
+
+// pointBase[touching]: the rows in all, after the exclusive scan of the counts.
+__global__ void k_point_total(BoxDev D, Colouring C, Learned L) {
+    int last = *C.touching - 1;
+    L.pointBase[last + 1] = L.pointBase[last] + point_rows(D, L, C.touch[last]);
 }
 
 // Features: normal (3), relative velocity at the point (3), depth, the carried impulse (normal, two tangent),
@@ -1787,7 +1809,9 @@ __global__ void k_point_features(BoxDev D, Colouring C, Learned L) {
     Bufs B = bufs(D);
     int p = C.touch[t];
     int count = (int)D.manifold[(size_t)p * MANIFOLD_WORDS + COUNT];
+    int row = L.pointBase[t];
     for (int k = 0; k < count; k++) {
+        if (L.fresh && !carried_nothing(D, p, k)) continue;  // a carried point keeps its impulse: no row
         PointAt P = point_at(D, B, p, k);
         float va[3], vb[3], rel[3], t1[3], t2[3];
         P.first.pointVelocity(P.rA, va);
@@ -1797,7 +1821,7 @@ __global__ void k_point_features(BoxDev D, Colouring C, Learned L) {
         const float *held = D.impulses + (size_t)p * IMPULSE_WORDS + POINT_IMPULSE * k;
         float tangent[3] = {held[1], held[2], held[3]};
         int balls = (P.first.sphere() ? 1 : 0) + (P.second.sphere() ? 1 : 0);
-        float *f = L.x + (size_t)(L.pointBase[t] + k) * LEARN_FEATURES;
+        float *f = L.x + (size_t)(row++) * LEARN_FEATURES;
         f[0] = P.n[0];
         f[1] = P.n[1];
         f[2] = P.n[2];
@@ -1859,16 +1883,20 @@ __global__ void k_pack_prediction(BoxDev D, Colouring C, Learned L, int touching
     int t = C.order[D.packFrom[j]];
     int p = C.touch[t];
     int count = min((int)D.manifold[(size_t)p * MANIFOLD_WORDS + COUNT], MOST_POINTS);
+    int row = L.pointBase[t];
     for (int k = 0; k < count; k++) {
-        PointAt P = point_at(D, B, p, k);
-        const float *y = L.y + (size_t)(L.pointBase[t] + k) * LEARN_OUTPUTS;
-        if (!(isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2]) && y[0] >= 0 && P.first.im + P.second.im > 0))
-            continue;
         float *imp = D.packImpulse + (size_t)(k * POINT_IMPULSE) * T + j;
         // A point the carry started (it found the point's last impulse) keeps it: the carried impulses are the last
         // step's converged, balanced set, and a prediction made point by point is not. The prediction starts the
-        // points the carry has nothing for (new contacts, and points that moved past the carry's reach).
-        if (imp[0] != 0.f || imp[T] != 0.f || imp[2 * T] != 0.f || imp[3 * T] != 0.f) continue;
+        // points the carry has nothing for (new contacts, and points that moved past the carry's reach); with
+        // L.fresh only those have rows, in order.
+        bool carried = imp[0] != 0.f || imp[T] != 0.f || imp[2 * T] != 0.f || imp[3 * T] != 0.f;
+        if (L.fresh && carried) continue;
+        const float *y = L.y + (size_t)(L.fresh ? row++ : L.pointBase[t] + k) * LEARN_OUTPUTS;
+        if (carried) continue;
+        PointAt P = point_at(D, B, p, k);
+        if (!(isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2]) && y[0] >= 0 && P.first.im + P.second.im > 0))
+            continue;
         float t1[3], t2[3];
         plane_basis(P.n, t1, t2);
         float along1 = unsquash(y[1]), along2 = unsquash(y[2]);
@@ -1926,6 +1954,8 @@ void launch_point_features(const BoxDev &D, const Colouring &C, const Learned &L
     k_point_counts<<<blocks_for(touchingPairs), THREADS, 0, s>>>(D, C, L);
     check_launch("point counts");
     vr::scan_exclusive(L.pointBase, L.pointBase, touchingPairs, nullptr, C.sort.scanTmp, s);
+    k_point_total<<<1, 1, 0, s>>>(D, C, L);
+    check_launch("point total");
     k_point_features<<<blocks_for(touchingPairs), THREADS, 0, s>>>(D, C, L);
     check_launch("point features");
 }

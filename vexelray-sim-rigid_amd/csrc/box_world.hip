@@ -980,8 +980,11 @@ struct BoxWorld {
             // BoxStepper's colouring (first fit in slot order), on the device; the host reads back the counts.
             colour(&p.used, &p.touching, &p.points, &p.unplaced, &p.unlisted);
             if ((net || logging) && p.touching > 0) {
+                // A logged step takes every point's features; a learned one only the points the carry left at zero,
+                // the only ones its prediction may start, their count known on the device.
+                L.fresh = net && !logging ? 1 : 0;
                 launch_point_features(D, C, L, p.touching, stream);
-                if (net) net->forward(L.x, p.points, L.y, stream);
+                if (net) net->forward(L.x, p.points, L.y, stream, L.fresh ? L.pointBase + p.touching : nullptr);
             }
         }
         return p;

@@ -24,7 +24,8 @@ struct Net {
     Net();
     ~Net();
     void pack(hipStream_t s);       // after params change
-    void forward(const float *x, int64_t n, float *y, hipStream_t s);         // x: n x 16, y: n x 3
+    // x: n x 16, y: n x 3; with `rows` (device memory), only the first *rows of the n (at most n) are taken.
+    void forward(const float *x, int64_t n, float *y, hipStream_t s, const int32_t *rows = nullptr);
     void gradient(const float *x, const float *t, int64_t n, hipStream_t s);  // mean squared error over n points
     void adam(float rate, hipStream_t s);
 };

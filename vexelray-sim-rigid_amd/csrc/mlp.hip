@@ -119,8 +119,11 @@ __device__ __forceinline__ void hidden_layer(const unsigned short *wt, int kDim,
     }
 }
 
-__global__ void __launch_bounds__(THREADS) k_forward(const Packed *W, const float *x, int64_t n, float *y) {
+__global__ void __launch_bounds__(THREADS) k_forward(const Packed *W, const float *x, int64_t n, float *y,
+                                                     const int32_t *rows) {
     __shared__ Packed w;
+    if (rows) n = min(n, (int64_t)*rows);  // the rows actually there, known on the device only
+    if ((int64_t)blockIdx.x * (blockDim.x / 64) * 16 >= n) return;  // the whole workgroup has none
     {
         const int *src = reinterpret_cast<const int *>(W);
         int *dst = reinterpret_cast<int *>(&w);
@@ -285,11 +288,11 @@ void Net::pack(hipStream_t s) {
     check_launch("mlp pack");
 }
 
-void Net::forward(const float *x, int64_t n, float *y, hipStream_t s) {
+void Net::forward(const float *x, int64_t n, float *y, hipStream_t s, const int32_t *rows) {
     if (n <= 0) return;
     int64_t waves = (n + 15) / 16;
     int blocks = (int)std::min<int64_t>((waves + 3) / 4, 4096);
-    k_forward<<<blocks, THREADS, 0, s>>>((const Packed *)packed, x, n, y);
+    k_forward<<<blocks, THREADS, 0, s>>>((const Packed *)packed, x, n, y, rows);
     check_launch("mlp forward");
 }
 
